@@ -1,0 +1,1283 @@
+// fehip.hip -- libfehip.so: host side of the MI355X fixed-effect solver and
+// its C-ABI (include/fehip.h).
+//
+// One handle = one set of fixed effects on one GPU, with its own HIP stream.
+// Set-up (fehip_create / fehip_update_weights) builds on the device:
+//   * int32 0-based refs, range-checked;
+//   * the stable CSR bucketing of every FE (radix sort, csr_build.h);
+//   * the internal observation order (stable sort by the FE with most levels);
+//   * scale (serial per level, in the reference's summation order) and cache.
+// A solve (fehip_solve_residuals*) runs the whole LSMR loop on the device:
+// each iteration is D*v, D'u (+ fix-up) and the fused vector update; the
+// scalar recurrences run in the last workgroup of each reduction, so the host
+// only replays a captured hipGraph of several iterations and polls one flag.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fehip.h"
+#include "csr_build.h"
+#include "fehip_common.h"
+#include "linmap.h"
+
+using namespace fehip;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(FEHIP_EHIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHK(expr)                 \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != 0) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+enum FeKind { KIND_SORTED = 0, KIND_CSR = 1, KIND_DENSE = 2 };
+
+struct FeInfo {
+    int64_t n = 0;
+    int kind = KIND_CSR;
+    int32_t *ref_int = nullptr;    // internal order, 0-based
+    int32_t *operm = nullptr;      // CSR over the caller's order (bit-exact bucketing)
+    int32_t *ooff = nullptr;
+    int32_t *iperm = nullptr;      // CSR over the internal order (KIND_CSR)
+    int32_t *ioff = nullptr;
+    uint64_t *heads = nullptr;
+    uint32_t *wprefix = nullptr;
+    int32_t *seg_level = nullptr;
+    int64_t nwords = 0;
+    double *z_orig = nullptr;      // interaction, caller's order (nullptr: UnitWeights)
+    void *scale_T = nullptr;       // n, storage precision
+    double *scale_d = nullptr;     // n, widened
+    void *cache_int = nullptr;     // N, internal order
+    void *cache_csr = nullptr;     // N, CSR order (== cache_int for KIND_SORTED)
+    std::vector<int32_t> h_ioff;   // host copy of segment offsets (tile schedule)
+};
+
+struct Plan;  // per-K buffers and schedule
+
+struct fehip_solver {
+    int64_t N = 0;
+    int32_t G = 0;
+    int64_t p = 0;
+    int precision = FEHIP_FP64;
+    size_t tsize = 8;
+    int s = 0;  // FE defining the internal order
+    std::vector<int64_t> n, off;
+    hipStream_t stream = nullptr;
+    std::vector<FeInfo> fe;
+    int32_t *perm_s = nullptr;  // internal position -> caller's observation
+    double *w_orig = nullptr;   // caller's order (nullptr: UnitWeights)
+    double *sqrtw_int = nullptr;
+    int32_t *d_flag = nullptr;  // range-check flag
+    int32_t *h_flag = nullptr;  // pinned
+    std::vector<void *> allocs;
+    std::map<int, Plan *> plans;
+    // sort scratch
+    int32_t *sk[2] = {nullptr, nullptr}, *sv[2] = {nullptr, nullptr};
+    uint32_t *hist = nullptr, *sums = nullptr;
+    int64_t sort_cap = 0;
+    int poll_iters = 8;
+};
+
+template <typename P>
+static int dalloc(fehip_solver *h, P **ptr, size_t count) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(P));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FEHIP_EOOM, "hipMalloc of " + std::to_string(count * sizeof(P)) + " bytes failed: " +
+                                    hipGetErrorString(e));
+    }
+    h->allocs.push_back(q);
+    *ptr = (P *)q;
+    return 0;
+}
+
+static int dalloc_bytes(fehip_solver *h, void **ptr, size_t bytes) {
+    return dalloc<unsigned char>(h, (unsigned char **)ptr, bytes);
+}
+
+static void dfree(fehip_solver *h, void *q) {
+    if (!q) return;
+    auto it = std::find(h->allocs.begin(), h->allocs.end(), q);
+    if (it != h->allocs.end()) h->allocs.erase(it);
+    (void)hipFree(q);
+}
+
+static inline unsigned grid_for(int64_t n, int per_block) {
+    int64_t g = (n + per_block - 1) / per_block;
+    return (unsigned)std::max<int64_t>(g, 1);
+}
+
+// ---------------------------------------------------------------------------
+// device sort: stable radix sort of (key, index) -> CSR
+// ---------------------------------------------------------------------------
+static int ensure_sort_scratch(fehip_solver *h, int64_t N) {
+    if (h->sort_cap >= N) return 0;
+    for (int b = 0; b < 2; b++) {
+        dfree(h, h->sk[b]);
+        dfree(h, h->sv[b]);
+        CHK(dalloc(h, &h->sk[b], N));
+        CHK(dalloc(h, &h->sv[b], N));
+    }
+    dfree(h, h->hist);
+    dfree(h, h->sums);
+    const int64_t ntiles = (N + kSortTile - 1) / kSortTile;
+    CHK(dalloc(h, &h->hist, 256 * ntiles + 1));
+    CHK(dalloc(h, &h->sums, (256 * ntiles + kSortTile - 1) / kSortTile + 1));
+    h->sort_cap = N;
+    return 0;
+}
+
+static void free_sort_scratch(fehip_solver *h) {
+    for (int b = 0; b < 2; b++) {
+        dfree(h, h->sk[b]);
+        dfree(h, h->sv[b]);
+        h->sk[b] = h->sv[b] = nullptr;
+    }
+    dfree(h, h->hist);
+    dfree(h, h->sums);
+    h->hist = h->sums = nullptr;
+    h->sort_cap = 0;
+}
+
+static int exclusive_scan_u32(fehip_solver *h, const uint32_t *in, int64_t n, uint32_t *out, bool write_total,
+                              uint32_t *sums) {
+    const unsigned nb = grid_for(n, kSortTile);
+    hipLaunchKernelGGL(k_scan_chunk_sums, dim3(nb), dim3(kBlock), 0, h->stream, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums_single, dim3(1), dim3(kBlock), 0, h->stream, sums, (int64_t)nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, h->stream, in, n, sums, out,
+                       write_total ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// keys (0..nlev-1) -> sorted keys + stable permutation (values = source index)
+static int radix_sort(fehip_solver *h, const int32_t *keys, int64_t N, int64_t nlev, int32_t *keys_out,
+                      int32_t *vals_out) {
+    CHK(ensure_sort_scratch(h, N));
+    int bits = 0;
+    while ((int64_t(1) << bits) < nlev) bits++;
+    const int passes = std::max(1, (bits + 7) / 8);
+    const int64_t ntiles = (N + kSortTile - 1) / kSortTile;
+    const int32_t *kin = keys, *vin = nullptr;
+    for (int ps = 0; ps < passes; ps++) {
+        const bool lastp = ps == passes - 1;
+        int32_t *kout = lastp ? keys_out : h->sk[ps & 1];
+        int32_t *vout = lastp ? vals_out : h->sv[ps & 1];
+        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, kin, N, 8 * ps,
+                           h->hist, ntiles);
+        CHK(exclusive_scan_u32(h, h->hist, 256 * ntiles, h->hist, false, h->sums));
+        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, kin, vin, kout,
+                           vout, N, 8 * ps, h->hist, ntiles);
+        HIPCHK(hipGetLastError());
+        kin = kout;
+        vin = vout;
+    }
+    return 0;
+}
+
+// offsets, head bits, word prefix and segment->level map of a sorted key array
+static int csr_products(fehip_solver *h, const int32_t *skeys, int64_t N, int64_t nlev, int32_t *off,
+                        FeInfo *fi) {
+    hipLaunchKernelGGL(k_offsets_from_sorted, dim3(grid_for(N + 1, kBlock)), dim3(kBlock), 0, h->stream, skeys,
+                       N, (int32_t)nlev, off);
+    HIPCHK(hipGetLastError());
+    if (!fi) return 0;
+    fi->nwords = N / 64 + 2;
+    CHK(dalloc(h, &fi->heads, fi->nwords));
+    CHK(dalloc(h, &fi->wprefix, fi->nwords + 1));
+    CHK(dalloc(h, &fi->seg_level, nlev));
+    uint32_t *cnt = nullptr;
+    CHK(dalloc(h, &cnt, fi->nwords + 1));
+    hipLaunchKernelGGL(k_head_bits, dim3(grid_for(fi->nwords, kWavesPerBlock)), dim3(kBlock), 0, h->stream, skeys,
+                       N, fi->heads, fi->nwords, cnt);
+    HIPCHK(hipGetLastError());
+    uint32_t *sums = nullptr;
+    CHK(dalloc(h, &sums, (fi->nwords + kSortTile - 1) / kSortTile + 2));
+    CHK(exclusive_scan_u32(h, cnt, fi->nwords, fi->wprefix, true, sums));
+    hipLaunchKernelGGL(k_seg_level, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, skeys, N, fi->heads,
+                       fi->wprefix, fi->seg_level);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, cnt);
+    dfree(h, sums);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// scale! / cache!  (src/SolverCPU.jl:78-95)
+// ---------------------------------------------------------------------------
+
+// One thread per level, serial over the level's observations in ascending
+// caller order -- the reference's summation order (:81-83) -- in double,
+// rounded into T after every add exactly as the Julia promotion does.
+template <typename T>
+__global__ void k_scale(const int32_t *__restrict__ off, const int32_t *__restrict__ perm, int64_t nlev,
+                        const double *__restrict__ w, const double *__restrict__ z, T *__restrict__ scale_T,
+                        double *__restrict__ scale_d) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nlev) return;
+    T s = (T)0;
+    for (int32_t q = off[k]; q < off[k + 1]; q++) {
+        const int32_t i = perm ? perm[q] : q;
+        const double zi = z ? z[i] : 1.0;
+        const double wi = w ? w[i] : 1.0;
+        s = (T)((double)s + (zi * zi) * wi);
+    }
+    const T r = s > (T)0 ? (T)1 / (T)sqrt((double)s) : (T)0;
+    scale_T[k] = r;
+    scale_d[k] = (double)r;
+}
+
+// cache[j] = z_i * sqrt(w_i) * scale[ref]  with i = perm_s[j]  (:93)
+template <typename T>
+__global__ void k_cache(const int32_t *__restrict__ perm_s, int64_t N, const int32_t *__restrict__ ref_int,
+                        const double *__restrict__ w, const double *__restrict__ z, const T *__restrict__ scale,
+                        T *__restrict__ cache) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const int32_t i = perm_s[j];
+    const double zi = z ? z[i] : 1.0;
+    const double wi = w ? w[i] : 1.0;
+    cache[j] = (T)((zi * sqrt(wi)) * (double)scale[ref_int[j]]);
+}
+
+__global__ void k_sqrtw(const double *__restrict__ w, const int32_t *__restrict__ perm_s, int64_t N,
+                        double *__restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < N) out[j] = sqrt(w[perm_s[j]]);
+}
+
+template <typename T>
+static int compute_scale_cache(fehip_solver *h) {
+    for (int g = 0; g < h->G; g++) {
+        FeInfo &f = h->fe[g];
+        // FE s: internal order within a level is the caller's order
+        const bool is_s = g == h->s;
+        const int32_t *off = f.ooff;
+        const int32_t *perm = is_s ? nullptr : f.operm;
+        if (is_s) {
+            // weights and interaction in the internal order for the FE s sums
+            double *wi = nullptr, *zi = nullptr;
+            if (h->w_orig) {
+                CHK(dalloc(h, &wi, h->N));
+                hipLaunchKernelGGL(k_gather_perm<double>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
+                                   h->w_orig, h->perm_s, h->N, wi);
+            }
+            if (f.z_orig) {
+                CHK(dalloc(h, &zi, h->N));
+                hipLaunchKernelGGL(k_gather_perm<double>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
+                                   f.z_orig, h->perm_s, h->N, zi);
+            }
+            hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(f.n, 64)), dim3(64), 0, h->stream, off, nullptr, f.n, wi,
+                               zi, (T *)f.scale_T, f.scale_d);
+            HIPCHK(hipStreamSynchronize(h->stream));
+            dfree(h, wi);
+            dfree(h, zi);
+        } else {
+            hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(f.n, 64)), dim3(64), 0, h->stream, off, perm, f.n,
+                               h->w_orig, f.z_orig, (T *)f.scale_T, f.scale_d);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    for (int g = 0; g < h->G; g++) {
+        FeInfo &f = h->fe[g];
+        hipLaunchKernelGGL(k_cache<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, h->perm_s, h->N,
+                           f.ref_int, h->w_orig, f.z_orig, (const T *)f.scale_T, (T *)f.cache_int);
+        if (f.kind == KIND_CSR)
+            hipLaunchKernelGGL(k_gather_perm<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
+                               (const T *)f.cache_int, f.iperm, h->N, (T *)f.cache_csr);
+        HIPCHK(hipGetLastError());
+    }
+    if (h->w_orig) {
+        if (!h->sqrtw_int) CHK(dalloc(h, &h->sqrtw_int, h->N));
+        hipLaunchKernelGGL(k_sqrtw, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, h->w_orig, h->perm_s,
+                           h->N, h->sqrtw_int);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static int do_update_weights(fehip_solver *h, const double *w) {
+    if (w) {
+        for (int64_t i = 0; i < h->N; i++)
+            if (!(w[i] > 0.0) || !std::isfinite(w[i]))
+                return fail(FEHIP_EARG, "weights must be positive and finite (observation " + std::to_string(i + 1) +
+                                            ")");
+        if (!h->w_orig) CHK(dalloc(h, &h->w_orig, h->N));
+        HIPCHK(hipMemcpyAsync(h->w_orig, w, sizeof(double) * h->N, hipMemcpyHostToDevice, h->stream));
+    } else {
+        dfree(h, h->w_orig);
+        dfree(h, h->sqrtw_int);
+        h->w_orig = nullptr;
+        h->sqrtw_int = nullptr;
+    }
+    if (h->precision == FEHIP_FP64) return compute_scale_cache<double>(h);
+    return compute_scale_cache<float>(h);
+}
+
+// ---------------------------------------------------------------------------
+// create
+// ---------------------------------------------------------------------------
+static int dense_limit(int precision) {
+    // levels whose per-lane LDS accumulators fit 32 KB for one wave
+    return precision == FEHIP_FP64 ? 64 : 64;
+}
+
+template <typename R>
+static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs, const int64_t *nlevels,
+                       const double *const *interaction, const double *weights, int32_t precision) {
+    if (!out) return fail(FEHIP_EARG, "out is NULL");
+    *out = nullptr;
+    if (N < 1) return fail(FEHIP_EDIM, "N must be >= 1");
+    if (N > (int64_t)INT32_MAX - 4096) return fail(FEHIP_EDIM, "N exceeds the int32 observation index range");
+    if (G < 1 || G > kMaxG) return fail(FEHIP_EARG, "G must be in 1.." + std::to_string(kMaxG));
+    if (!refs || !nlevels) return fail(FEHIP_EARG, "refs and nlevels are required");
+    if (precision != FEHIP_FP64 && precision != FEHIP_FP32 && precision != FEHIP_MIXED)
+        return fail(FEHIP_EARG, "unknown precision");
+    for (int g = 0; g < G; g++) {
+        if (!refs[g]) return fail(FEHIP_EARG, "refs[" + std::to_string(g) + "] is NULL");
+        if (nlevels[g] < 1 || nlevels[g] > (int64_t)INT32_MAX - 2)
+            return fail(FEHIP_EDIM, "nlevels[" + std::to_string(g) + "] out of range");
+    }
+    std::unique_ptr<fehip_solver> hp(new fehip_solver());
+    fehip_solver *h = hp.get();
+    h->N = N;
+    h->G = G;
+    h->precision = precision;
+    h->tsize = precision == FEHIP_FP64 ? 8 : 4;
+    h->n.assign(nlevels, nlevels + G);
+    h->off.resize(G + 1);
+    h->off[0] = 0;
+    for (int g = 0; g < G; g++) h->off[g + 1] = h->off[g] + h->n[g];
+    h->p = h->off[G];
+    h->fe.resize(G);
+    h->s = 0;
+    for (int g = 1; g < G; g++)
+        if (h->n[g] > h->n[h->s]) h->s = g;
+    struct Cleanup {
+        fehip_solver *h;
+        bool armed = true;
+        ~Cleanup() {
+            if (!armed) return;
+            for (void *q : h->allocs) (void)hipFree(q);
+            h->allocs.clear();
+            if (h->h_flag) (void)hipHostFree(h->h_flag);
+            if (h->stream) (void)hipStreamDestroy(h->stream);
+        }
+    } cleanup{h};
+    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void **)&h->h_flag, sizeof(int32_t) * 4, hipHostMallocDefault));
+    CHK(dalloc(h, &h->d_flag, 4));
+    HIPCHK(hipMemsetAsync(h->d_flag, 0, sizeof(int32_t) * 4, h->stream));
+
+    // refs -> int32 0-based, range-checked (caller's order)
+    std::vector<int32_t *> ref_orig(G, nullptr);
+    {
+        R *tmp = nullptr;
+        CHK(dalloc(h, &tmp, N));
+        for (int g = 0; g < G; g++) {
+            CHK(dalloc(h, &ref_orig[g], N));
+            HIPCHK(hipMemcpyAsync(tmp, refs[g], sizeof(R) * N, hipMemcpyHostToDevice, h->stream));
+            hipLaunchKernelGGL(k_refs_to_i32<R>, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, tmp, N,
+                               h->n[g], ref_orig[g], h->d_flag);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(h->h_flag, h->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (h->h_flag[0])
+                return fail(FEHIP_EREF, "refs[" + std::to_string(g) + "] has a value outside 1.." +
+                                            std::to_string(h->n[g]));
+        }
+        dfree(h, tmp);
+    }
+    // FE s: its stable bucketing defines the internal order
+    {
+        FeInfo &f = h->fe[h->s];
+        f.n = h->n[h->s];
+        f.kind = KIND_SORTED;
+        CHK(dalloc(h, &f.ref_int, N));
+        CHK(dalloc(h, &f.operm, N));
+        CHK(dalloc(h, &f.ooff, f.n + 1));
+        CHK(radix_sort(h, ref_orig[h->s], N, f.n, f.ref_int, f.operm));
+        CHK(csr_products(h, f.ref_int, N, f.n, f.ooff, &f));
+        f.ioff = f.ooff;
+        h->perm_s = f.operm;
+    }
+    for (int g = 0; g < G; g++) {
+        if (g == h->s) continue;
+        FeInfo &f = h->fe[g];
+        f.n = h->n[g];
+        f.kind = f.n <= dense_limit(precision) ? KIND_DENSE : KIND_CSR;
+        CHK(dalloc(h, &f.ref_int, N));
+        hipLaunchKernelGGL(k_gather_perm<int32_t>, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
+                           ref_orig[g], h->perm_s, N, f.ref_int);
+        HIPCHK(hipGetLastError());
+        // bucketing over the caller's order (bit-exact with bucketize_refs; used by scale!)
+        CHK(dalloc(h, &f.operm, N));
+        CHK(dalloc(h, &f.ooff, f.n + 1));
+        int32_t *skeys = nullptr;
+        CHK(dalloc(h, &skeys, N));
+        CHK(radix_sort(h, ref_orig[g], N, f.n, skeys, f.operm));
+        CHK(csr_products(h, skeys, N, f.n, f.ooff, nullptr));
+        if (f.kind == KIND_CSR) {
+            // bucketing over the internal order (locality of the D'u reads)
+            CHK(dalloc(h, &f.iperm, N));
+            CHK(dalloc(h, &f.ioff, f.n + 1));
+            CHK(radix_sort(h, f.ref_int, N, f.n, skeys, f.iperm));
+            CHK(csr_products(h, skeys, N, f.n, f.ioff, &f));
+        }
+        HIPCHK(hipStreamSynchronize(h->stream));
+        dfree(h, skeys);
+    }
+    for (int g = 0; g < G; g++) dfree(h, ref_orig[g]);
+    free_sort_scratch(h);
+    // interactions (caller's order), scales and caches
+    for (int g = 0; g < G; g++) {
+        FeInfo &f = h->fe[g];
+        if (interaction && interaction[g]) {
+            CHK(dalloc(h, &f.z_orig, N));
+            HIPCHK(hipMemcpyAsync(f.z_orig, interaction[g], sizeof(double) * N, hipMemcpyHostToDevice, h->stream));
+        }
+        CHK(dalloc_bytes(h, &f.scale_T, h->tsize * f.n));
+        CHK(dalloc(h, &f.scale_d, f.n));
+        CHK(dalloc_bytes(h, &f.cache_int, h->tsize * N));
+        if (f.kind == KIND_CSR) CHK(dalloc_bytes(h, &f.cache_csr, h->tsize * N));
+        else f.cache_csr = f.cache_int;
+        if (f.kind != KIND_DENSE) {
+            f.h_ioff.resize(f.n + 1);
+            HIPCHK(hipMemcpyAsync(f.h_ioff.data(), f.ioff, sizeof(int32_t) * (f.n + 1), hipMemcpyDeviceToHost,
+                                  h->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    CHK(do_update_weights(h, weights));
+    cleanup.armed = false;
+    *out = hp.release();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// per-K plan: tile schedule, fix-up items, buffers
+// ---------------------------------------------------------------------------
+static int tile_positions(int K) { return K <= 8 ? 1024 / K : 128; }
+
+struct Plan {
+    int K = 0, TP = 0;
+    int32_t total_tiles = 0, nseg = 0, ndense = 0, dense_blocks = 0, dense_chunk = 0, nitems = 0;
+    std::vector<int> seg_fe, dense_fe;
+    std::vector<int32_t> tile0, blk0;
+    std::vector<int64_t> drow0;
+    FixItem *items = nullptr;
+    double *tileA = nullptr, *tileB = nullptr, *dpart = nullptr;
+    unsigned db_blocks = 0, seg_blocks = 0, fix_blocks = 0, upd_blocks = 0;
+    double *part_db = nullptr, *part_seg = nullptr, *part_fix = nullptr, *part_upd = nullptr;
+    unsigned int *counters = nullptr;  // [0] db, [1] fixup, [2] update
+    void *state = nullptr;
+    ColOps *ops_db = nullptr, *ops_dtx = nullptr, *ops_once = nullptr;
+    int32_t *all_done = nullptr;
+    void *u = nullptr, *r = nullptr, *x = nullptr, *v = nullptr, *hh = nullptr, *hbar = nullptr;
+    double *dY = nullptr;  // staging for host columns (N x K doubles)
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+};
+
+static int get_plan(fehip_solver *h, int K, Plan **out) {
+    auto it = h->plans.find(K);
+    if (it != h->plans.end()) {
+        *out = it->second;
+        return 0;
+    }
+    std::unique_ptr<Plan> pl(new Plan());
+    Plan *P = pl.get();
+    P->K = K;
+    P->TP = tile_positions(K);
+    const int64_t N = h->N;
+    const int64_t ntile_fe = (N + P->TP - 1) / P->TP;
+    std::vector<FixItem> items;
+    int32_t t0 = 0;
+    for (int g = 0; g < h->G; g++) {
+        const FeInfo &f = h->fe[g];
+        if (f.kind == KIND_DENSE) continue;
+        P->seg_fe.push_back(g);
+        P->tile0.push_back(t0);
+        for (int64_t k = 0; k < f.n; k++) {
+            const int64_t st = f.h_ioff[k], en = f.h_ioff[k + 1];
+            if (en <= st) continue;
+            const int64_t ts = st / P->TP, te = (en - 1) / P->TP;
+            if (ts == te) continue;
+            FixItem fi{};
+            fi.kind = 0;
+            fi.level = (int32_t)k;
+            fi.coef_off = h->off[g];
+            fi.first = t0 + ts;
+            fi.count = (int32_t)(te - ts);
+            items.push_back(fi);
+        }
+        t0 += (int32_t)ntile_fe;
+    }
+    P->nseg = (int32_t)P->seg_fe.size();
+    P->total_tiles = t0;
+    // dense FEs: chunk so that each dense FE gets ~256 chunks
+    const int Pq = kWave / K;
+    int64_t chunk = (N + 255) / 256;
+    chunk = std::max<int64_t>(chunk, 4096);
+    chunk = (chunk + Pq - 1) / Pq * Pq;
+    P->dense_chunk = (int32_t)chunk;
+    int32_t b0 = 0;
+    int64_t row0 = 0;
+    for (int g = 0; g < h->G; g++) {
+        const FeInfo &f = h->fe[g];
+        if (f.kind != KIND_DENSE) continue;
+        const int32_t nblk = (int32_t)((N + chunk - 1) / chunk);
+        P->dense_fe.push_back(g);
+        P->blk0.push_back(b0);
+        P->drow0.push_back(row0);
+        for (int64_t k = 0; k < f.n; k++) {
+            FixItem fi{};
+            fi.kind = 1;
+            fi.level = (int32_t)k;
+            fi.coef_off = h->off[g];
+            fi.first = row0 + k * nblk;
+            fi.count = nblk;
+            items.push_back(fi);
+        }
+        b0 += nblk;
+        row0 += f.n * nblk;
+    }
+    P->ndense = (int32_t)P->dense_fe.size();
+    P->dense_blocks = b0;
+    P->nitems = (int32_t)items.size();
+    P->db_blocks = (unsigned)std::min<int64_t>(grid_for((N + Pq - 1) / Pq, kWavesPerBlock), 2048);
+    P->seg_blocks = grid_for(P->total_tiles, kWavesPerBlock);
+    P->fix_blocks = grid_for(P->nitems, kWavesPerBlock);
+    P->upd_blocks = (unsigned)std::min<int64_t>(grid_for(h->p * K, kBlock), 1024);
+    const size_t ts = h->tsize;
+    CHK(dalloc(h, &P->items, std::max<int32_t>(P->nitems, 1)));
+    if (P->nitems)
+        HIPCHK(hipMemcpyAsync(P->items, items.data(), sizeof(FixItem) * P->nitems, hipMemcpyHostToDevice, h->stream));
+    CHK(dalloc(h, &P->tileA, (size_t)std::max<int32_t>(P->total_tiles, 1) * K));
+    CHK(dalloc(h, &P->tileB, (size_t)std::max<int32_t>(P->total_tiles, 1) * K));
+    CHK(dalloc(h, &P->dpart, (size_t)std::max<int64_t>(row0, 1) * K));
+    CHK(dalloc(h, &P->part_db, (size_t)P->db_blocks * K));
+    CHK(dalloc(h, &P->part_seg, (size_t)P->seg_blocks * K));
+    CHK(dalloc(h, &P->part_fix, (size_t)P->fix_blocks * K));
+    CHK(dalloc(h, &P->part_upd, (size_t)P->upd_blocks * K));
+    CHK(dalloc(h, &P->counters, 4));
+    HIPCHK(hipMemsetAsync(P->counters, 0, sizeof(unsigned) * 4, h->stream));
+    CHK(dalloc_bytes(h, &P->state, sizeof(LsmrState<double>) * K));
+    CHK(dalloc(h, &P->ops_db, 1));
+    CHK(dalloc(h, &P->ops_dtx, 1));
+    CHK(dalloc(h, &P->ops_once, 1));
+    CHK(dalloc(h, &P->all_done, 1));
+    CHK(dalloc_bytes(h, &P->u, ts * N * K));
+    CHK(dalloc_bytes(h, &P->r, ts * N * K));
+    CHK(dalloc_bytes(h, &P->x, ts * h->p * K));
+    CHK(dalloc_bytes(h, &P->v, ts * h->p * K));
+    CHK(dalloc_bytes(h, &P->hh, ts * h->p * K));
+    CHK(dalloc_bytes(h, &P->hbar, ts * h->p * K));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->plans[K] = pl.release();
+    *out = P;
+    return 0;
+}
+
+static void destroy_plan(fehip_solver *h, Plan *P) {
+    if (P->graph) (void)hipGraphExecDestroy(P->graph);
+    void *ptrs[] = {P->items, P->tileA, P->tileB, P->dpart, P->part_db, P->part_seg, P->part_fix, P->part_upd,
+                    P->counters, P->state, P->ops_db, P->ops_dtx, P->ops_once, P->all_done, P->u, P->r, P->x,
+                    P->v, P->hh, P->hbar, P->dY};
+    for (void *q : ptrs) dfree(h, q);
+    delete P;
+}
+
+// ---------------------------------------------------------------------------
+// small kernels of the solve
+// ---------------------------------------------------------------------------
+__global__ void k_set_ops(ColOps *ops, double a, double b, int K, int Kreal) {
+    const int c = threadIdx.x;
+    if (c < kMaxK) {
+        ops->a[c] = a;
+        ops->b[c] = b;
+        ops->on[c] = c < Kreal ? 1 : 0;
+    }
+}
+
+template <typename Tr>
+__global__ void k_lsmr_setup(LsmrState<Tr> *st, int K, int Kreal, double atol, double btol, double conlim,
+                             int64_t maxiter, int32_t *all_done) {
+    const int c = threadIdx.x;
+    if (c < K) {
+        LsmrState<Tr> s{};
+        s.atol = atol;
+        s.btol = btol;
+        s.ctol = conlim > 0 ? (Tr)(1.0 / conlim) : (Tr)0;
+        s.maxiter = maxiter;
+        s.active = c < Kreal ? 1 : 0;
+        s.done = c < Kreal ? 0 : 1;
+        st[c] = s;
+    }
+    if (c == 0) *all_done = 0;
+}
+
+// Y (column-major, caller's order, double) -> r (N x K, internal order, T),
+// weighted by sqrt(w) (src/AbstractFixedEffectSolver.jl:51-55)
+template <typename T>
+__global__ void k_load_cols(const double *__restrict__ Y, int64_t ldY, int K, int Kreal,
+                            const int32_t *__restrict__ perm_s, const double *__restrict__ sqrtw, int64_t N,
+                            T *__restrict__ r, T *__restrict__ u) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * K) return;
+    const int64_t j = e / K;
+    const int c = (int)(e % K);
+    T val = (T)0;
+    if (c < Kreal) {
+        val = (T)Y[(int64_t)c * ldY + perm_s[j]];
+        if (sqrtw) val = (T)((double)val * sqrtw[j]);
+    }
+    r[e] = val;
+    if (u) u[e] = val;
+}
+
+// r -> Y, unweighted (:63-66)
+template <typename T>
+__global__ void k_store_cols(double *__restrict__ Y, int64_t ldY, int K, int Kreal,
+                             const int32_t *__restrict__ perm_s, const double *__restrict__ sqrtw, int64_t N,
+                             const T *__restrict__ r) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * K) return;
+    const int64_t j = e / K;
+    const int c = (int)(e % K);
+    if (c >= Kreal) return;
+    T val = r[e];
+    if (sqrtw) val = (T)((double)val / sqrtw[j]);
+    Y[(int64_t)c * ldY + perm_s[j]] = (double)val;
+}
+
+template <typename T>
+__global__ void k_copy(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = a[e];
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ a, int64_t n, T val) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) a[e] = val;
+}
+
+// x (p doubles) -> T ; T -> double scaled (x .*= scale, :155-157)
+template <typename T>
+__global__ void k_to_T(const double *__restrict__ a, T *__restrict__ b, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = (T)a[e];
+}
+template <typename T>
+__global__ void k_to_double(const T *__restrict__ a, double *__restrict__ b, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = (double)a[e];
+}
+template <typename T>
+__global__ void k_scale_coef(const T *__restrict__ x, const T *__restrict__ scale, int64_t n, double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = (double)(x[e] * scale[e]);
+}
+
+// ---------------------------------------------------------------------------
+// engine: launches of one precision / column count
+// ---------------------------------------------------------------------------
+template <typename T, typename Acc, typename Tr, int K>
+struct Engine {
+    static constexpr int TP = K <= 8 ? 1024 / K : 128;
+
+    // target <- b*target + a*A*coef ; mode != NONE forms ||target|| per column
+    static void db(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *coef, T *target,
+                   const int32_t *early_exit) {
+        DbParams<T> p{};
+        p.N = h->N;
+        p.G = h->G;
+        for (int g = 0; g < h->G; g++) {
+            p.refs[g] = h->fe[g].ref_int;
+            p.cache[g] = (const T *)h->fe[g].cache_int;
+            p.coef_off[g] = h->off[g];
+        }
+        p.coef = coef;
+        p.target = target;
+        p.ops = ops;
+        p.partials = P->part_db;
+        p.counter = P->counters + 0;
+        p.all_done = early_exit;
+        p.mode = mode;
+        p.state = P->state;
+        p.ops_next = P->ops_dtx;
+        hipLaunchKernelGGL((k_db<T, Tr, K>), dim3(P->db_blocks), dim3(kBlock), 0, h->stream, p);
+    }
+
+    // coef <- b*coef + a*A'y : segmented tiles, few-level chunks, fix-up
+    static void dtx_seg(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
+                        const int32_t *early_exit) {
+        if (!P->nseg) return;
+        DtxSegParams<T> p{};
+        p.N = h->N;
+        p.nseg = P->nseg;
+        p.total_tiles = P->total_tiles;
+        for (int q = 0; q < P->nseg; q++) {
+            const FeInfo &f = h->fe[P->seg_fe[q]];
+            SegFE<T> &s = p.fe[q];
+            s.perm = f.kind == KIND_SORTED ? nullptr : f.iperm;
+            s.cache = (const T *)f.cache_csr;
+            s.heads = f.heads;
+            s.wprefix = f.wprefix;
+            s.seg_level = f.seg_level;
+            s.coef_off = h->off[P->seg_fe[q]];
+            s.tile0 = P->tile0[q];
+            s.ntiles = (int32_t)((h->N + P->TP - 1) / P->TP);
+        }
+        p.u = y;
+        p.v = coef;
+        p.tileA = P->tileA;
+        p.tileB = P->tileB;
+        p.ops = ops;
+        p.partials = P->part_seg;
+        p.all_done = early_exit;
+        p.mode = mode;
+        hipLaunchKernelGGL((k_dtx_seg<T, Acc, K, TP>), dim3(P->seg_blocks), dim3(kBlock), 0, h->stream, p);
+    }
+
+    static void dtx_dense(fehip_solver *h, Plan *P, const ColOps *ops, const T *y, const int32_t *early_exit) {
+        if (!P->ndense) return;
+        DtxDenseParams<T> p{};
+        p.N = h->N;
+        p.ndense = P->ndense;
+        p.total_blocks = P->dense_blocks;
+        p.chunk = P->dense_chunk;
+        int maxlev = 0;
+        for (int q = 0; q < P->ndense; q++) {
+            const FeInfo &f = h->fe[P->dense_fe[q]];
+            DenseFE<T> &d = p.fe[q];
+            d.refs = f.ref_int;
+            d.cache = (const T *)f.cache_int;
+            d.nlev = (int32_t)f.n;
+            d.blk0 = P->blk0[q];
+            d.nblk = (int32_t)((h->N + P->dense_chunk - 1) / P->dense_chunk);
+            d.coef_off = h->off[P->dense_fe[q]];
+            d.dpart_row0 = P->drow0[q];
+            maxlev = std::max<int>(maxlev, (int)f.n);
+        }
+        p.u = y;
+        p.dpart = P->dpart;
+        p.ops = ops;
+        p.all_done = early_exit;
+        const size_t lds = sizeof(Acc) * kWave * maxlev;
+        hipLaunchKernelGGL((k_dtx_dense<T, Acc, K>), dim3(P->dense_blocks), dim3(kWave), lds, h->stream, p);
+    }
+
+    static void dtx_fixup(fehip_solver *h, Plan *P, int mode, const ColOps *ops, T *coef,
+                          const int32_t *early_exit) {
+        FixParams<T> f{};
+        f.nitems = P->nitems;
+        f.items = P->items;
+        f.tileA = P->tileA;
+        f.tileB = P->tileB;
+        f.dpart = P->dpart;
+        f.v = coef;
+        f.ops = ops;
+        f.partials = P->part_fix;
+        f.main_partials = P->part_seg;
+        f.n_main = P->nseg ? P->seg_blocks : 0;
+        f.counter = P->counters + 1;
+        f.all_done = early_exit;
+        f.all_done_w = P->all_done;
+        f.mode = mode;
+        f.state = P->state;
+        f.ops_next_db = P->ops_db;
+        hipLaunchKernelGGL((k_dtx_fixup<T, Tr, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
+    }
+
+    static void dtx(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
+                    const int32_t *early_exit) {
+        dtx_seg(h, P, mode, ops, y, coef, early_exit);
+        dtx_dense(h, P, ops, y, early_exit);
+        dtx_fixup(h, P, mode, ops, coef, early_exit);
+    }
+
+    // Average device time of each kernel of one iteration over `reps`
+    // isolated launches (HIP events on the handle's stream).
+    static int time_kernels(fehip_solver *h, int reps, double *out_ms) {
+        Plan *P = nullptr;
+        CHK(get_plan(h, K, &P));
+        const int64_t NK = h->N * K, pK = h->p * K;
+        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->u, NK, (T)1);
+        void *pv[] = {P->x, P->v, P->hh, P->hbar};
+        for (void *q : pv)
+            hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(pK, kBlock)), dim3(kBlock), 0, h->stream, (T *)q, pK, (T)0.5);
+        hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, K, 0.0,
+                           0.0, 0.0, (int64_t)1 << 40, P->all_done);
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.5, K, K);
+        HIPCHK(hipGetLastError());
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        auto timed = [&](auto launch) -> double {
+            launch();  // warm-up
+            (void)hipEventRecord(e0, h->stream);
+            for (int r = 0; r < reps; r++) launch();
+            (void)hipEventRecord(e1, h->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            return (double)ms / reps;
+        };
+        out_ms[0] = timed([&] { db(h, P, MODE_NONE, P->ops_once, (const T *)P->v, (T *)P->u, nullptr); });
+        out_ms[1] = P->nseg ? timed([&] { dtx_seg(h, P, MODE_NONE, P->ops_once, (const T *)P->u, (T *)P->v, nullptr); })
+                            : 0.0;
+        out_ms[2] = P->ndense ? timed([&] { dtx_dense(h, P, P->ops_once, (const T *)P->u, nullptr); }) : 0.0;
+        out_ms[3] = timed([&] { dtx_fixup(h, P, MODE_NONE, P->ops_once, (T *)P->v, nullptr); });
+        out_ms[4] = timed([&] { update(h, P); });
+        HIPCHK(hipGetLastError());
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+
+    static void update(fehip_solver *h, Plan *P) {
+        UpdParams<T> p{};
+        p.pK = h->p * K;
+        p.v = (T *)P->v;
+        p.h = (T *)P->hh;
+        p.hbar = (T *)P->hbar;
+        p.x = (T *)P->x;
+        p.partials = P->part_upd;
+        p.counter = P->counters + 2;
+        p.all_done = P->all_done;
+        p.all_done_w = P->all_done;
+        p.state = P->state;
+        p.ops_next_db = P->ops_db;
+        hipLaunchKernelGGL((k_update<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, p);
+    }
+
+    static void iteration(fehip_solver *h, Plan *P) {
+        db(h, P, MODE_ITER, P->ops_db, (const T *)P->v, (T *)P->u, P->all_done);
+        dtx(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done);
+        update(h, P);
+    }
+
+    static int ensure_graph(fehip_solver *h, Plan *P) {
+        const int M = h->poll_iters;
+        if (P->graph && P->graph_iters == M) return 0;
+        if (P->graph) {
+            (void)hipGraphExecDestroy(P->graph);
+            P->graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < M; it++) iteration(h, P);
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (e != hipSuccess) return fail(FEHIP_EHIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(FEHIP_EHIP, std::string("graph instantiate failed: ") + hipGetErrorString(e));
+        P->graph_iters = M;
+        return 0;
+    }
+
+    // LSMR on the columns held in P->u (b) with x as the initial iterate
+    // (src/utils/lsmr.jl:46-222); per-column results in the state array.
+    static int lsmr(fehip_solver *h, Plan *P, int Kreal, double tol, int64_t maxiter) {
+        hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, Kreal,
+                           tol, tol, 1e8, maxiter, P->all_done);
+        // u = b - A x ; beta = ||u||  (:63-65)
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
+        db(h, P, MODE_INIT, P->ops_once, (const T *)P->x, (T *)P->u, nullptr);
+        // v = A'u ; alpha = ||v|| and the scalar set-up  (:66-98)
+        dtx(h, P, MODE_INIT, P->ops_dtx, (const T *)P->u, (T *)P->v, nullptr);
+        hipLaunchKernelGGL((k_init_vectors<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, h->p * K,
+                           (T *)P->v, (T *)P->hh, (T *)P->hbar, (const void *)P->state);
+        HIPCHK(hipGetLastError());
+        CHK(ensure_graph(h, P));
+        int64_t launched = 0;
+        for (;;) {
+            HIPCHK(hipMemcpyAsync(h->h_flag, P->all_done, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            if (h->h_flag[0]) break;
+            if (launched > maxiter + 2 * P->graph_iters)
+                return fail(FEHIP_EHIP, "LSMR did not reach its stopping rule within maxiter");
+            HIPCHK(hipGraphLaunch(P->graph, h->stream));
+            launched += P->graph_iters;
+        }
+        return 0;
+    }
+
+    static int read_state(fehip_solver *h, Plan *P, int Kreal, std::vector<int64_t> &iters,
+                          std::vector<int> &conv) {
+        std::vector<LsmrState<Tr>> st(K);
+        HIPCHK(hipMemcpyAsync(st.data(), P->state, sizeof(LsmrState<Tr>) * K, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        iters.resize(Kreal);
+        conv.resize(Kreal);
+        for (int c = 0; c < Kreal; c++) {
+            iters[c] = st[c].iter;
+            const int is = st[c].istop;
+            conv[c] = !(is == 3 || is == 6 || is == 7);  // src/utils/lsmr.jl:218
+        }
+        return 0;
+    }
+
+    // inner solve_residuals! (src/AbstractFixedEffectSolver.jl:49-68) on Kreal
+    // device columns
+    static int solve_residuals(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
+                               int64_t *iters, uint8_t *converged) {
+        Plan *P = nullptr;
+        CHK(get_plan(h, K, &P));
+        const int64_t NK = h->N * K;
+        // r = sqrt(w) .* y ; b = copy(r)   (:51-55)
+        hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
+                           h->perm_s, h->sqrtw_int, h->N, (T *)P->r, (T *)P->u);
+        // x = A'b  (:56)
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.0, K, Kreal);
+        dtx(h, P, MODE_NONE, P->ops_once, (const T *)P->r, (T *)P->x, nullptr);
+        HIPCHK(hipGetLastError());
+        std::vector<int64_t> it(Kreal, 0);
+        std::vector<int> cv(Kreal, 1);
+        if (h->G > 1) {  // :58-61
+            CHK(lsmr(h, P, Kreal, tol, maxiter - 1));
+            CHK(read_state(h, P, Kreal, it, cv));
+            for (int c = 0; c < Kreal; c++) it[c] += 1;
+        } else {
+            for (int c = 0; c < Kreal; c++) it[c] = 1;
+        }
+        // r = r - A x ; r ./= sqrt(w)  (:62-66)
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
+        db(h, P, MODE_NONE, P->ops_once, (const T *)P->x, (T *)P->r, nullptr);
+        hipLaunchKernelGGL(k_store_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
+                           h->perm_s, h->sqrtw_int, h->N, (const T *)P->r);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int c = 0; c < Kreal; c++) {
+            if (iters) iters[c] = it[c];
+            if (converged) converged[c] = (uint8_t)cv[c];
+        }
+        return 0;
+    }
+
+    // inner solve_coefficients! up to the collect (:147-158)
+    static int solve_coefficients(fehip_solver *h, double *dy, double tol, int64_t maxiter, double *dcoef,
+                                  int64_t *iters, uint8_t *converged) {
+        Plan *P = nullptr;
+        CHK(get_plan(h, K, &P));
+        const int64_t NK = h->N * K;
+        hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, K, 1,
+                           h->perm_s, h->sqrtw_int, h->N, (T *)P->u, (T *)nullptr);
+        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(h->p * K, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->x,
+                           h->p * K, (T)0);
+        CHK(lsmr(h, P, 1, tol, maxiter));
+        std::vector<int64_t> it;
+        std::vector<int> cv;
+        CHK(read_state(h, P, 1, it, cv));
+        for (int g = 0; g < h->G; g++)
+            hipLaunchKernelGGL(k_scale_coef<T>, dim3(grid_for(h->n[g], kBlock)), dim3(kBlock), 0, h->stream,
+                               (const T *)P->x + h->off[g], (const T *)h->fe[g].scale_T, h->n[g], dcoef + h->off[g]);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (iters) *iters = it[0] / 2;  // :159
+        if (converged) *converged = (uint8_t)cv[0];
+        return 0;
+    }
+
+    // test hooks: one mul! on K = 1 (caller's order in and out)
+    static int mul_A(fehip_solver *h, double *dy, const double *dx, double a, double b) {
+        Plan *P = nullptr;
+        CHK(get_plan(h, K, &P));
+        hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, 1, 1,
+                           h->perm_s, (const double *)nullptr, h->N, (T *)P->r, (T *)nullptr);
+        hipLaunchKernelGGL(k_to_T<T>, dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream, dx, (T *)P->x, h->p);
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, a, b, K, 1);
+        db(h, P, MODE_NONE, P->ops_once, (const T *)P->x, (T *)P->r, nullptr);
+        hipLaunchKernelGGL(k_store_cols<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, 1, 1,
+                           h->perm_s, (const double *)nullptr, h->N, (const T *)P->r);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+
+    static int mul_At(fehip_solver *h, double *dx, const double *dy, double a, double b) {
+        Plan *P = nullptr;
+        CHK(get_plan(h, K, &P));
+        hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, 1, 1,
+                           h->perm_s, (const double *)nullptr, h->N, (T *)P->r, (T *)nullptr);
+        hipLaunchKernelGGL(k_to_T<T>, dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream, (const double *)dx,
+                           (T *)P->x, h->p);
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, a, b, K, 1);
+        dtx(h, P, MODE_NONE, P->ops_once, (const T *)P->r, (T *)P->x, nullptr);
+        hipLaunchKernelGGL(k_to_double<T>, dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream, (const T *)P->x,
+                           dx, h->p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+};
+
+template <typename T, typename Acc, typename Tr, typename F>
+static int for_K(int K, F &&f) {
+    switch (K) {
+        case 1: return f(Engine<T, Acc, Tr, 1>{});
+        case 2: return f(Engine<T, Acc, Tr, 2>{});
+        case 4: return f(Engine<T, Acc, Tr, 4>{});
+        case 8: return f(Engine<T, Acc, Tr, 8>{});
+        case 16: return f(Engine<T, Acc, Tr, 16>{});
+        case 32: return f(Engine<T, Acc, Tr, 32>{});
+        default: return fail(FEHIP_EARG, "unsupported batch width");
+    }
+}
+
+// runtime dispatch over (precision, K): f receives an Engine instance
+template <typename F>
+static int dispatch(int precision, int K, F &&f) {
+    if (precision == FEHIP_FP64) return for_K<double, double, double>(K, f);
+    if (precision == FEHIP_FP32) return for_K<float, float, float>(K, f);
+    return for_K<float, F32x2, double>(K, f);
+}
+
+static int padded_K(int K) {
+    int k = 1;
+    while (k < K) k <<= 1;
+    return k;
+}
+
+static int solve_residuals_dev(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
+                               int64_t *iters, uint8_t *converged) {
+    return dispatch(h->precision, padded_K(Kreal), [&](auto e) {
+        return decltype(e)::solve_residuals(h, dY, Kreal, ldY, tol, maxiter, iters, converged);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int fehip_create(fehip_t **out, int64_t N, int32_t G, const int64_t *const *refs, const int64_t *nlevels,
+                 const double *const *interaction, const double *weights, int32_t precision) {
+    g_err.clear();
+    return create_impl<int64_t>(out, N, G, refs, nlevels, interaction, weights, precision);
+}
+
+int fehip_create_i32(fehip_t **out, int64_t N, int32_t G, const int32_t *const *refs, const int64_t *nlevels,
+                     const double *const *interaction, const double *weights, int32_t precision) {
+    g_err.clear();
+    return create_impl<int32_t>(out, N, G, refs, nlevels, interaction, weights, precision);
+}
+
+int fehip_update_weights(fehip_t *h, const double *weights) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    return do_update_weights(h, weights);
+}
+
+int fehip_solve_residuals_device(fehip_t *h, double *dY, int32_t K, int64_t ldY, double tol, int64_t maxiter,
+                                 int64_t *iters, uint8_t *converged) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (K < 1) return fail(FEHIP_EARG, "K must be >= 1");
+    if (!dY) return fail(FEHIP_EARG, "dY is NULL");
+    if (ldY < h->N) return fail(FEHIP_EDIM, "ldY < N");
+    if (maxiter < 1) return fail(FEHIP_EARG, "maxiter must be >= 1");
+    for (int32_t c0 = 0; c0 < K; c0 += kMaxK) {
+        const int kb = std::min<int32_t>(kMaxK, K - c0);
+        CHK(solve_residuals_dev(h, dY + (int64_t)c0 * ldY, kb, ldY, tol, maxiter, iters ? iters + c0 : nullptr,
+                                converged ? converged + c0 : nullptr));
+    }
+    return 0;
+}
+
+int fehip_solve_residuals(fehip_t *h, double *const *cols, int32_t K, double tol, int64_t maxiter, int64_t *iters,
+                          uint8_t *converged) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (K < 1 || !cols) return fail(FEHIP_EARG, "need K >= 1 columns");
+    if (maxiter < 1) return fail(FEHIP_EARG, "maxiter must be >= 1");
+    const int32_t kb = std::min<int32_t>(kMaxK, K);
+    double *dY = nullptr;
+    CHK(dalloc(h, &dY, (size_t)h->N * kb));
+    int rc = 0;
+    for (int32_t c0 = 0; c0 < K && rc == 0; c0 += kMaxK) {
+        const int k = std::min<int32_t>(kMaxK, K - c0);
+        for (int c = 0; c < k && rc == 0; c++) {
+            hipError_t e = hipMemcpyAsync(dY + (int64_t)c * h->N, cols[c0 + c], sizeof(double) * h->N,
+                                          hipMemcpyHostToDevice, h->stream);
+            if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+        }
+        if (rc == 0)
+            rc = solve_residuals_dev(h, dY, k, h->N, tol, maxiter, iters ? iters + c0 : nullptr,
+                                     converged ? converged + c0 : nullptr);
+        for (int c = 0; c < k && rc == 0; c++) {
+            hipError_t e = hipMemcpyAsync(cols[c0 + c], dY + (int64_t)c * h->N, sizeof(double) * h->N,
+                                          hipMemcpyDeviceToHost, h->stream);
+            if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+        }
+        if (rc == 0) {
+            hipError_t e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+        }
+    }
+    dfree(h, dY);
+    return rc;
+}
+
+int fehip_solve_coefficients(fehip_t *h, const double *y, double tol, int64_t maxiter, double *const *coef,
+                             int64_t *iters, uint8_t *converged) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (!y || !coef) return fail(FEHIP_EARG, "y and coef are required");
+    if (maxiter < 0) return fail(FEHIP_EARG, "maxiter must be >= 0");
+    double *dy = nullptr, *dc = nullptr;
+    CHK(dalloc(h, &dy, h->N));
+    CHK(dalloc(h, &dc, h->p));
+    int rc = 0;
+    {
+        hipError_t e = hipMemcpyAsync(dy, y, sizeof(double) * h->N, hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    if (rc == 0)
+        rc = dispatch(h->precision, 1, [&](auto e) {
+            return decltype(e)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged);
+        });
+    for (int g = 0; g < h->G && rc == 0; g++) {
+        hipError_t e = hipMemcpy(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    dfree(h, dy);
+    dfree(h, dc);
+    return rc;
+}
+
+int fehip_get_csr(fehip_t *h, int32_t g, int32_t *perm, int32_t *offsets) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (g < 0 || g >= h->G) return fail(FEHIP_EARG, "no such fixed effect");
+    const FeInfo &f = h->fe[g];
+    if (perm) HIPCHK(hipMemcpy(perm, f.operm, sizeof(int32_t) * h->N, hipMemcpyDeviceToHost));
+    if (offsets) HIPCHK(hipMemcpy(offsets, f.ooff, sizeof(int32_t) * (f.n + 1), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fehip_get_scale(fehip_t *h, int32_t g, double *scale) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (g < 0 || g >= h->G || !scale) return fail(FEHIP_EARG, "bad arguments");
+    HIPCHK(hipMemcpy(scale, h->fe[g].scale_d, sizeof(double) * h->fe[g].n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int fehip_get_cache(fehip_t *h, int32_t g, double *cache) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (g < 0 || g >= h->G || !cache) return fail(FEHIP_EARG, "bad arguments");
+    double *tmp = nullptr, *tmp2 = nullptr;
+    CHK(dalloc(h, &tmp, h->N));
+    CHK(dalloc(h, &tmp2, h->N));
+    if (h->precision == FEHIP_FP64)
+        hipLaunchKernelGGL(k_to_double<double>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
+                           (const double *)h->fe[g].cache_int, tmp, h->N);
+    else
+        hipLaunchKernelGGL(k_to_double<float>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
+                           (const float *)h->fe[g].cache_int, tmp, h->N);
+    hipLaunchKernelGGL(k_scatter_perm<double>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, tmp,
+                       h->perm_s, h->N, tmp2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(cache, tmp2, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, tmp);
+    dfree(h, tmp2);
+    return 0;
+}
+
+static int mul_common(fehip_t *h, double *io, const double *other, double a, double b, bool adjoint) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (!io || !other) return fail(FEHIP_EARG, "bad arguments");
+    double *dy = nullptr, *dx = nullptr;
+    CHK(dalloc(h, &dy, h->N));
+    CHK(dalloc(h, &dx, h->p));
+    const double *y_host = adjoint ? other : io;
+    double *x_host = adjoint ? io : (double *)other;
+    HIPCHK(hipMemcpy(dy, y_host, sizeof(double) * h->N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dx, x_host, sizeof(double) * h->p, hipMemcpyHostToDevice));
+    const int rc = dispatch(h->precision, 1, [&](auto e) {
+        using E = decltype(e);
+        return adjoint ? E::mul_At(h, dx, dy, a, b) : E::mul_A(h, dy, dx, a, b);
+    });
+    if (rc == 0) {
+        if (adjoint) HIPCHK(hipMemcpy(io, dx, sizeof(double) * h->p, hipMemcpyDeviceToHost));
+        else HIPCHK(hipMemcpy(io, dy, sizeof(double) * h->N, hipMemcpyDeviceToHost));
+    }
+    dfree(h, dy);
+    dfree(h, dx);
+    return rc;
+}
+
+int fehip_mul_A(fehip_t *h, double *y, const double *x, double a, double b) {
+    return mul_common(h, y, x, a, b, false);
+}
+
+int fehip_mul_At(fehip_t *h, double *x, const double *y, double a, double b) {
+    return mul_common(h, x, y, a, b, true);
+}
+
+int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms) {
+    g_err.clear();
+    if (!h || !out_ms) return fail(FEHIP_EARG, "bad arguments");
+    if (K < 1 || K > kMaxK || reps < 1) return fail(FEHIP_EARG, "K must be in 1..32 and reps >= 1");
+    return dispatch(h->precision, padded_K(K), [&](auto e) { return decltype(e)::time_kernels(h, reps, out_ms); });
+}
+
+int fehip_dims(const fehip_t *h, int64_t *N, int32_t *G, int64_t *p) {
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (N) *N = h->N;
+    if (G) *G = h->G;
+    if (p) *p = h->p;
+    return 0;
+}
+
+const char *fehip_last_error(void) { return g_err.c_str(); }
+
+void fehip_destroy(fehip_t *h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto &kv : h->plans) destroy_plan(h, kv.second);
+    h->plans.clear();
+    for (void *q : h->allocs) (void)hipFree(q);
+    h->allocs.clear();
+    if (h->h_flag) (void)hipHostFree(h->h_flag);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// fehip_common.h -- shared types and wave helpers for libfehip (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fehip {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;               // 4 waves per workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kMaxG = 16;                 // fixed effects per handle
+constexpr int kMaxK = 32;                 // columns per batched launch
+
+// What a reduction's last workgroup does with the column norms it forms.
+enum Mode : int {
+    MODE_NONE = 0,  // plain mul!: no norm, no LSMR bookkeeping
+    MODE_INIT = 1,  // LSMR set-up (src/utils/lsmr.jl:63-98)
+    MODE_ITER = 2,  // LSMR iteration (src/utils/lsmr.jl:102-215)
+};
+
+// Per-column LSMR state, resident in device memory for the whole solve.
+// Tr is the reference's scalar type Tr (src/utils/lsmr.jl:59-60): Float64 for
+// FEHIP_FP64 and FEHIP_MIXED, Float32 for FEHIP_FP32.
+template <typename Tr>
+struct LsmrState {
+    Tr alpha, beta;
+    Tr zetabar, alphabar, rho, rhobar, cbar, sbar;      // :70-75
+    Tr betadd, betad, rhodold, tautildeold, thetatilde;  // :81-85
+    Tr zeta, d;                                          // :86-87
+    Tr normA2, maxrbar, minrbar;                         // :90-92
+    Tr normb, normr, normAr, normA, condA, normx;        // :89, :95-98
+    Tr inv_beta;   // stored u is raw: u = u_raw * inv_beta (:65, :107)
+    Tr inv_alpha;  // v is normalized by the update kernel (:68, :110)
+    Tr c_hbar, c_x, c_h;  // coefficients of the pending h/hbar/x update (:138-142)
+    Tr ctol;
+    double atol, btol;
+    int64_t iter, maxiter;
+    int32_t istop, done, vskip, active;
+};
+
+// Coefficients the LSMR bookkeeping hands to the next matvec launch, per
+// column: target <- b[c]*target + a[c]*(A x or A'y); on[c] == 0 skips column c.
+struct ColOps {
+    double a[kMaxK];
+    double b[kMaxK];
+    int32_t on[kMaxK];
+};
+
+__device__ __forceinline__ uint64_t lanes_le(int l) {
+    return l >= 63 ? ~0ull : ((2ull << l) - 1ull);
+}
+__device__ __forceinline__ uint64_t lanes_lt(int l) {
+    return l <= 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Sum over the lanes of one column c (lanes c, c+K, c+2K, ...) with a fixed
+// butterfly order; every lane of the column receives the total.
+template <int K, typename A>
+__device__ __forceinline__ A column_allreduce(A v) {
+#pragma unroll
+    for (int d = K; d < kWave; d <<= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Block-wide per-column sum of a double (lane layout lane = psub*K + c).
+// Result valid in threads 0..K-1 (thread c holds column c).  Uses `red`
+// (kWavesPerBlock*K doubles of LDS).
+template <int K>
+__device__ __forceinline__ double block_column_sum(double v, double *red) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    v = column_allreduce<K>(v);
+    if (lane < K) red[wave * K + lane] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x < K) {
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; w++) s += red[w * K + threadIdx.x];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Deterministic sum of nparts per-column partials laid out [part*K + c]
+// (several arrays concatenated through two spans).  Every thread of the block
+// takes part; thread c < K returns column c's total.  Fixed order: thread t
+// sums parts t/K, t/K + 256/K, ... of column t%K, then a fixed tree.
+template <int K>
+__device__ double block_sum_partials(const double *p0, int64_t n0, const double *p1, int64_t n1,
+                                     double *red /* kBlock doubles */) {
+    constexpr int R = kBlock / K;  // threads per column
+    const int t = threadIdx.x, c = t % K, r = t / K;
+    double s = 0.0;
+    for (int64_t j = r; j < n0; j += R) s += p0[j * K + c];
+    for (int64_t j = r; j < n1; j += R) s += p1[j * K + c];
+    red[t] = s;
+    __syncthreads();
+    for (int w = R / 2; w >= 1; w >>= 1) {
+        if (r < w) red[t] += red[t + w * K];
+        __syncthreads();
+    }
+    double out = (t < K) ? red[t] : 0.0;
+    __syncthreads();
+    return out;
+}
+
+// Last-workgroup election: every block publishes its partials, then bumps a
+// device-scope counter; the block that sees count == gridDim.x - 1 is last and
+// may read every partial.  The counter is re-armed by the last block, so the
+// pattern replays inside a captured graph without a memset.
+__device__ __forceinline__ bool elect_last_block(unsigned int *counter, int *flag_lds) {
+    __threadfence();  // release this block's partials (device scope)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int prev = atomicAdd(counter, 1u);
+        *flag_lds = (prev == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    bool last = *flag_lds != 0;
+    if (last) {
+        __threadfence();  // acquire the other blocks' partials
+        if (threadIdx.x == 0) atomicExch(counter, 0u);
+    }
+    return last;
+}
+
+}  // namespace fehip

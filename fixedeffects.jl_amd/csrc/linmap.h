@@ -1,0 +1,645 @@
+// linmap.h -- the fixed-effect linear map A (N x p) and its adjoint on gfx950,
+// batched over K columns, plus the fused LSMR vector update.
+//
+// Layout in HBM (internal observation order = observations sorted stably by
+// the level of the FE with most levels, "FE s"):
+//   u, r      : N x K, observation-major (u[i*K + c]); one ref/cache load
+//               serves all K columns.
+//   x,v,h,hbar: p x K, level-major (x[(off_g + k)*K + c]).
+//   refs_g    : int32, 0-based, internal order.   cache_g: T, internal order.
+//   CSR of FE g != s: perm (CSR position -> internal observation) and the
+//               cache in CSR order; FE s needs neither (its levels are
+//               contiguous runs of the internal order).
+//
+// Wave lane layout for K columns: lane = psub*K + c, P = 64/K positions per
+// wave step.  Reductions are formed in a fixed order (no float atomics), so
+// results are bitwise reproducible run to run.
+#pragma once
+
+#include "fehip_common.h"
+
+namespace fehip {
+
+// ---------------------------------------------------------------------------
+// Accumulator types.  FP64 and FP32 modes accumulate in the storage type, as
+// the reference's gather!/scatter! do (src/SolverCPU.jl:21-34).  The mixed
+// mode accumulates FP32 products with a compensated (two-sum) pair.
+// ---------------------------------------------------------------------------
+struct F32x2 {
+    float hi, lo;
+    __device__ F32x2() : hi(0.f), lo(0.f) {}
+    __device__ F32x2(float x) : hi(x), lo(0.f) {}
+    __device__ F32x2(float h, float l) : hi(h), lo(l) {}
+    __device__ explicit operator double() const { return (double)hi + (double)lo; }
+    __device__ explicit operator float() const { return hi + lo; }
+};
+// Knuth two-sum of the high parts, low parts summed, then renormalized.
+__device__ __forceinline__ F32x2 operator+(F32x2 a, F32x2 b) {
+#pragma clang fp contract(off)
+    const float s = a.hi + b.hi;
+    const float bp = s - a.hi;
+    const float e = (a.hi - (s - bp)) + (b.hi - bp);
+    const float lo = (a.lo + b.lo) + e;
+    const float h = s + lo;
+    return F32x2(h, lo - (h - s));
+}
+__device__ __forceinline__ F32x2 &operator+=(F32x2 &a, F32x2 b) { return a = a + b; }
+__device__ __forceinline__ F32x2 __shfl_up(F32x2 v, unsigned d, int w) {
+    return F32x2(::__shfl_up(v.hi, d, w), ::__shfl_up(v.lo, d, w));
+}
+__device__ __forceinline__ F32x2 __shfl_xor(F32x2 v, int m, int w) {
+    return F32x2(::__shfl_xor(v.hi, m, w), ::__shfl_xor(v.lo, m, w));
+}
+__device__ __forceinline__ F32x2 __shfl(F32x2 v, int l, int w) {
+    return F32x2(::__shfl(v.hi, l, w), ::__shfl(v.lo, l, w));
+}
+using ::__shfl;
+using ::__shfl_up;
+using ::__shfl_xor;
+
+template <typename Acc, typename T>
+__device__ __forceinline__ Acc mul_acc(T a, T b) {
+    return Acc(a * b);
+}
+// exact product of two floats as a pair (fma gives the rounding error)
+template <>
+__device__ __forceinline__ F32x2 mul_acc<F32x2, float>(float a, float b) {
+    const float p = a * b;
+    return F32x2(p, __builtin_fmaf(a, b, -p));
+}
+
+template <typename Acc>
+__device__ __forceinline__ double acc_to_double(Acc a) {
+    return (double)a;
+}
+template <typename T, typename Acc>
+__device__ __forceinline__ T acc_to_T(Acc a) {
+    return (T)a;
+}
+
+// Segmented inclusive scan over the positions of one wave step.  rs is the
+// psub of the latest segment head at or before this lane (0 if none); lanes
+// only add values of their own run.
+template <int K, typename A>
+__device__ __forceinline__ A seg_scan(A v, int psub, int rs) {
+    constexpr int P = kWave / K;
+#pragma unroll
+    for (int d = 1; d < P; d <<= 1) {
+        A up = __shfl_up(v, (unsigned)(d * K), kWave);
+        if (psub - d >= rs) v += up;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// LSMR scalar bookkeeping (src/utils/lsmr.jl), run by one thread per column
+// in the last workgroup of each reduction.  Contraction is disabled so the
+// scalar recurrences round like the reference's.
+// ---------------------------------------------------------------------------
+template <typename Tr>
+__device__ __forceinline__ Tr big_minrbar() { return (Tr)1e100; }
+template <>
+__device__ __forceinline__ float big_minrbar<float>() { return __builtin_inff(); }  // Float32(1e100) = Inf32
+
+// after u = A v - alpha u (or u = b - A x at set-up): beta = ||u||  (:64-65, :105-107)
+template <typename Tr>
+__device__ void fin_after_db(LsmrState<Tr> &s, double usq, int mode, int c, ColOps *nx) {
+#pragma clang fp contract(off)
+    if (s.done) {
+        nx->on[c] = 0;
+        return;
+    }
+    const Tr beta = (Tr)sqrt(usq);
+    s.beta = beta;
+    if (mode == MODE_INIT) {
+        s.normb = beta;
+        s.inv_beta = beta > (Tr)0 ? (Tr)1 / beta : (Tr)1;
+        nx->a[c] = (double)s.inv_beta;  // v = A'u (:66)
+        nx->b[c] = 0.0;
+        nx->on[c] = 1;
+    } else {
+        s.iter += 1;
+        if (beta > (Tr)0) {
+            s.inv_beta = (Tr)1 / beta;
+            s.vskip = 0;
+            nx->a[c] = (double)s.inv_beta;  // v = A'u - beta v (:108)
+            nx->b[c] = (double)(-beta);
+            nx->on[c] = 1;
+        } else {
+            s.inv_beta = (Tr)1;
+            s.vskip = 1;  // the reference skips the v update (:106-111)
+            nx->on[c] = 0;
+        }
+    }
+}
+
+// after v = A'u - beta v: alpha = ||v|| and the plane rotations (:67-98 at
+// set-up; :109-182 per iteration).  Leaves the h/hbar/x coefficients.
+template <typename Tr>
+__device__ void fin_after_dtx(LsmrState<Tr> &s, double vsq, int mode, int c, ColOps *nx_db) {
+#pragma clang fp contract(off)
+    if (s.done) {
+        nx_db->on[c] = 0;
+        return;
+    }
+    if (mode == MODE_INIT) {
+        const Tr alpha = (Tr)sqrt(vsq);
+        s.alpha = alpha;
+        s.inv_alpha = alpha > (Tr)0 ? (Tr)1 / alpha : (Tr)1;
+        s.zetabar = alpha * s.beta;
+        s.alphabar = alpha;
+        s.rho = 1; s.rhobar = 1; s.cbar = 1; s.sbar = 0;
+        s.betadd = s.beta; s.betad = 0; s.rhodold = 1; s.tautildeold = 0; s.thetatilde = 0;
+        s.zeta = 0; s.d = 0;
+        s.normA = -1; s.condA = -1; s.normx = -1;
+        s.normA2 = alpha * alpha;
+        s.maxrbar = 0;
+        s.minrbar = big_minrbar<Tr>();
+        s.normb = s.beta;
+        s.istop = 0;
+        s.normr = s.beta;
+        s.normAr = alpha * s.beta;
+        s.iter = 0;
+        s.vskip = 0;
+        if (s.normAr == (Tr)0 || s.maxiter <= 0) s.done = 1;  // :101-102
+        nx_db->a[c] = 1.0;
+        nx_db->b[c] = (double)(-(alpha * s.inv_beta));
+        nx_db->on[c] = s.done ? 0 : 1;
+        return;
+    }
+    if (!s.vskip) {
+        const Tr alpha = (Tr)sqrt(vsq);
+        s.alpha = alpha;
+        s.inv_alpha = alpha > (Tr)0 ? (Tr)1 / alpha : (Tr)1;
+    } else {
+        s.inv_alpha = (Tr)1;
+    }
+    const Tr alpha = s.alpha, beta = s.beta, lambda = (Tr)0;
+    // Qhat_{k,2k+1} (:114-116)
+    const Tr alphahat = (Tr)sqrt((double)(s.alphabar * s.alphabar + lambda * lambda));
+    const Tr chat = s.alphabar / alphahat;
+    const Tr shat = lambda / alphahat;
+    // Q_i (:119-124)
+    const Tr rhoold = s.rho;
+    s.rho = (Tr)sqrt((double)(alphahat * alphahat + beta * beta));
+    const Tr cc = alphahat / s.rho;
+    const Tr ss = beta / s.rho;
+    const Tr thetanew = ss * alpha;
+    s.alphabar = cc * alpha;
+    // Qbar_i (:127-135)
+    const Tr rhobarold = s.rhobar;
+    const Tr zetaold = s.zeta;
+    const Tr thetabar = s.sbar * s.rho;
+    const Tr rhotemp = s.cbar * s.rho;
+    s.rhobar = (Tr)sqrt((double)((s.cbar * s.rho) * (s.cbar * s.rho) + thetanew * thetanew));
+    s.cbar = (s.cbar * s.rho) / s.rhobar;
+    s.sbar = thetanew / s.rhobar;
+    s.zeta = s.cbar * s.zetabar;
+    s.zetabar = (-s.sbar) * s.zetabar;
+    // h, hbar, x coefficients (:138-142)
+    s.c_hbar = ((-thetabar) * s.rho) / (rhoold * rhobarold);
+    s.c_x = s.zeta / (s.rho * s.rhobar);
+    s.c_h = (-thetanew) / s.rho;
+    // ||r|| estimate (:151-170)
+    const Tr betaacute = chat * s.betadd;
+    const Tr betacheck = (-shat) * s.betadd;
+    const Tr betahat = cc * betaacute;
+    s.betadd = (-ss) * betaacute;
+    const Tr thetatildeold = s.thetatilde;
+    const Tr rhotildeold = (Tr)sqrt((double)(s.rhodold * s.rhodold + thetabar * thetabar));
+    const Tr ctildeold = s.rhodold / rhotildeold;
+    const Tr stildeold = thetabar / rhotildeold;
+    s.thetatilde = stildeold * s.rhobar;
+    s.rhodold = ctildeold * s.rhobar;
+    s.betad = (-stildeold) * s.betad + ctildeold * betahat;
+    s.tautildeold = (zetaold - thetatildeold * s.tautildeold) / rhotildeold;
+    const Tr taud = (s.zeta - s.thetatilde * s.tautildeold) / s.rhodold;
+    s.d = s.d + betacheck * betacheck;
+    s.normr = (Tr)sqrt((double)((s.d + (s.betad - taud) * (s.betad - taud)) + s.betadd * s.betadd));
+    // ||A|| (:173-175)
+    s.normA2 = s.normA2 + beta * beta;
+    s.normA = (Tr)sqrt((double)s.normA2);
+    s.normA2 = s.normA2 + alpha * alpha;
+    // cond(A) (:178-182)
+    s.maxrbar = s.maxrbar > rhobarold ? s.maxrbar : rhobarold;
+    if (s.iter > 1) s.minrbar = s.minrbar < rhobarold ? s.minrbar : rhobarold;
+    s.condA = (s.maxrbar > rhotemp ? s.maxrbar : rhotemp) / (s.minrbar < rhotemp ? s.minrbar : rhotemp);
+    s.normAr = s.zetabar < (Tr)0 ? -s.zetabar : s.zetabar;  // :191
+}
+
+// after the h/hbar/x update: ||x|| and the stopping rules (:192-215).
+template <typename Tr>
+__device__ void fin_after_update(LsmrState<Tr> &s, double xsq, int c, ColOps *nx_db) {
+#pragma clang fp contract(off)
+    if (s.done) {
+        nx_db->on[c] = 0;
+        return;
+    }
+    s.normx = (Tr)sqrt(xsq);
+    const Tr test1 = s.normr / s.normb;
+    const Tr test2 = s.normAr / (s.normA * s.normr);
+    const Tr test3 = (Tr)1 / s.condA;
+    const Tr t1 = test1 / ((Tr)1 + s.normA * s.normx / s.normb);
+    const double rtol = s.btol + ((s.atol * (double)s.normA) * (double)s.normx) / (double)s.normb;
+    int istop = 0;
+    if (s.iter >= s.maxiter) istop = 7;
+    else if ((Tr)1 + test3 <= (Tr)1) istop = 6;
+    else if ((Tr)1 + test2 <= (Tr)1) istop = 5;
+    else if ((Tr)1 + t1 <= (Tr)1) istop = 4;
+    else if (test3 <= s.ctol) istop = 3;
+    else if ((double)test2 <= s.atol) istop = 2;
+    else if ((double)test1 <= rtol) istop = 1;
+    if (istop) {
+        s.istop = istop;
+        s.done = 1;
+        nx_db->on[c] = 0;
+    } else {
+        nx_db->a[c] = 1.0;  // u = A v - alpha u (:104)
+        nx_db->b[c] = (double)(-(s.alpha * s.inv_beta));
+        nx_db->on[c] = 1;
+    }
+}
+
+// all columns done -> *all_done = 1 (threads 0..K-1 hold their column's flag)
+template <int K, typename Tr>
+__device__ __forceinline__ void publish_all_done(const LsmrState<Tr> *st, int32_t *all_done) {
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int c = 0; c < K; c++) all &= st[c].done ? 1 : 0;
+        *all_done = all;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// D*x: target[i,c] <- b[c]*target[i,c] + a[c] * sum_g cache_g[i]*coef_g[ref_g[i], c]
+// (reference scatter!/mul!, src/SolverCPU.jl:29-34, src/AbstractFixedEffectLinearMap.jl:37-44)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DbParams {
+    int64_t N;
+    int32_t G;
+    const int32_t *refs[kMaxG];
+    const T *cache[kMaxG];
+    int64_t coef_off[kMaxG];
+    const T *coef;  // p x K
+    T *target;      // N x K
+    const ColOps *ops;
+    double *partials;  // gridDim.x x K
+    unsigned int *counter;
+    const int32_t *all_done;  // early exit when set (iterating launches)
+    int32_t *all_done_w;
+    int mode;
+    void *state;      // LsmrState<Tr>[K]
+    ColOps *ops_next; // written by the finalize (for the following D'x)
+};
+
+template <typename T, typename Tr, int K>
+__global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
+    if (p.all_done && *p.all_done) return;
+    constexpr int P = kWave / K;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int psub = lane / K, c = lane % K;
+    const int on = p.ops->on[c];
+    const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
+    const bool keep = p.ops->b[c] != 0.0;
+    const int64_t nrounds = (p.N + P - 1) / P;
+    const int64_t gw = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
+    double sq = 0.0;
+    if (on) {
+        for (int64_t rr = gw; rr < nrounds; rr += nw) {
+            const int64_t i = rr * P + psub;
+            if (i < p.N) {
+                T acc = (T)0;
+                for (int g = 0; g < p.G; g++) {
+                    const int32_t r = __builtin_nontemporal_load(p.refs[g] + i);
+                    const T cv = __builtin_nontemporal_load(p.cache[g] + i);
+                    acc += cv * p.coef[(p.coef_off[g] + r) * K + c];
+                }
+                T *tp = p.target + i * K + c;
+                T out = a * acc;
+                if (keep) out = b * (*tp) + out;
+                *tp = out;
+                sq += (double)out * (double)out;
+            }
+        }
+    }
+    if (p.mode == MODE_NONE) return;
+    __shared__ double red[kBlock];
+    __shared__ int lastflag;
+    const double s = block_column_sum<K>(sq, red);
+    if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+    if (!elect_last_block(p.counter, &lastflag)) return;
+    const double tot = block_sum_partials<K>(p.partials, gridDim.x, nullptr, 0, red);
+    if (threadIdx.x < K) {
+        LsmrState<Tr> *st = (LsmrState<Tr> *)p.state;
+        fin_after_db<Tr>(st[threadIdx.x], tot, p.mode, threadIdx.x, p.ops_next);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// D'y: coef[k,c] <- b[c]*coef[k,c] + a[c] * sum_{i in level k} cache[i]*y[i,c]
+// (reference gather!/mul!', src/SolverCPU.jl:21-27, src/AbstractFixedEffectLinearMap.jl:26-35)
+//
+// Segmented FEs (FE s and CSR FEs) are cut into tiles of TP CSR positions,
+// one wave per tile.  A segment that lies inside a tile is reduced and
+// written by that wave.  A segment crossing tile boundaries leaves a partial
+// in its first tile (tileB) and in each later tile it covers (tileA); the
+// fix-up launch combines them in tile order.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SegFE {
+    const int32_t *perm;      // CSR position -> internal observation; nullptr = identity
+    const T *cache;           // cache in CSR order
+    const uint64_t *heads;    // segment head bits, 1 beyond N
+    const uint32_t *wprefix;  // heads before each 64-position word
+    const int32_t *seg_level; // ordinal of non-empty segment -> level
+    int64_t coef_off;
+    int32_t tile0, ntiles;
+};
+
+template <typename T>
+struct DtxSegParams {
+    int64_t N;
+    int32_t nseg, total_tiles;
+    SegFE<T> fe[kMaxG];
+    const T *u;
+    T *v;
+    double *tileA, *tileB;  // [tile*K + c]
+    const ColOps *ops;
+    double *partials;  // gridDim.x x K
+    const int32_t *all_done;
+    int mode;
+};
+
+template <typename T, typename Acc, int K, int TP>
+__global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
+    if (p.all_done && *p.all_done) return;
+    constexpr int P = kWave / K;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int psub = lane / K, c = lane % K;
+    const int t = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    const int on = p.ops->on[c];
+    const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
+    const bool keep = p.ops->b[c] != 0.0;
+    double sq = 0.0;
+    if (t < p.total_tiles) {
+        int f = 0;
+        while (f + 1 < p.nseg && t >= p.fe[f + 1].tile0) f++;
+        const SegFE<T> fe = p.fe[f];
+        const int64_t start = (int64_t)(t - fe.tile0) * TP;
+        const int64_t end = start + TP < p.N ? start + TP : p.N;
+        T *v = p.v + fe.coef_off * K;
+        Acc carry = Acc(0.f);
+        bool cont = true;    // no segment head seen yet in this tile
+        bool open = false;   // a run is still open after the last step
+        for (int64_t base = start; base < end; base += P) {
+            const int64_t pos = base + psub;
+            const bool valid = pos < end;
+            Acc val = Acc(0.f);
+            if (valid && on) {
+                const int64_t i = fe.perm ? (int64_t)__builtin_nontemporal_load(fe.perm + pos) : pos;
+                const T cv = __builtin_nontemporal_load(fe.cache + pos);
+                val = mul_acc<Acc, T>(cv, p.u[i * K + c]);
+            }
+            const uint64_t w = fe.heads[base >> 6];
+            const int sh = (int)(base & 63);
+            const uint64_t hb = (P == 64) ? w : ((w >> sh) & ((1ull << (P & 63)) - 1ull));
+            const uint64_t nb = (sh + P < 64) ? ((w >> (sh + P)) & 1ull) : (fe.heads[(base >> 6) + 1] & 1ull);
+            const uint64_t hle = hb & lanes_le(psub);
+            const int rs = hle ? (63 - __clzll(hle)) : 0;
+            if (psub == 0 && !(hb & 1ull)) val += carry;
+            val = seg_scan<K>(val, psub, rs);
+            const bool isend = (psub == P - 1) ? (nb != 0) : (((hb >> (psub + 1)) & 1ull) != 0);
+            const bool mycont = cont && (hle == 0);
+            if (valid && on && isend) {
+                if (mycont) {
+                    p.tileA[(int64_t)t * K + c] = acc_to_double(val);
+                } else {
+                    const uint32_t ord = fe.wprefix[pos >> 6] +
+                                         (uint32_t)__popcll(w & lanes_le((int)(pos & 63))) - 1u;
+                    const int32_t lev = fe.seg_level[ord];
+                    T *dst = v + (int64_t)lev * K + c;
+                    T out = a * acc_to_T<T, Acc>(val);
+                    if (keep) out = b * (*dst) + out;
+                    *dst = out;
+                    sq += (double)out * (double)out;
+                }
+            }
+            const Acc last = __shfl(val, (P - 1) * K + c, kWave);
+            open = (nb == 0);
+            carry = open ? last : Acc(0.f);
+            cont = cont && (hb == 0);
+        }
+        if (open && on && psub == 0) {
+            if (cont) p.tileA[(int64_t)t * K + c] = acc_to_double(carry);
+            else p.tileB[(int64_t)t * K + c] = acc_to_double(carry);
+        }
+    }
+    if (p.mode == MODE_NONE) return;
+    __shared__ double red[kBlock];
+    const double s = block_column_sum<K>(sq, red);
+    if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+}
+
+// FEs with few levels: one wave per chunk of observations (internal order,
+// fully sequential reads); every lane owns a private LDS accumulator column
+// per level, so adds never collide.  Per-chunk level sums go to dpart,
+// level-major, and are combined by the fix-up launch.
+template <typename T>
+struct DenseFE {
+    const int32_t *refs;
+    const T *cache;
+    int32_t nlev, blk0, nblk, pad;
+    int64_t coef_off;
+    int64_t dpart_row0;  // first row (of K doubles) of this FE in dpart
+};
+
+template <typename T>
+struct DtxDenseParams {
+    int64_t N;
+    int32_t ndense, total_blocks, chunk, pad;
+    DenseFE<T> fe[kMaxG];
+    const T *u;
+    double *dpart;
+    const ColOps *ops;
+    const int32_t *all_done;
+};
+
+template <typename T, typename Acc, int K>
+__global__ __launch_bounds__(kWave) void k_dtx_dense(DtxDenseParams<T> p) {
+    if (p.all_done && *p.all_done) return;
+    extern __shared__ __align__(16) unsigned char dense_smem[];
+    Acc *acc = reinterpret_cast<Acc *>(dense_smem);
+    constexpr int P = kWave / K;
+    const int lane = threadIdx.x;
+    const int psub = lane / K, c = lane % K;
+    int f = 0;
+    while (f + 1 < p.ndense && (int)blockIdx.x >= p.fe[f + 1].blk0) f++;
+    const DenseFE<T> fe = p.fe[f];
+    const int bl = blockIdx.x - fe.blk0;
+    for (int k = 0; k < fe.nlev; k++) acc[k * kWave + lane] = Acc(0.f);
+    const int on = p.ops->on[c];
+    const int64_t start = (int64_t)bl * p.chunk;
+    const int64_t end = start + p.chunk < p.N ? start + p.chunk : p.N;
+    if (on) {
+        for (int64_t base = start; base < end; base += P) {
+            const int64_t pos = base + psub;
+            if (pos < end) {
+                const int32_t r = __builtin_nontemporal_load(fe.refs + pos);
+                const T cv = __builtin_nontemporal_load(fe.cache + pos);
+                acc[r * kWave + lane] += mul_acc<Acc, T>(cv, p.u[pos * K + c]);
+            }
+        }
+    }
+    __syncthreads();
+    double *out = p.dpart + fe.dpart_row0 * K;
+    for (int k = 0; k < fe.nlev; k++) {
+        Acc x = acc[k * kWave + lane];
+        x = column_allreduce<K>(x);
+        if (psub == 0) out[((int64_t)k * fe.nblk + bl) * K + c] = acc_to_double(x);
+    }
+}
+
+// Fix-up items: kind 0 = a segment crossing tiles (total = tileB[first] +
+// sum_{q=1..count} tileA[first+q]); kind 1 = a dense level (total = sum of
+// count dpart rows from row first).  One wave per item, fixed-order sums.
+struct FixItem {
+    int32_t kind, level;
+    int64_t coef_off;
+    int64_t first;
+    int32_t count, pad;
+};
+
+template <typename T>
+struct FixParams {
+    int32_t nitems, pad;
+    const FixItem *items;
+    const double *tileA, *tileB, *dpart;
+    T *v;
+    const ColOps *ops;
+    double *partials;             // gridDim.x x K
+    const double *main_partials;  // from k_dtx_seg
+    int64_t n_main;
+    unsigned int *counter;
+    const int32_t *all_done;
+    int32_t *all_done_w;
+    int mode;
+    void *state;
+    ColOps *ops_next_db;
+};
+
+template <typename T, typename Tr, int K>
+__global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
+    if (p.all_done && *p.all_done) return;
+    constexpr int P = kWave / K;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int psub = lane / K, c = lane % K;
+    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    const int on = p.ops->on[c];
+    const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
+    const bool keep = p.ops->b[c] != 0.0;
+    double sq = 0.0;
+    if (item < p.nitems) {
+        const FixItem it = p.items[item];
+        double s = 0.0;
+        if (on) {
+            if (it.kind == 0) {
+                for (int q = 1 + psub; q <= it.count; q += P) s += p.tileA[(it.first + q) * K + c];
+            } else {
+                for (int q = psub; q < it.count; q += P) s += p.dpart[(it.first + q) * K + c];
+            }
+        }
+        s = column_allreduce<K>(s);
+        if (on && psub == 0) {
+            if (it.kind == 0) s = p.tileB[it.first * K + c] + s;
+            T *dst = p.v + (it.coef_off + it.level) * K + c;
+            T out = a * (T)s;
+            if (keep) out = b * (*dst) + out;
+            *dst = out;
+            sq = (double)out * (double)out;
+        }
+    }
+    if (p.mode == MODE_NONE) return;
+    __shared__ double red[kBlock];
+    __shared__ int lastflag;
+    const double s = block_column_sum<K>(sq, red);
+    if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+    if (!elect_last_block(p.counter, &lastflag)) return;
+    const double tot = block_sum_partials<K>(p.main_partials, p.n_main, p.partials, gridDim.x, red);
+    LsmrState<Tr> *st = (LsmrState<Tr> *)p.state;
+    if (threadIdx.x < K) fin_after_dtx<Tr>(st[threadIdx.x], tot, p.mode, threadIdx.x, p.ops_next_db);
+    __syncthreads();
+    if (p.mode == MODE_INIT) publish_all_done<K, Tr>(st, p.all_done_w);
+}
+
+// ---------------------------------------------------------------------------
+// Fused LSMR vector update (src/utils/lsmr.jl:110, :138-142, :192):
+//   v <- v/alpha ; hbar <- c_hbar*hbar + h ; x <- x + c_x*hbar ;
+//   h <- c_h*h + v ; ||x||^2 partials -> stopping rules in the last block.
+// Replaces rmul!/axpy!/norm on FixedEffectCoefficients
+// (src/FixedEffectCoefficients.jl:21-55).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct UpdParams {
+    int64_t pK;
+    T *v, *h, *hbar, *x;
+    double *partials;
+    unsigned int *counter;
+    const int32_t *all_done;
+    int32_t *all_done_w;
+    void *state;
+    ColOps *ops_next_db;
+};
+
+template <typename T, typename Tr, int K>
+__global__ __launch_bounds__(kBlock) void k_update(UpdParams<T> p) {
+    if (*p.all_done) return;
+    LsmrState<Tr> *st = (LsmrState<Tr> *)p.state;
+    const int c = threadIdx.x % K;  // stride is a multiple of K
+    const bool act = !st[c].done;
+    const bool vs = st[c].vskip != 0;
+    const T ia = (T)st[c].inv_alpha, chb = (T)st[c].c_hbar, cx = (T)st[c].c_x, ch = (T)st[c].c_h;
+    double sq = 0.0;
+    if (act) {
+        for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < p.pK; e += (int64_t)gridDim.x * kBlock) {
+            T vn = p.v[e];
+            if (!vs) {
+                vn = vn * ia;
+                p.v[e] = vn;
+            }
+            const T hv = p.h[e];
+            const T hb = p.hbar[e] * chb + hv;
+            p.hbar[e] = hb;
+            const T xv = p.x[e] + cx * hb;
+            p.x[e] = xv;
+            p.h[e] = hv * ch + vn;
+            sq += (double)xv * (double)xv;
+        }
+    }
+    __shared__ double red[kBlock];
+    __shared__ int lastflag;
+    const double s = block_column_sum<K>(sq, red);
+    if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+    if (!elect_last_block(p.counter, &lastflag)) return;
+    const double tot = block_sum_partials<K>(p.partials, gridDim.x, nullptr, 0, red);
+    if (threadIdx.x < K) fin_after_update<Tr>(st[threadIdx.x], tot, threadIdx.x, p.ops_next_db);
+    __syncthreads();
+    publish_all_done<K, Tr>(st, p.all_done_w);
+}
+
+// LSMR set-up of the p-length vectors (:68, :77-78): v /= alpha; h = v; hbar = 0.
+template <typename T, typename Tr, int K>
+__global__ __launch_bounds__(kBlock) void k_init_vectors(int64_t pK, T *v, T *h, T *hbar, const void *state) {
+    const LsmrState<Tr> *st = (const LsmrState<Tr> *)state;
+    const int c = threadIdx.x % K;
+    const T ia = (T)st[c].inv_alpha;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < pK; e += (int64_t)gridDim.x * kBlock) {
+        const T vn = v[e] * ia;
+        v[e] = vn;
+        h[e] = vn;
+        hbar[e] = (T)0;
+    }
+}
+
+}  // namespace fehip

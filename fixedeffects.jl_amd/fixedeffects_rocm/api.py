@@ -1,0 +1,426 @@
+"""Host-side mirror of FixedEffects.jl's public API for the ``:rocm`` backend.
+
+Names, argument meanings, defaults and error behaviour follow the reference:
+
+* ``FixedEffect``                    -- src/FixedEffect.jl:7-21, :44-50
+* ``AbstractFixedEffectSolver``      -- the backend constructor
+  ``AbstractFixedEffectSolver{T}(fes, w, Val{method})`` (src/SolverCPU.jl:55)
+* ``FixedEffectSolverROCm.update_weights`` -- update_weights! (src/SolverCPU.jl:67-76)
+* ``solve_residuals``                -- src/AbstractFixedEffectSolver.jl:37-68, :74-97
+* ``solve_coefficients``             -- src/AbstractFixedEffectSolver.jl:137-160
+* ``normalize``, ``components``, ``full`` -- src/FixedEffect.jl:57-140
+
+All arithmetic of the solve runs in libfehip.so on the GPU; there is no CPU
+path in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+from ._lib import (FEHIP_FP32, FEHIP_FP64, FEHIP_MIXED, check, lib)
+
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class DimensionMismatch(ValueError):
+    """Julia's DimensionMismatch (src/FixedEffect.jl:12-13)."""
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+# ---------------------------------------------------------------------------
+# FixedEffect
+# ---------------------------------------------------------------------------
+class FixedEffect:
+    """A categorical variable: 1-based ``refs``, optional ``interaction``, ``n`` levels.
+
+    ``FixedEffect(keys1, keys2, ..., interaction=None)`` groups the key arrays
+    into dense 1-based refs (the reference uses GroupedArray with
+    ``sort = nothing``, src/FixedEffect.jl:18-21; here levels are numbered in
+    sorted key order -- level numbering does not change residuals, which is
+    what test/solve.jl:18-24 checks).  ``FixedEffect.from_refs(refs, n,
+    interaction)`` is the inner constructor (src/FixedEffect.jl:11-15).
+    """
+
+    __slots__ = ("refs", "interaction", "n")
+
+    def __init__(self, *keys, interaction=None):
+        if len(keys) == 0:
+            raise ValueError("FixedEffect needs at least one key vector")
+        arrs = [np.asarray(k) for k in keys]
+        N = len(arrs[0])
+        if any(len(a) != N for a in arrs):
+            raise DimensionMismatch("all key vectors must have the same length")
+        if len(arrs) == 1:
+            _, inv = np.unique(arrs[0], return_inverse=True)
+        else:
+            rec = np.rec.fromarrays(arrs)
+            _, inv = np.unique(rec, return_inverse=True)
+        refs = inv.astype(np.int64).reshape(-1) + 1
+        n = int(refs.max()) if N else 0
+        self._set(refs, interaction, n)
+
+    @classmethod
+    def from_refs(cls, refs, n: int, interaction=None) -> "FixedEffect":
+        fe = cls.__new__(cls)
+        fe._set(np.ascontiguousarray(refs, dtype=np.int64), interaction, int(n))
+        return fe
+
+    def _set(self, refs, interaction, n):
+        if interaction is not None:
+            interaction = np.ascontiguousarray(interaction, dtype=np.float64)
+            if len(interaction) != len(refs):
+                raise DimensionMismatch(
+                    f"cannot match refs of length {len(refs)} with interaction of length {len(interaction)}")
+        self.refs = refs
+        self.interaction = interaction  # None = UnitWeights
+        self.n = n
+
+    def __len__(self) -> int:
+        return len(self.refs)
+
+    def __getitem__(self, esample) -> "FixedEffect":
+        """getindex(fe, esample) (src/FixedEffect.jl:42-50): keeps n."""
+        if isinstance(esample, slice) and esample == slice(None):
+            return self
+        idx = np.asarray(esample)
+        if idx.ndim != 1:
+            raise TypeError("esample must be one-dimensional")
+        z = None if self.interaction is None else self.interaction[idx]
+        return FixedEffect.from_refs(self.refs[idx], self.n, z)
+
+    def __repr__(self) -> str:
+        return "Fixed Effects"
+
+
+# ---------------------------------------------------------------------------
+# solver (the :rocm backend object)
+# ---------------------------------------------------------------------------
+_PRECISIONS = {"fp64": FEHIP_FP64, "fp32": FEHIP_FP32, "mixed": FEHIP_MIXED}
+
+
+class FixedEffectSolverROCm:
+    """``FixedEffectSolverROCm{T} <: AbstractFixedEffectSolver{T}``.
+
+    Holds a libfehip handle: refs, CSR bucketings, scales and caches live in
+    device memory for the life of the object.
+    """
+
+    def __init__(self, fes: Sequence[FixedEffect], weights=None, precision: str = "fp64"):
+        fes = list(fes)
+        if not fes:
+            raise ValueError("at least one FixedEffect is required")
+        N = len(fes[0])
+        if any(len(fe) != N for fe in fes):
+            raise ValueError("FixedEffects must have the same length as y")
+        if precision not in _PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
+        self.fes = fes
+        self.N = N
+        self.G = len(fes)
+        self.n = np.array([fe.n for fe in fes], dtype=np.int64)
+        self.p = int(self.n.sum())
+        self.precision = precision
+        self.T = np.float64 if precision == "fp64" else np.float32
+        self._refs = [np.ascontiguousarray(fe.refs, dtype=np.int64) for fe in fes]
+        self._z = [fe.interaction for fe in fes]
+        self.weights = None
+        w = self._weights_array(weights)
+        refs_arr = (_i64p * self.G)(*[_ptr(r, C.c_int64) for r in self._refs])
+        z_arr = (_dp * self.G)(*[C.cast(None, _dp) if z is None else _ptr(z, C.c_double) for z in self._z])
+        h = C.c_void_p()
+        check(lib.fehip_create(C.byref(h), N, self.G, refs_arr, _ptr(self.n, C.c_int64), z_arr,
+                               C.cast(None, _dp) if w is None else _ptr(w, C.c_double), _PRECISIONS[precision]))
+        self._h = h
+        self.weights = w
+
+    def _weights_array(self, weights):
+        if weights is None:
+            return None
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if len(w) != self.N:
+            raise ValueError("weights must have the same length as the FixedEffects")
+        if np.all(w == 1.0):
+            # UnitWeights and an all-ones Weights vector give the same arithmetic
+            return None
+        return w
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.fehip_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- reference methods -------------------------------------------------
+    def update_weights(self, weights) -> "FixedEffectSolverROCm":
+        """update_weights!(feM, w) (src/SolverCPU.jl:67-76)."""
+        w = self._weights_array(weights)
+        check(lib.fehip_update_weights(self._h, C.cast(None, _dp) if w is None else _ptr(w, C.c_double)))
+        self.weights = w
+        return self
+
+    def _default_tol(self, tol):
+        if tol is None:
+            return float(np.sqrt(np.finfo(self.T).eps))  # sqrt(eps(T)) (:49)
+        return float(tol)
+
+    def solve_residuals(self, y, tol: Optional[float] = None, maxiter: int = 100_000):
+        """Inner solve_residuals!(r, feM; tol, maxiter) (:49-68) on one vector,
+        overwritten in place; returns (r, iterations, converged)."""
+        y = _as_inplace_vector(y, self.N)
+        col = (_dp * 1)(_ptr(y, C.c_double))
+        it = np.zeros(1, np.int64)
+        cv = np.zeros(1, np.uint8)
+        check(lib.fehip_solve_residuals(self._h, col, 1, self._default_tol(tol), int(maxiter), _ptr(it, C.c_int64),
+                                        _ptr(cv, C.c_uint8)))
+        return y, int(it[0]), bool(cv[0])
+
+    def solve_residuals_many(self, xs, tol: Optional[float] = None, maxiter: int = 100_000,
+                             progress_bar: bool = False):
+        """Multi-column solve_residuals!(xs, feM) (:74-97): all columns in one
+        batched solve (up to 32 per launch); returns (xs, iterations, convergeds)."""
+        cols = _columns(xs, self.N)
+        K = len(cols)
+        if K == 0:
+            return xs, [], []
+        t0 = time.time()
+        arr = (_dp * K)(*[_ptr(c, C.c_double) for c in cols])
+        it = np.zeros(K, np.int64)
+        cv = np.zeros(K, np.uint8)
+        check(lib.fehip_solve_residuals(self._h, arr, K, self._default_tol(tol), int(maxiter), _ptr(it, C.c_int64),
+                                        _ptr(cv, C.c_uint8)))
+        if progress_bar and time.time() - t0 > 2:
+            print(f"Demean Variables: {K}/{K}")
+        _write_back(xs, cols)
+        return xs, [int(i) for i in it], [bool(c) for c in cv]
+
+    def solve_residuals_device(self, dptr: int, K: int, ld: int, tol: Optional[float] = None,
+                               maxiter: int = 100_000):
+        """Residualize K device-resident columns (column c at dptr + 8*c*ld)."""
+        it = np.zeros(K, np.int64)
+        cv = np.zeros(K, np.uint8)
+        check(lib.fehip_solve_residuals_device(self._h, C.c_void_p(int(dptr)), int(K), int(ld),
+                                               self._default_tol(tol), int(maxiter), _ptr(it, C.c_int64),
+                                               _ptr(cv, C.c_uint8)))
+        return [int(i) for i in it], [bool(c) for c in cv]
+
+    def solve_coefficients(self, y, tol: Optional[float] = None, maxiter: int = 100_000):
+        """Inner solve_coefficients!(r, feM; tol, maxiter) (:147-160): returns
+        (G per-observation coefficient vectors, div(iterations, 2), converged)."""
+        blocks, it, cv = self.solve_coefficients_raw(y, tol, maxiter)
+        blocks = normalize(blocks, self.fes)
+        return full(blocks, self.fes), it, cv
+
+    def solve_coefficients_raw(self, y, tol: Optional[float] = None, maxiter: int = 100_000):
+        """LSMR from zero, x .*= scale (:149-158), before normalize!/full."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if len(y) != self.N:
+            raise ValueError("FixedEffects must have the same length as y")
+        blocks = [np.empty(int(k), np.float64) for k in self.n]
+        arr = (_dp * self.G)(*[_ptr(b, C.c_double) for b in blocks])
+        it = C.c_int64()
+        cv = C.c_uint8()
+        check(lib.fehip_solve_coefficients(self._h, _ptr(y, C.c_double), self._default_tol(tol), int(maxiter), arr,
+                                           C.byref(it), C.byref(cv)))
+        return blocks, int(it.value), bool(cv.value)
+
+    # -- inspection (test hooks of the C-ABI) ------------------------------
+    def csr(self, g: int):
+        """CSR bucketing of FE g (0-based perm and offsets)."""
+        perm = np.empty(self.N, np.int32)
+        off = np.empty(int(self.n[g]) + 1, np.int32)
+        check(lib.fehip_get_csr(self._h, g, _ptr(perm, C.c_int32), _ptr(off, C.c_int32)))
+        return perm, off
+
+    def scale(self, g: int) -> np.ndarray:
+        out = np.empty(int(self.n[g]), np.float64)
+        check(lib.fehip_get_scale(self._h, g, _ptr(out, C.c_double)))
+        return out
+
+    def cache(self, g: int) -> np.ndarray:
+        out = np.empty(self.N, np.float64)
+        check(lib.fehip_get_cache(self._h, g, _ptr(out, C.c_double)))
+        return out
+
+    def mul_A(self, x, y=None, a: float = 1.0, b: float = 0.0) -> np.ndarray:
+        """mul!(y, A, x, a, b) (src/AbstractFixedEffectLinearMap.jl:37-44)."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.N, np.float64) if y is None else np.array(y, np.float64, copy=True)
+        check(lib.fehip_mul_A(self._h, _ptr(y, C.c_double), _ptr(x, C.c_double), a, b))
+        return y
+
+    def mul_At(self, y, x=None, a: float = 1.0, b: float = 0.0) -> np.ndarray:
+        """mul!(x, A', y, a, b) (src/AbstractFixedEffectLinearMap.jl:26-35)."""
+        y = np.ascontiguousarray(y, np.float64)
+        x = np.zeros(self.p, np.float64) if x is None else np.array(x, np.float64, copy=True)
+        check(lib.fehip_mul_At(self._h, _ptr(x, C.c_double), _ptr(y, C.c_double), a, b))
+        return x
+
+
+def _as_inplace_vector(y, N):
+    if not (isinstance(y, np.ndarray) and y.dtype == np.float64 and y.ndim == 1 and y.flags.c_contiguous):
+        raise TypeError("y must be a contiguous 1-D float64 numpy array (it is overwritten in place)")
+    if len(y) != N:
+        raise ValueError("FixedEffects must have the same length as y")
+    return y
+
+
+def _columns(xs, N) -> List[np.ndarray]:
+    if isinstance(xs, np.ndarray) and xs.ndim == 2:
+        cols = [np.ascontiguousarray(xs[:, j], dtype=np.float64) for j in range(xs.shape[1])]
+    else:
+        cols = [_as_inplace_vector(x, N) for x in xs]
+    for c in cols:
+        if len(c) != N:
+            raise ValueError("FixedEffects must have the same length as y")
+    return cols
+
+
+def _write_back(xs, cols):
+    if isinstance(xs, np.ndarray) and xs.ndim == 2:
+        for j, c in enumerate(cols):
+            xs[:, j] = c
+
+
+def AbstractFixedEffectSolver(fes, weights=None, method: str = "rocm", double_precision: bool = True,
+                              precision: Optional[str] = None) -> FixedEffectSolverROCm:
+    """Backend constructor AbstractFixedEffectSolver{T}(fes, w, Val{method})
+    (src/SolverCPU.jl:55, src/AbstractFixedEffectSolver.jl:43).  Only
+    method = "rocm" exists here; precision="mixed" selects the FP32-stream /
+    FP64-recurrence mode (not a reference option)."""
+    if method != "rocm":
+        raise ValueError(f"method {method!r} is not provided by this package (method = 'rocm')")
+    if precision is None:
+        precision = "fp64" if double_precision else "fp32"
+    return FixedEffectSolverROCm(fes, weights, precision)
+
+
+def _outer(method, double_precision, tol):
+    if double_precision is None:
+        double_precision = method == "cpu"  # :39 -- GPU methods default to Float32
+    if tol is None:
+        tol = 1e-8 if double_precision else 1e-6  # :40
+    return double_precision, tol
+
+
+def solve_residuals(y, fes, w=None, *, method: str = "rocm", double_precision: Optional[bool] = None,
+                    tol: Optional[float] = None, maxiter: int = 10000, precision: Optional[str] = None,
+                    progress_bar: bool = True):
+    """solve_residuals!(y, fes, w; method, double_precision, tol, maxiter)
+    (src/AbstractFixedEffectSolver.jl:37-45), and the solver-object forms
+    solve_residuals!(y, feM) / solve_residuals!(xs, feM) (:49-97).
+
+    y is overwritten in place.  Returns (r, iterations, converged), or
+    (xs, iterations, convergeds) for a collection / 2-D array of columns."""
+    if isinstance(fes, FixedEffectSolverROCm):
+        feM = fes
+        kw = {} if tol is None else {"tol": tol}
+        if _is_single(y):
+            return feM.solve_residuals(y, maxiter=maxiter, **kw)
+        return feM.solve_residuals_many(y, maxiter=maxiter, progress_bar=progress_bar, **kw)
+    fes = list(fes)
+    N = len(y) if _is_single(y) else (y.shape[0] if isinstance(y, np.ndarray) else len(y[0]))
+    if any(len(fe) != N for fe in fes):
+        raise ValueError("FixedEffects must have the same length as y")  # :42
+    dp, tol = _outer(method, double_precision, tol)
+    with AbstractFixedEffectSolver(fes, w, method, dp, precision) as feM:
+        if _is_single(y):
+            return feM.solve_residuals(y, tol=tol, maxiter=maxiter)
+        return feM.solve_residuals_many(y, tol=tol, maxiter=maxiter, progress_bar=progress_bar)
+
+
+def solve_coefficients(y, fes, w=None, *, method: str = "rocm", double_precision: Optional[bool] = None,
+                       tol: Optional[float] = None, maxiter: int = 10000, precision: Optional[str] = None):
+    """solve_coefficients!(y, fes, w; ...) (src/AbstractFixedEffectSolver.jl:137-145)."""
+    if isinstance(fes, FixedEffectSolverROCm):
+        kw = {} if tol is None else {"tol": tol}
+        return fes.solve_coefficients(y, maxiter=maxiter, **kw)
+    fes = list(fes)
+    if any(len(fe) != len(y) for fe in fes):
+        raise ValueError("FixedEffects must have the same length as y")  # :142
+    dp, tol = _outer(method, double_precision, tol)
+    with AbstractFixedEffectSolver(fes, w, method, dp, precision) as feM:
+        return feM.solve_coefficients(y, tol=tol, maxiter=maxiter)
+
+
+def _is_single(y) -> bool:
+    return isinstance(y, np.ndarray) and y.ndim == 1
+
+
+# ---------------------------------------------------------------------------
+# coefficient post-processing (host side, as in the reference)
+# ---------------------------------------------------------------------------
+def components(fes: Sequence[FixedEffect]):
+    """Connected components of the observation/level graph
+    (src/FixedEffect.jl:68-98) as per-FE level labels (-1 = level not observed)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    n = [fe.n for fe in fes]
+    offs = np.concatenate([[0], np.cumsum(n)])
+    tot = int(offs[-1])
+    if len(fes) == 1:
+        rows = np.asarray(fes[0].refs) - 1
+        cols = rows
+    else:
+        r0 = np.asarray(fes[0].refs, np.int64) - 1
+        rows = np.concatenate([r0 for _ in fes[1:]])
+        cols = np.concatenate([np.asarray(fe.refs, np.int64) - 1 + offs[j] for j, fe in enumerate(fes) if j > 0])
+    A = coo_matrix((np.ones(len(rows), np.int8), (rows, cols)), shape=(tot, tot))
+    _, lab = connected_components(A, directed=False)
+    seen = np.zeros(tot, bool)
+    for j, fe in enumerate(fes):
+        seen[np.asarray(fe.refs, np.int64) - 1 + offs[j]] = True
+    lab = np.where(seen, lab, -1)
+    return [lab[offs[j]:offs[j + 1]] for j in range(len(fes))]
+
+
+def normalize(fecoefs: List[np.ndarray], fes: Sequence[FixedEffect]) -> List[np.ndarray]:
+    """normalize! (src/FixedEffect.jl:106-111) with rescale! (:113-135): within
+    each connected component, FEs 2..end (UnitWeights interaction only) are
+    demeaned and their means added to the first."""
+    idx = [j for j, fe in enumerate(fes) if fe.interaction is None]
+    if len(idx) < 2:
+        return fecoefs
+    sel = [fes[j] for j in idx]
+    labels = components(sel)
+    ncomp = max(int(lb.max()) for lb in labels) + 1
+    m = np.zeros(ncomp)
+    for jj in range(len(idx) - 1, 0, -1):
+        c = fecoefs[idx[jj]]
+        lb = labels[jj]
+        ok = lb >= 0
+        sums = np.bincount(lb[ok], weights=c[ok], minlength=ncomp)
+        cnt = np.bincount(lb[ok], minlength=ncomp)
+        mj = np.divide(sums, cnt, out=np.zeros(ncomp), where=cnt > 0)
+        c[ok] -= mj[lb[ok]]
+        m += mj
+    c1 = fecoefs[idx[0]]
+    lb = labels[0]
+    ok = lb >= 0
+    c1[ok] += m[lb[ok]]
+    return fecoefs
+
+
+def full(fecoefs: List[np.ndarray], fes: Sequence[FixedEffect]) -> List[np.ndarray]:
+    """full (src/FixedEffect.jl:137-140): per-observation coefficient vectors."""
+    return [c[np.asarray(fe.refs, np.int64) - 1] for c, fe in zip(fecoefs, fes)]

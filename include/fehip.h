@@ -1,0 +1,154 @@
+/*
+ * fehip.h -- C-ABI of libfehip.so, the MI355X (gfx950) fixed-effect
+ * residualization solver.
+ *
+ * The library replaces, for one new backend (`method = :rocm`), the part of
+ * FixedEffects.jl v3.1.0 that a backend plugs in behind
+ * `AbstractFixedEffectSolver{T}(fes, w, Val{method})`:
+ *   - FixedEffectLinearMap mul! and its adjoint (src/AbstractFixedEffectLinearMap.jl:26-44,
+ *     kernels src/SolverCPU.jl:21-34),
+ *   - the scale / cache construction (src/SolverCPU.jl:67-95),
+ *   - the LSMR loop (src/utils/lsmr.jl:46-222),
+ *   - the inner solve_residuals! / solve_coefficients! methods
+ *     (src/AbstractFixedEffectSolver.jl:49-68, :74-97, :147-160).
+ * FixedEffect construction, normalize!/full and the outer keyword methods stay
+ * with the caller.
+ *
+ * Conventions
+ *   - refs are 1-based, as FixedEffect.refs (src/FixedEffect.jl:7-21); the
+ *     create call validates 1 <= ref <= n (the reference leaves an out-of-range
+ *     ref undefined under @inbounds).
+ *   - interaction[g] == NULL means UnitWeights (no interaction); weights ==
+ *     NULL means UnitWeights.
+ *   - every host buffer is copied; the caller keeps ownership.
+ *   - calls block until results are in the caller's buffers.  A handle is not
+ *     thread-safe; each handle owns its own HIP stream.
+ *   - every entry point returns 0 on success or a negative FEHIP_E* code;
+ *     fehip_last_error() then holds a message (thread-local).
+ */
+#ifndef FEHIP_H
+#define FEHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fehip_solver fehip_t;
+
+/* status codes */
+#define FEHIP_OK 0
+#define FEHIP_EDIM -1  /* inconsistent lengths / sizes */
+#define FEHIP_EREF -2  /* a ref outside 1..n */
+#define FEHIP_EHIP -3  /* HIP runtime error */
+#define FEHIP_EOOM -4  /* device allocation failed */
+#define FEHIP_EARG -5  /* invalid argument */
+
+/* arithmetic of the solve */
+#define FEHIP_FP64 0   /* Float64 everywhere: double_precision = true */
+#define FEHIP_FP32 1   /* Float32 everywhere, LSMR scalars included:
+                          double_precision = false (src/utils/lsmr.jl:59) */
+#define FEHIP_MIXED 2  /* Float32 streams with compensated segment sums,
+                          Float64 LSMR scalars, Float64 refinement sweeps */
+
+/* Builds the solver: uploads refs, builds the per-FE CSR permutations on the
+ * device, computes scales and caches.
+ * Replaces AbstractFixedEffectSolver{T}(fes, w, Val{:cpu})
+ * (src/SolverCPU.jl:55-64) and FixedEffectLinearMapCPU{T}(fes)
+ * (src/SolverCPU.jl:13-17).
+ *   refs[g]        : N int64 refs of FE g (Julia Vector{Int}), 1-based
+ *   nlevels[g]     : fe.n
+ *   interaction[g] : N doubles or NULL (UnitWeights)
+ *   weights        : N doubles or NULL (UnitWeights) */
+int fehip_create(fehip_t **out, int64_t N, int32_t G,
+                 const int64_t *const *refs, const int64_t *nlevels,
+                 const double *const *interaction, const double *weights,
+                 int32_t precision);
+
+/* Same as fehip_create with int32 refs (1-based). */
+int fehip_create_i32(fehip_t **out, int64_t N, int32_t G,
+                     const int32_t *const *refs, const int64_t *nlevels,
+                     const double *const *interaction, const double *weights,
+                     int32_t precision);
+
+/* update_weights!(feM, w)  (src/SolverCPU.jl:67-76): recomputes scales and
+ * caches; w NULL = UnitWeights. */
+int fehip_update_weights(fehip_t *h, const double *weights);
+
+/* Residualizes K columns in one batched solve.
+ * Replaces the inner solve_residuals!(r, feM; tol, maxiter)
+ * (src/AbstractFixedEffectSolver.jl:49-68) and, for K > 1, its multi-column
+ * loop (:74-97): each column gets its own LSMR scalars and stopping test.
+ *   cols[c]       : host column c (N doubles), overwritten with residuals
+ *   iters[c]      : lsmr iterations + 1, or 1 when G == 1 (:57-61)
+ *   converged[c]  : istop not in (3, 6, 7) (src/utils/lsmr.jl:218)
+ * maxiter is the outer maxiter; LSMR runs with maxiter - 1 (:59). */
+int fehip_solve_residuals(fehip_t *h, double *const *cols, int32_t K,
+                          double tol, int64_t maxiter, int64_t *iters,
+                          uint8_t *converged);
+
+/* Same as fehip_solve_residuals with the K columns already in device memory:
+ * column c at dY + c * ldY (device pointer, doubles), overwritten in place.
+ * This is the entry point that keeps inputs resident in HBM. */
+int fehip_solve_residuals_device(fehip_t *h, double *dY, int32_t K,
+                                 int64_t ldY, double tol, int64_t maxiter,
+                                 int64_t *iters, uint8_t *converged);
+
+/* Replaces the inner solve_coefficients!(r, feM; tol, maxiter)
+ * (src/AbstractFixedEffectSolver.jl:147-160) up to the collect at :158:
+ * LSMR from x0 = 0 with maxiter, then x .*= scale.  coef[g] receives the
+ * nlevels[g] unnormalized coefficients of FE g; the caller applies
+ * normalize! and full (src/FixedEffect.jl:106-140).
+ *   iters     : div(lsmr iterations, 2) (:159)
+ *   converged : istop not in (3, 6, 7) */
+int fehip_solve_coefficients(fehip_t *h, const double *y, double tol,
+                             int64_t maxiter, double *const *coef,
+                             int64_t *iters, uint8_t *converged);
+
+/* Test hook: the CSR bucketing of FE g over the caller's observation order,
+ * as bucketize_refs builds it (ext/MetalExt.jl:40-64) but 0-based:
+ * perm (N) lists observations grouped by level, ascending within a level;
+ * offsets (nlevels+1) delimit the levels.  Built on the device. */
+int fehip_get_csr(fehip_t *h, int32_t g, int32_t *perm, int32_t *offsets);
+
+/* Test hook: scale vector of FE g (nlevels[g] doubles), as scale!
+ * computes it (src/SolverCPU.jl:78-89), widened to double. */
+int fehip_get_scale(fehip_t *h, int32_t g, double *scale);
+
+/* Test hook: cache vector of FE g in the caller's observation order
+ * (N doubles), as cache! computes it (src/SolverCPU.jl:91-95). */
+int fehip_get_cache(fehip_t *h, int32_t g, double *cache);
+
+/* Test hook: one application of the linear map on K = 1 column, in the
+ * caller's observation order and with the handle's precision.
+ *   fehip_mul_A : y <- b*y + a*A*x   (src/AbstractFixedEffectLinearMap.jl:37-44)
+ *   fehip_mul_At: x <- b*x + a*A'*y  (src/AbstractFixedEffectLinearMap.jl:26-35)
+ * x is the concatenation of the G coefficient blocks (sum(nlevels) doubles). */
+int fehip_mul_A(fehip_t *h, double *y, const double *x, double a, double b);
+int fehip_mul_At(fehip_t *h, double *x, const double *y, double a, double b);
+
+/* Measurement hook (no reference counterpart): average device time in ms of
+ * each kernel of one LSMR iteration on K columns, from HIP events around
+ * `reps` isolated launches on the handle's stream, after filling the solve
+ * vectors with constants:
+ *   out_ms[0] D*v          (the reference's scatter!, src/SolverCPU.jl:29-34)
+ *   out_ms[1] D'u, FEs with many levels (gather!, src/SolverCPU.jl:21-27)
+ *   out_ms[2] D'u, FEs with few levels
+ *   out_ms[3] D'u fix-up of segments split across tiles
+ *   out_ms[4] LSMR vector update (src/utils/lsmr.jl:138-142, :192) */
+int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms);
+
+/* Dimensions of a handle. */
+int fehip_dims(const fehip_t *h, int64_t *N, int32_t *G, int64_t *p);
+
+/* Message for the last failed call on this thread ("" if none). */
+const char *fehip_last_error(void);
+
+/* Frees the handle and its device memory (NULL is a no-op). */
+void fehip_destroy(fehip_t *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

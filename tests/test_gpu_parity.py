@@ -1,0 +1,301 @@
+"""GPU parity: libfehip (through its C-ABI, via the host mirror) against the
+oracle on the same seeded inputs.
+
+Bars:
+* CSR permutation and offsets: bit-exact (integer).
+* scale / cache (FP64): bit-exact -- same summation order, same IEEE ops.
+* one matvec (FP64): rtol 1e-13 (parallel summation order differs).
+* residuals (FP64, tol 1e-8): ||r_gpu - r_oracle|| / ||r_oracle|| <= 1e-8,
+  LSMR iterations within +-2 of the oracle's.
+* reference Float32 mode: Float32.(r) matches at sqrt(eps(Float32)), as
+  test/solve.jl:37 checks.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from fixedeffects_rocm import api, synth
+from fixedeffects_rocm._lib import FEHIP_EREF, FehipError
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "solve_jl.json")))
+RES_RTOL = 1e-8
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _fes(refs, n, z=None):
+    z = z or [None] * len(refs)
+    return [api.FixedEffect.from_refs(r, k, zz) for r, k, zz in zip(refs, n, z)]
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _rand_problem(seed, N, n, interaction=(), weights=True, empty_level=False):
+    rng = np.random.default_rng(seed)
+    refs = []
+    for g, k in enumerate(n):
+        r = rng.integers(1, k + 1, N).astype(np.int64)
+        if N >= k:
+            r[:k] = np.arange(1, k + 1)
+            rng.shuffle(r)
+        if empty_level and k > 2:
+            r[r == 2] = 1  # level 2 never observed
+        refs.append(r)
+    z = [rng.uniform(0.5, 1.5, N) if g in interaction else None for g in range(len(n))]
+    w = rng.lognormal(0, 0.5, N) if weights else None
+    y = rng.standard_normal(N)
+    return refs, list(n), z, w, y
+
+
+# -- golden vectors (test/solve.jl) ---------------------------------------------
+
+def test_golden_fp64(gpu):
+    _say("case: golden fp64")
+    x = np.array(GOLD["x"])
+    fes = [api.FixedEffect(np.array(GOLD["p1"])), api.FixedEffect(np.array(GOLD["p2"]))]
+    r, it, cv = api.solve_residuals(x.copy(), fes, double_precision=True)
+    assert cv
+    np.testing.assert_allclose(r, GOLD["r_ols"], rtol=GOLD["rtol_float64"], atol=0)
+    coefs, it, cv = api.solve_coefficients(x.copy(), fes, double_precision=True)
+    assert cv
+    np.testing.assert_allclose(x - coefs[0] - coefs[1], GOLD["r_ols"], rtol=GOLD["rtol_float64"], atol=0)
+
+
+def test_golden_fp32_default(gpu):
+    # method = :rocm defaults to double_precision = false, tol = 1e-6 (:39-40)
+    _say("case: golden fp32")
+    x = np.array(GOLD["x"])
+    fes = [api.FixedEffect(np.array(GOLD["p1"])), api.FixedEffect(np.array(GOLD["p2"]))]
+    r, it, cv = api.solve_residuals(x.copy(), fes)
+    a, b = r.astype(np.float32), np.array(GOLD["r_ols"], np.float32)
+    assert np.linalg.norm(a - b) <= np.sqrt(np.finfo(np.float32).eps) * np.linalg.norm(b)
+
+
+# -- CSR bucketing -----------------------------------------------------------------
+
+@pytest.mark.parametrize("N,n,skew", [(1, 1, False), (5, 3, False), (4097, 7, False), (100_000, 100_000, False),
+                                      (300_000, 1000, True), (262_144, 65_536, False), (1_000_003, 300_000, True)])
+def test_csr_bit_exact(gpu, N, n, skew):
+    _say(f"case: csr N={N} n={n} skew={skew}")
+    rng = np.random.default_rng(N)
+    if skew:
+        sizes = synth.zipf_sizes(N, n, 1.1, 1)
+        r0 = np.repeat(np.arange(1, n + 1), sizes)
+        rng.shuffle(r0)
+    else:
+        r0 = rng.integers(1, n + 1, N)
+    r1 = rng.integers(1, max(2, n // 3) + 1, N)
+    refs = [r0.astype(np.int64), r1.astype(np.int64)]
+    ns = [n, max(2, n // 3)]
+    with api.FixedEffectSolverROCm(_fes(refs, ns), None, "fp64") as s:
+        for g in range(2):
+            perm, off = s.csr(g)
+            operm, ooff = oracle.bucketize(refs[g], ns[g])
+            np.testing.assert_array_equal(perm, operm - 1)
+            np.testing.assert_array_equal(off, ooff - 1)
+
+
+def test_bad_refs_rejected(gpu):
+    _say("case: bad refs")
+    with pytest.raises(FehipError) as ei:
+        api.FixedEffectSolverROCm([api.FixedEffect.from_refs(np.array([1, 2, 4]), 3)], None, "fp64")
+    assert ei.value.code == FEHIP_EREF
+    with pytest.raises(FehipError):
+        api.FixedEffectSolverROCm([api.FixedEffect.from_refs(np.array([1, 0, 2]), 3)], None, "fp64")
+
+
+# -- scale! / cache! -------------------------------------------------------------
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_scale_cache_exact(gpu, prec):
+    _say(f"case: scale/cache {prec}")
+    refs, n, z, w, _ = _rand_problem(21, 50_000, (9000, 700, 12), interaction=(2,), empty_level=True)
+    o = oracle.OracleSolver(refs, n, z, w, double_precision=(prec == "fp64"))
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, prec) as s:
+        for g in range(3):
+            np.testing.assert_array_equal(s.scale(g), o.scale(g).astype(np.float64))
+            np.testing.assert_array_equal(s.cache(g), o.cache(g).astype(np.float64))
+        # update_weights! recomputes both
+        w2 = np.random.default_rng(5).lognormal(0, 1, len(w))
+        s.update_weights(w2)
+        o.update_weights(w2)
+        for g in range(3):
+            np.testing.assert_array_equal(s.scale(g), o.scale(g).astype(np.float64))
+            np.testing.assert_array_equal(s.cache(g), o.cache(g).astype(np.float64))
+
+
+# -- one matvec in each direction ---------------------------------------------------
+
+@pytest.mark.parametrize("n", [(5000, 300, 20), (200_000, 40), (64, 3000, 7, 2)])
+def test_matvec_fp64(gpu, n):
+    _say(f"case: matvec n={n}")
+    refs, n, z, w, y = _rand_problem(31, 200_003, n, interaction=(1,), empty_level=True)
+    o = oracle.OracleSolver(refs, n, z, w)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(o.p)
+    y0 = rng.standard_normal(o.N)
+    x0 = rng.standard_normal(o.p)
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, "fp64") as s:
+        for a, b in ((1.0, 0.0), (-1.0, 1.0), (0.37, -2.5)):
+            got = s.mul_A(x, y0, a, b)
+            np.testing.assert_allclose(got, o.mul_A(x, y0, a, b), rtol=1e-13, atol=1e-13)
+            got = s.mul_At(y, x0, a, b)
+            np.testing.assert_allclose(got, o.mul_At(y, x0, a, b), rtol=1e-12, atol=1e-12)
+
+
+# -- full solves ---------------------------------------------------------------------
+
+def _check_solve(refs, n, z, w, Y, prec="fp64", tol=1e-8, iter_slack=2):
+    K = Y.shape[1]
+    o = oracle.OracleSolver(refs, n, z, w, double_precision=(prec == "fp64"), nthreads=8)
+    want = []
+    for c in range(K):
+        r, it, cv = o.solve_residuals(Y[:, c], tol=tol, maxiter=10000)
+        want.append((r, it, cv))
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, prec) as s:
+        if K == 1:
+            y = np.ascontiguousarray(Y[:, 0])
+            r, it, cv = s.solve_residuals(y, tol=tol, maxiter=10000)
+            got = [(r, it, cv)]
+        else:
+            cols = [np.ascontiguousarray(Y[:, c]) for c in range(K)]
+            _, its, cvs = s.solve_residuals_many(cols, tol=tol, maxiter=10000)
+            got = list(zip(cols, its, cvs))
+    worst = 0.0
+    for (rg, ig, cg), (ro, io, co) in zip(got, want):
+        worst = max(worst, _rel(rg, ro))
+        assert cg == co
+        assert abs(ig - io) <= iter_slack, (ig, io)
+    return worst, [g[1] for g in got], [w_[1] for w_ in want]
+
+
+def test_solve_cpu_two_way_config(gpu):
+    # N=100,000, FE1 uniform 1:10000, FE2 uniform 1:100, unit weights
+    _say("case: cpu_two_way full size")
+    p = synth.config("cpu_two_way")
+    worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y)
+    _say("  rel err", worst, "iters gpu", ig, "oracle", io)
+    assert worst <= RES_RTOL
+
+
+def test_solve_worker_firm_scaled(gpu):
+    # the 1e7 worker/firm/year workload at 1/50 scale: Zipf workers, mobility, weights
+    _say("case: worker_firm scale 0.02")
+    p = synth.config("worker_firm_1e7", scale=0.02)
+    worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y)
+    _say("  rel err", worst, "iters gpu", ig, "oracle", io)
+    assert worst <= RES_RTOL
+
+
+def test_solve_interaction_and_single_fe(gpu):
+    _say("case: interaction / single FE")
+    refs, n, z, w, y = _rand_problem(41, 30_000, (3000, 200, 9), interaction=(2,))
+    worst, _, _ = _check_solve(refs, n, z, w, y[:, None])
+    assert worst <= RES_RTOL
+    refs1, n1, z1, w1, y1 = _rand_problem(42, 30_000, (3000,), interaction=(0,))
+    worst, ig, _ = _check_solve(refs1, n1, z1, w1, y1[:, None])
+    assert worst <= 1e-12 and ig == [1]
+
+
+def test_solve_batched_columns(gpu):
+    _say("case: batched K=5 and K=32")
+    p = synth.config("worker_firm_1e7_k32", scale=0.01)
+    worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y[:, :5])
+    _say("  K=5 rel err", worst, ig, io)
+    assert worst <= RES_RTOL
+    worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y)
+    _say("  K=32 rel err", worst)
+    assert worst <= RES_RTOL
+
+
+def test_solve_fp32_reference_mode(gpu):
+    _say("case: fp32 mode")
+    refs, n, z, w, y = _rand_problem(51, 40_000, (4000, 300, 10))
+    o = oracle.OracleSolver(refs, n, z, w, double_precision=False)
+    ro, io, co = o.solve_residuals(y.copy(), tol=1e-6, maxiter=10000)
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, "fp32") as s:
+        r, it, cv = s.solve_residuals(y.copy(), tol=1e-6, maxiter=10000)
+    assert cv == co
+    assert _rel(r, ro) <= 1e-4
+    assert abs(it - io) <= max(3, io // 10)
+
+
+def test_coefficients_match_oracle(gpu):
+    _say("case: coefficients")
+    refs, n, z, w, y = _rand_problem(61, 20_000, (2000, 150))
+    o = oracle.OracleSolver(refs, n, z, w)
+    co, io, cvo = o.solve_coefficients(y, tol=1e-10, maxiter=10000)
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, "fp64") as s:
+        cg, ig, cvg = s.solve_coefficients(y, tol=1e-10, maxiter=10000)
+    assert cvg and cvo and abs(ig - io) <= 1
+    fit_o = sum(co)
+    fit_g = sum(cg)
+    assert _rel(fit_g, fit_o) <= 1e-8
+    for a, b in zip(cg, co):
+        np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-6)
+
+
+def test_update_weights_matches_fresh_solver(gpu):
+    # test/solve.jl:45-55 computes these but does not assert them; here they are asserted
+    _say("case: update_weights")
+    p1 = np.array(GOLD["p1"])
+    x = np.array(GOLD["x"])
+    fes = [api.FixedEffect(p1)]
+    with api.FixedEffectSolverROCm(fes, np.ones(10), "fp64") as feM:
+        for wv in (GOLD["weights_update_1"], GOLD["weights_update_2"]):
+            wv = np.array(wv, np.float64)
+            feM.update_weights(wv)
+            r1, _, _ = feM.solve_residuals(x.copy())
+            r2, _, _ = api.solve_residuals(x.copy(), fes, wv, double_precision=True)
+            np.testing.assert_allclose(r1, r2, rtol=1e-14, atol=1e-15)
+            ro, _, _ = oracle.solve_residuals(x, [p1.astype(np.int64)], [5], wv)
+            np.testing.assert_allclose(r1, ro, rtol=1e-12, atol=1e-14)
+
+
+def test_interaction_cases_converge(gpu):
+    # test/solve.jl:57-66: interacted FE alone, and with the same refs uninteracted
+    _say("case: interaction golden")
+    p1 = np.array(GOLD["p1"])
+    x = np.array(GOLD["x"])
+    inter = np.array(GOLD["interaction"])
+    for fes_def, zs in (([p1], [inter]), ([p1, p1], [None, inter])):
+        fes = [api.FixedEffect(r, interaction=z) for r, z in zip(fes_def, zs)]
+        r, it, cv = api.solve_residuals(x.copy(), fes, double_precision=True)
+        assert cv
+        ro, io, co = oracle.solve_residuals(x, [q.astype(np.int64) for q in fes_def], [5] * len(fes_def), None, zs)
+        assert co
+        np.testing.assert_allclose(r, ro, rtol=1e-8, atol=1e-10)
+
+
+def test_maxiter_and_stop_codes(gpu):
+    _say("case: maxiter")
+    refs, n, z, w, y = _rand_problem(71, 10_000, (1000, 100))
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, "fp64") as s:
+        r, it, cv = s.solve_residuals(y.copy(), tol=1e-14, maxiter=3)
+        assert it == 3 and not cv
+        r, it, cv = s.solve_residuals(y.copy(), tol=1e-14, maxiter=1)
+        assert it == 1
+        r, it, cv = s.solve_residuals(np.zeros(len(y)), tol=1e-8, maxiter=100)
+        assert cv and it == 1 and not np.any(r)
+
+
+def test_deterministic(gpu):
+    _say("case: determinism")
+    p = synth.config("worker_firm_1e7", scale=0.01)
+    with api.FixedEffectSolverROCm(_fes(p.refs, p.n), p.weights, "fp64") as s:
+        a, ia, _ = s.solve_residuals(np.ascontiguousarray(p.Y[:, 0]))
+        b, ib, _ = s.solve_residuals(np.ascontiguousarray(p.Y[:, 0]))
+    assert ia == ib
+    np.testing.assert_array_equal(a, b)

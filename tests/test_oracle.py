@@ -1,0 +1,297 @@
+"""Pins the CPU oracle (oracle/, the C restatement of the reference) before it
+is trusted as the checker.  Each check is independent of the oracle's own
+formulas: the reference's golden vectors, closed forms, scipy's LSMR (the same
+Fong-Saunders algorithm) on the explicit matrix, dense least squares, graph
+components from scipy, and invariants of the least-squares problem.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
+from scipy.sparse.linalg import lsmr as scipy_lsmr
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "solve_jl.json")))
+
+
+def _golden():
+    x = np.array(GOLD["x"])
+    refs = [np.array(GOLD["p1"], np.int64), np.array(GOLD["p2"], np.int64)]
+    return x, refs, [5, 5], np.array(GOLD["r_ols"])
+
+
+def _rand_problem(rng, N, n, interaction=(), weights=True):
+    refs = [rng.integers(1, k + 1, N).astype(np.int64) for k in n]
+    # make levels dense (every level observed), as GroupedArray produces
+    for g, k in enumerate(n):
+        refs[g][:k] = np.arange(1, k + 1)
+        rng.shuffle(refs[g])
+    z = [rng.uniform(0.5, 1.5, N) if g in interaction else None for g in range(len(n))]
+    w = rng.lognormal(0, 0.5, N) if weights else None
+    y = rng.standard_normal(N)
+    return refs, list(n), z, w, y
+
+
+def _explicit_A(s: oracle.OracleSolver):
+    rows, cols, vals = [], [], []
+    off = 0
+    for g in range(s.G):
+        rows.append(np.arange(s.N))
+        cols.append(s._refs[g] - 1 + off)
+        vals.append(s.cache(g).astype(np.float64))
+        off += s.n[g]
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(s.N, s.p))
+
+
+# -- golden vectors of the reference (test/solve.jl) --------------------------
+
+def test_golden_residuals_fp64():
+    x, refs, n, r_ols = _golden()
+    r, it, cv = oracle.solve_residuals(x, refs, n)           # test/solve.jl:15-16
+    assert cv
+    np.testing.assert_allclose(r, r_ols, rtol=GOLD["rtol_float64"], atol=0)
+
+
+def test_golden_coefficients_fp64():
+    x, refs, n, r_ols = _golden()
+    coefs, it, cv = oracle.solve_coefficients(x, refs, n)    # test/solve.jl:9-12
+    assert cv
+    np.testing.assert_allclose(x - coefs[0] - coefs[1], r_ols, rtol=GOLD["rtol_float64"], atol=0)
+
+
+def test_golden_residuals_fp32():
+    x, refs, n, r_ols = _golden()
+    r, it, cv = oracle.solve_residuals(x, refs, n, double_precision=False)   # test/solve.jl:34-38
+    rtol = float(np.sqrt(np.finfo(np.float32).eps))
+    a, b = r.astype(np.float32), r_ols.astype(np.float32)
+    assert np.linalg.norm(a - b) <= rtol * max(np.linalg.norm(a), np.linalg.norm(b))
+
+
+def test_golden_relabel_invariance():
+    # test/solve.jl:18-24: a different level encoding gives the same residuals
+    x, refs, n, r_ols = _golden()
+    relabel = [np.array([3, 5, 1, 2, 4])[r - 1] for r in refs]
+    r, _, _ = oracle.solve_residuals(x, relabel, n)
+    np.testing.assert_allclose(r, r_ols, rtol=GOLD["rtol_float64"], atol=0)
+
+
+def test_length_mismatch_error():
+    with pytest.raises(ValueError, match="FixedEffects must have the same length as y"):   # test/solve.jl:41-42
+        oracle.solve_residuals(np.ones(100), [np.array([1, 2])], [2])
+
+
+# -- bucketize_refs: stable counting sort ------------------------------------
+
+@pytest.mark.parametrize("N,n", [(1, 1), (10, 3), (1000, 7), (5000, 5000), (20000, 300)])
+def test_bucketize_matches_stable_argsort(N, n):
+    rng = np.random.default_rng(N + n)
+    refs = rng.integers(1, n + 1, N)
+    perm, off = oracle.bucketize(refs, n)
+    np.testing.assert_array_equal(perm - 1, np.argsort(refs, kind="stable"))
+    np.testing.assert_array_equal(off, np.concatenate([[1], 1 + np.cumsum(np.bincount(refs - 1, minlength=n))]))
+
+
+def test_bucketize_empty_levels():
+    refs = np.array([4, 2, 4, 4, 2], np.int64)
+    perm, off = oracle.bucketize(refs, 6)
+    np.testing.assert_array_equal(perm, [2, 5, 1, 3, 4])
+    np.testing.assert_array_equal(off, [1, 1, 3, 3, 6, 6, 6])
+
+
+# -- scale! / cache! ------------------------------------------------------------
+
+def test_scale_cache_closed_form():
+    rng = np.random.default_rng(3)
+    refs, n, z, w, _ = _rand_problem(rng, 3000, (200, 17), interaction=(1,))
+    s = oracle.OracleSolver(refs, n, z, w)
+    for g in range(2):
+        zz = z[g] if z[g] is not None else np.ones(len(w))
+        ssum = np.bincount(refs[g] - 1, weights=zz * zz * w, minlength=n[g])   # same sequential order
+        expect = np.where(ssum > 0, 1.0 / np.sqrt(ssum), 0.0)
+        np.testing.assert_array_equal(s.scale(g), expect)
+        np.testing.assert_array_equal(s.cache(g), zz * np.sqrt(w) * expect[refs[g] - 1])
+
+
+def test_scale_jacobi_property():
+    # each column of the preconditioned design has unit norm: diag(A'A) = 1
+    rng = np.random.default_rng(4)
+    refs, n, z, w, _ = _rand_problem(rng, 2000, (150, 9), interaction=(0,))
+    s = oracle.OracleSolver(refs, n, z, w)
+    A = _explicit_A(s)
+    np.testing.assert_allclose(np.asarray(A.multiply(A).sum(axis=0)).ravel(), 1.0, rtol=1e-13)
+
+
+def test_scale_issue97_zero_interaction():
+    # a level whose interaction is zero everywhere gets scale 0 (src/SolverCPU.jl:84-88)
+    refs = [np.array([1, 1, 2, 2, 3], np.int64)]
+    z = [np.array([0.0, 0.0, 1.0, 2.0, 3.0])]
+    s = oracle.OracleSolver(refs, [3], z, None)
+    np.testing.assert_array_equal(s.scale(0), [0.0, 1 / np.sqrt(5.0), 1 / 3.0])
+
+
+# -- gather! / scatter! (A'y and A x) -----------------------------------------
+
+def test_linear_map_matches_explicit_matrix():
+    rng = np.random.default_rng(5)
+    refs, n, z, w, y = _rand_problem(rng, 4000, (300, 40, 6), interaction=(2,))
+    s = oracle.OracleSolver(refs, n, z, w)
+    A = _explicit_A(s)
+    x = rng.standard_normal(s.p)
+    y0 = rng.standard_normal(s.N)
+    x0 = rng.standard_normal(s.p)
+    np.testing.assert_allclose(s.mul_A(x, y0, 2.0, -0.5), -0.5 * y0 + 2.0 * (A @ x), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(s.mul_At(y, x0, 0.7, 1.5), 1.5 * x0 + 0.7 * (A.T @ y), rtol=1e-12, atol=1e-12)
+    # adjoint identity
+    assert np.isclose(np.dot(s.mul_A(x), y), np.dot(x, s.mul_At(y)), rtol=1e-12)
+
+
+# -- lsmr! against scipy's LSMR on the explicit matrix ---------------------------
+
+def test_lsmr_zero_start_matches_scipy():
+    rng = np.random.default_rng(6)
+    refs, n, z, w, y = _rand_problem(rng, 3000, (400, 60, 5))
+    s = oracle.OracleSolver(refs, n, z, w)
+    A = _explicit_A(s)
+    b = y * np.sqrt(w)
+    for tol in (1e-4, 1e-8):
+        x, info = s.lsmr(np.zeros(s.p), b, atol=tol, btol=tol, conlim=1e8, maxiter=10000)
+        xs, istop, itn = scipy_lsmr(A, b, atol=tol, btol=tol, conlim=1e8, maxiter=10000)[:3]
+        assert info.iters == itn
+        assert info.istop == istop
+        np.testing.assert_allclose(x, xs, rtol=1e-9, atol=1e-11 * np.abs(xs).max())
+
+
+def test_lsmr_warm_start_iterates_match_scipy():
+    # x0 = A'b as solve_residuals! uses; tolerances 0 so neither stops early and
+    # the k-th iterates are compared directly
+    rng = np.random.default_rng(7)
+    refs, n, z, w, y = _rand_problem(rng, 3000, (500, 80, 7))
+    s = oracle.OracleSolver(refs, n, z, w)
+    A = _explicit_A(s)
+    b = y * np.sqrt(w)
+    x0 = A.T @ b
+    for k in (1, 3, 10):
+        x, info = s.lsmr(x0, b, atol=0.0, btol=0.0, conlim=0.0, maxiter=k)
+        xs = scipy_lsmr(A, b, atol=0.0, btol=0.0, conlim=0.0, maxiter=k, x0=x0)[0]
+        assert info.iters == k and info.istop == 7
+        np.testing.assert_allclose(x, xs, rtol=1e-10, atol=1e-12 * np.abs(xs).max())
+
+
+def test_lsmr_stop_codes():
+    rng = np.random.default_rng(8)
+    refs, n, z, w, y = _rand_problem(rng, 500, (40, 10))
+    s = oracle.OracleSolver(refs, n, z, w)
+    # b = 0: normAr == 0, immediate exit, istop 0, converged (src/utils/lsmr.jl:101)
+    _, info = s.lsmr(np.zeros(s.p), np.zeros(s.N))
+    assert info.iters == 0 and info.istop == 0 and info.converged
+    # maxiter reached: istop 7, not converged
+    _, info = s.lsmr(np.zeros(s.p), y, atol=1e-14, btol=1e-14, maxiter=2)
+    assert info.iters == 2 and info.istop == 7 and not info.converged
+
+
+# -- full solves -----------------------------------------------------------------
+
+def _dense_wls_residuals(refs, n, z, w, y):
+    N = len(y)
+    cols = []
+    for g, k in enumerate(n):
+        D = np.zeros((N, k))
+        D[np.arange(N), refs[g] - 1] = 1.0 if z[g] is None else z[g]
+        cols.append(D)
+    X = np.hstack(cols)
+    sw = np.ones(N) if w is None else np.sqrt(w)
+    beta = np.linalg.lstsq(X * sw[:, None], y * sw, rcond=None)[0]
+    return y - X @ beta
+
+
+@pytest.mark.parametrize("seed,n,inter", [(10, (30, 8), ()), (11, (50, 20, 4), (2,)), (12, (25, 25), (0,))])
+def test_residuals_match_dense_least_squares(seed, n, inter):
+    rng = np.random.default_rng(seed)
+    refs, n, z, w, y = _rand_problem(rng, 600, n, interaction=inter)
+    r, it, cv = oracle.solve_residuals(y, refs, n, w, z, tol=1e-13, maxiter=100000)
+    np.testing.assert_allclose(r, _dense_wls_residuals(refs, n, z, w, y), rtol=1e-8, atol=1e-9)
+
+
+def test_single_fe_closed_form():
+    rng = np.random.default_rng(13)
+    refs, n, z, w, y = _rand_problem(rng, 2000, (90,), interaction=(0,))
+    r, it, cv = oracle.solve_residuals(y, refs, n, w, z)
+    zz, k = z[0], refs[0] - 1
+    num = np.bincount(k, weights=w * zz * y, minlength=n[0])
+    den = np.bincount(k, weights=w * zz * zz, minlength=n[0])
+    np.testing.assert_allclose(r, y - zz * (num / den)[k], rtol=1e-11, atol=1e-12)
+    assert it == 1 and cv
+
+
+def test_residual_invariants():
+    rng = np.random.default_rng(14)
+    refs, n, z, w, y = _rand_problem(rng, 5000, (500, 60, 5))
+    kw = dict(tol=1e-12, maxiter=100000)
+    r, _, _ = oracle.solve_residuals(y, refs, n, w, z, **kw)
+    # normal equations: A'(sqrt(w) r) = 0
+    s = oracle.OracleSolver(refs, n, z, w)
+    g = s.mul_At(np.sqrt(w) * r)
+    assert np.abs(g).max() <= 1e-9 * np.linalg.norm(np.sqrt(w) * y)
+    # idempotence
+    r2, _, _ = oracle.solve_residuals(r, refs, n, w, z, **kw)
+    np.testing.assert_allclose(r2, r, rtol=1e-8, atol=1e-9)
+    # linearity
+    y2 = rng.standard_normal(len(y))
+    ra, _, _ = oracle.solve_residuals(2 * y - 3 * y2, refs, n, w, z, **kw)
+    rb, _, _ = oracle.solve_residuals(y2, refs, n, w, z, **kw)
+    np.testing.assert_allclose(ra, 2 * r - 3 * rb, rtol=1e-7, atol=1e-8)
+    # weights scale invariance
+    rc, _, _ = oracle.solve_residuals(y, refs, n, 7.5 * w, z, **kw)
+    np.testing.assert_allclose(rc, r, rtol=1e-8, atol=1e-9)
+    # row permutation invariance
+    pm = rng.permutation(len(y))
+    rp, _, _ = oracle.solve_residuals(y[pm], [q[pm] for q in refs], n, w[pm], z, **kw)
+    np.testing.assert_allclose(rp, r[pm], rtol=1e-8, atol=1e-9)
+
+
+# -- components / normalize! / full ------------------------------------------
+
+def _partition(labels_per_fe):
+    return sorted(tuple(sorted((g, k) for g, lb in enumerate(labels_per_fe) for k in np.flatnonzero(lb == c)))
+                  for c in set(np.concatenate(labels_per_fe).tolist()) if c >= 0)
+
+
+def test_components_match_scipy():
+    rng = np.random.default_rng(15)
+    N, n = 400, (120, 90)
+    refs = [rng.integers(1, k + 1, N).astype(np.int64) for k in n]
+    nc, labels = oracle.components(refs, n)
+    offs = [0, n[0]]
+    A = sp.coo_matrix((np.ones(N), (refs[0] - 1, refs[1] - 1 + offs[1])), shape=(sum(n), sum(n)))
+    ncs, lab = connected_components(A, directed=False)
+    seen = [np.isin(np.arange(1, k + 1), r) for k, r in zip(n, refs)]
+    sl = [np.where(seen[g], lab[offs[g]:offs[g] + n[g]], -1) for g in range(2)]
+    assert _partition(labels) == _partition(sl)
+    assert nc == len(_partition(sl))
+
+
+def test_normalize_properties():
+    rng = np.random.default_rng(16)
+    N, n = 300, (100, 80, 5)
+    refs = [rng.integers(1, k + 1, N).astype(np.int64) for k in n]
+    coefs = [rng.standard_normal(k) for k in n]
+    fitted = sum(c[r - 1] for c, r in zip(coefs, refs))
+    out = oracle.normalize([c.copy() for c in coefs], refs, n, [True, True, True])
+    # fitted values unchanged
+    np.testing.assert_allclose(sum(c[r - 1] for c, r in zip(out, refs)), fitted, rtol=1e-12, atol=1e-12)
+    # within each component, FEs 2..G have mean zero over their observed levels
+    nc, labels = oracle.components(refs, n)
+    for g in (1, 2):
+        for c in range(nc):
+            m = labels[g] == c
+            if m.any():
+                assert abs(out[g][m].mean()) < 1e-12
+    # an interacted FE is left alone, and with < 2 unit FEs nothing changes
+    out2 = oracle.normalize([c.copy() for c in coefs], refs, n, [True, False, False])
+    for a, b in zip(out2, coefs):
+        np.testing.assert_array_equal(a, b)
