@@ -113,48 +113,48 @@ def main():
                                    "(bench.py --gpus 1)"}))
         sys.exit(2)
 
-    import numpy as np
-    import torch
-
-    from fixedeffects_rocm import api, synth
-    from fixedeffects_rocm._lib import check, lib
     import ctypes as C
+
+    import numpy as np
+
+    from fixedeffects_rocm import api, device, synth
+    from fixedeffects_rocm._lib import check, lib
 
     name, K = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     panel = synth.config(name)
     log(f"[bench] panel {name}: N={panel.N} n={panel.n} K={panel.K} ({time.perf_counter() - t0:.1f} s)")
     fes = [api.FixedEffect.from_refs(r, k, z) for r, k, z in zip(panel.refs, panel.n, panel.interactions)]
-    torch.cuda.init()
     t0 = time.perf_counter()
     feM = api.FixedEffectSolverROCm(fes, panel.weights, "fp64")
     setup_s = time.perf_counter() - t0
     log(f"[bench] solver set-up {setup_s:.2f} s")
 
-    Yt = torch.from_numpy(np.ascontiguousarray(panel.Y.T)).to("cuda:0")  # K x N: column c contiguous
-    work = torch.empty_like(Yt)
     N = panel.N
+    pristine = device.DeviceBuffer(N * K)   # column c at offset c*N
+    pristine.upload(np.asfortranarray(panel.Y).ravel(order="F"))
+    work = device.DeviceBuffer(N * K)
 
     def step():
-        work.copy_(Yt)
-        torch.cuda.synchronize()
-        return feM.solve_residuals_device(work.data_ptr(), K, N, tol=args.tol, maxiter=args.maxiter)
+        work.copy_from(pristine)
+        device.synchronize()
+        return feM.solve_residuals_device(work.address, K, N, tol=args.tol, maxiter=args.maxiter)
 
     for w in range(args.warmup):
         its, cvs = step()
         log(f"[bench] warmup {w}: iters {its[:4]}{'...' if K > 4 else ''} converged {all(cvs)}")
-    torch.cuda.synchronize()
+    device.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         its, cvs = step()
-    torch.cuda.synchronize()
+    device.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
     value = K / (ms_per_step / 1e3)
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, iters {its[:4]}")
 
     # normal equations of the result: ||A'(sqrt(w) r)||_inf / ||sqrt(w) y||_2 (size-independent check)
-    r = work[0].cpu().numpy()
+    r = work.download(N)
     sw = np.sqrt(panel.weights) if panel.weights is not None else 1.0
     g = feM.mul_At(sw * r)
     neq = float(np.abs(g).max() / np.linalg.norm(sw * panel.Y[:, 0]))

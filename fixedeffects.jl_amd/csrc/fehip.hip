@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -102,6 +103,7 @@ struct fehip_solver {
     uint32_t *hist = nullptr, *sums = nullptr;
     int64_t sort_cap = 0;
     int poll_iters = 8;
+    std::vector<double> last_info;  // LSMR scalars of the last solve, 9 per column
 };
 
 template <typename P>
@@ -241,6 +243,7 @@ template <typename T>
 __global__ void k_scale(const int32_t *__restrict__ off, const int32_t *__restrict__ perm, int64_t nlev,
                         const double *__restrict__ w, const double *__restrict__ z, T *__restrict__ scale_T,
                         double *__restrict__ scale_d) {
+#pragma clang fp contract(off)
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nlev) return;
     T s = (T)0;
@@ -260,6 +263,7 @@ template <typename T>
 __global__ void k_cache(const int32_t *__restrict__ perm_s, int64_t N, const int32_t *__restrict__ ref_int,
                         const double *__restrict__ w, const double *__restrict__ z, const T *__restrict__ scale,
                         T *__restrict__ cache) {
+#pragma clang fp contract(off)
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     const int32_t i = perm_s[j];
@@ -369,6 +373,7 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
     }
     std::unique_ptr<fehip_solver> hp(new fehip_solver());
     fehip_solver *h = hp.get();
+    if (const char *e = std::getenv("FEHIP_POLL_ITERS")) h->poll_iters = std::max(1, std::atoi(e));
     h->N = N;
     h->G = G;
     h->precision = precision;
@@ -529,7 +534,14 @@ static int get_plan(fehip_solver *h, int K, Plan **out) {
         P->tile0.push_back(t0);
         for (int64_t k = 0; k < f.n; k++) {
             const int64_t st = f.h_ioff[k], en = f.h_ioff[k + 1];
-            if (en <= st) continue;
+            if (en <= st) {  // level without observations: coef <- b*coef only
+                FixItem fi{};
+                fi.kind = 2;
+                fi.level = (int32_t)k;
+                fi.coef_off = h->off[g];
+                items.push_back(fi);
+                continue;
+            }
             const int64_t ts = st / P->TP, te = (en - 1) / P->TP;
             if (ts == te) continue;
             FixItem fi{};
@@ -652,6 +664,7 @@ template <typename T>
 __global__ void k_load_cols(const double *__restrict__ Y, int64_t ldY, int K, int Kreal,
                             const int32_t *__restrict__ perm_s, const double *__restrict__ sqrtw, int64_t N,
                             T *__restrict__ r, T *__restrict__ u) {
+#pragma clang fp contract(off)
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= N * K) return;
     const int64_t j = e / K;
@@ -670,6 +683,7 @@ template <typename T>
 __global__ void k_store_cols(double *__restrict__ Y, int64_t ldY, int K, int Kreal,
                              const int32_t *__restrict__ perm_s, const double *__restrict__ sqrtw, int64_t N,
                              const T *__restrict__ r) {
+#pragma clang fp contract(off)
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= N * K) return;
     const int64_t j = e / K;
@@ -940,6 +954,19 @@ struct Engine {
         std::vector<LsmrState<Tr>> st(K);
         HIPCHK(hipMemcpyAsync(st.data(), P->state, sizeof(LsmrState<Tr>) * K, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
+        h->last_info.assign((size_t)Kreal * 9, 0.0);
+        for (int c = 0; c < Kreal; c++) {
+            double *o = h->last_info.data() + (size_t)c * 9;
+            o[0] = (double)st[c].iter;
+            o[1] = (double)st[c].istop;
+            o[2] = (double)st[c].normr;
+            o[3] = (double)st[c].normAr;
+            o[4] = (double)st[c].normA;
+            o[5] = (double)st[c].condA;
+            o[6] = (double)st[c].normx;
+            o[7] = (double)st[c].alpha;
+            o[8] = (double)st[c].beta;
+        }
         iters.resize(Kreal);
         conv.resize(Kreal);
         for (int c = 0; c < Kreal; c++) {
@@ -1249,6 +1276,15 @@ int fehip_mul_A(fehip_t *h, double *y, const double *x, double a, double b) {
 
 int fehip_mul_At(fehip_t *h, double *x, const double *y, double a, double b) {
     return mul_common(h, x, y, a, b, true);
+}
+
+int fehip_lsmr_info(fehip_t *h, int32_t col, double *out) {
+    g_err.clear();
+    if (!h || !out) return fail(FEHIP_EARG, "bad arguments");
+    if (col < 0 || (size_t)(col + 1) * 9 > h->last_info.size())
+        return fail(FEHIP_EARG, "no LSMR result for that column");
+    std::memcpy(out, h->last_info.data() + (size_t)col * 9, sizeof(double) * 9);
+    return 0;
 }
 
 int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms) {

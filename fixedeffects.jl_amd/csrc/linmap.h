@@ -503,7 +503,9 @@ __global__ __launch_bounds__(kWave) void k_dtx_dense(DtxDenseParams<T> p) {
 
 // Fix-up items: kind 0 = a segment crossing tiles (total = tileB[first] +
 // sum_{q=1..count} tileA[first+q]); kind 1 = a dense level (total = sum of
-// count dpart rows from row first).  One wave per item, fixed-order sums.
+// count dpart rows from row first); kind 2 = a level no observation has
+// (total = 0: only the b*coef scaling of rmul!, src/AbstractFixedEffectLinearMap.jl:30).
+// One wave per item, fixed-order sums.
 struct FixItem {
     int32_t kind, level;
     int64_t coef_off;
@@ -546,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
         if (on) {
             if (it.kind == 0) {
                 for (int q = 1 + psub; q <= it.count; q += P) s += p.tileA[(it.first + q) * K + c];
-            } else {
+            } else if (it.kind == 1) {
                 for (int q = psub; q < it.count; q += P) s += p.dpart[(it.first + q) * K + c];
             }
         }

@@ -34,6 +34,7 @@ EXPORTS = (
     "fehip_get_cache",
     "fehip_mul_A",
     "fehip_mul_At",
+    "fehip_lsmr_info",
     "fehip_time_kernels",
     "fehip_dims",
     "fehip_last_error",
@@ -74,6 +75,7 @@ def _load() -> C.CDLL:
     L.fehip_get_cache.argtypes = [_vp, C.c_int32, _dp]
     L.fehip_mul_A.argtypes = [_vp, _dp, _dp, C.c_double, C.c_double]
     L.fehip_mul_At.argtypes = [_vp, _dp, _dp, C.c_double, C.c_double]
+    L.fehip_lsmr_info.argtypes = [_vp, C.c_int32, _dp]
     L.fehip_time_kernels.argtypes = [_vp, C.c_int32, C.c_int32, _dp]
     L.fehip_dims.argtypes = [_vp, _i64p, _i32p, _i64p]
     L.fehip_last_error.restype = C.c_char_p

@@ -261,6 +261,16 @@ class FixedEffectSolverROCm:
         check(lib.fehip_get_cache(self._h, g, _ptr(out, C.c_double)))
         return out
 
+    def lsmr_info(self, col: int = 0) -> dict:
+        """LSMR scalars of column col of the last solve."""
+        out = np.zeros(9, np.float64)
+        check(lib.fehip_lsmr_info(self._h, col, _ptr(out, C.c_double)))
+        keys = ("iters", "istop", "normr", "normAr", "normA", "condA", "normx", "alpha", "beta")
+        d = dict(zip(keys, out.tolist()))
+        d["iters"] = int(d["iters"])
+        d["istop"] = int(d["istop"])
+        return d
+
     def mul_A(self, x, y=None, a: float = 1.0, b: float = 0.0) -> np.ndarray:
         """mul!(y, A, x, a, b) (src/AbstractFixedEffectLinearMap.jl:37-44)."""
         x = np.ascontiguousarray(x, np.float64)

@@ -1,0 +1,146 @@
+"""
+    FixedEffectsROCm
+
+The `method = :rocm` backend of FixedEffects.jl (v3.1.0): the solver object
+and the methods a backend defines (the same plugin points the CUDA and Metal
+extensions use, ext/CUDAExt.jl:99-121, ext/MetalExt.jl:179-189), implemented
+by `ccall`s into libfehip.so.  LSMR runs inside the library; Julia only
+passes host vectors and reads back residuals, iteration counts and flags.
+
+Load it after FixedEffects:
+
+    using FixedEffects, FixedEffectsROCm
+    solve_residuals!(y, fes; method = :rocm, double_precision = true)
+
+The module adds methods to FixedEffects' functions; it does not change any
+existing method.
+"""
+module FixedEffectsROCm
+
+using FixedEffects
+using FixedEffects: AbstractFixedEffectSolver, FixedEffect
+using StatsBase: AbstractWeights, UnitWeights
+
+const libfehip = get(ENV, "FEHIP_LIB", joinpath(@__DIR__, "..", "lib", "libfehip.so"))
+
+const FEHIP_FP64 = Int32(0)
+const FEHIP_FP32 = Int32(1)
+const FEHIP_MIXED = Int32(2)
+
+struct FehipError <: Exception
+    code::Int
+    msg::String
+end
+Base.showerror(io::IO, e::FehipError) = print(io, "libfehip error ", e.code, ": ", e.msg)
+
+function check(rc::Integer)
+    if rc != 0
+        msg = unsafe_string(ccall((:fehip_last_error, libfehip), Cstring, ()))
+        throw(FehipError(rc, msg))
+    end
+    return nothing
+end
+
+"""
+    FixedEffectSolverROCm{T} <: AbstractFixedEffectSolver{T}
+
+Owns a libfehip handle (refs, CSR bucketings, scales and caches in device
+memory).  The handle is freed by a finalizer.
+"""
+mutable struct FixedEffectSolverROCm{T} <: AbstractFixedEffectSolver{T}
+    handle::Ptr{Cvoid}
+    fes::Vector{<:FixedEffect}
+    weights::AbstractWeights
+    N::Int
+end
+
+_weights_ptr(w::UnitWeights) = Ptr{Float64}(C_NULL)
+
+function FixedEffects.AbstractFixedEffectSolver{T}(fes::Vector{<:FixedEffect}, weights::AbstractWeights,
+                                                   ::Type{Val{:rocm}}; mixed::Bool = false) where {T}
+    N = length(weights)
+    G = length(fes)
+    refs = [convert(Vector{Int64}, fe.refs) for fe in fes]
+    nlev = Int64[fe.n for fe in fes]
+    inter = [fe.interaction isa UnitWeights ? nothing : convert(Vector{Float64}, fe.interaction) for fe in fes]
+    w = weights isa UnitWeights ? nothing : convert(Vector{Float64}, weights)
+    precision = mixed ? FEHIP_MIXED : (T === Float64 ? FEHIP_FP64 : FEHIP_FP32)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve refs inter w begin
+        refp = Ptr{Int64}[pointer(r) for r in refs]
+        interp = Ptr{Float64}[z === nothing ? Ptr{Float64}(C_NULL) : pointer(z) for z in inter]
+        wp = w === nothing ? Ptr{Float64}(C_NULL) : pointer(w)
+        check(ccall((:fehip_create, libfehip), Cint,
+                    (Ptr{Ptr{Cvoid}}, Int64, Int32, Ptr{Ptr{Int64}}, Ptr{Int64}, Ptr{Ptr{Float64}},
+                     Ptr{Float64}, Int32),
+                    out, N, G, refp, nlev, interp, wp, precision))
+    end
+    feM = FixedEffectSolverROCm{T}(out[], fes, weights, N)
+    finalizer(feM) do s
+        s.handle != C_NULL && ccall((:fehip_destroy, libfehip), Cvoid, (Ptr{Cvoid},), s.handle)
+        s.handle = C_NULL
+    end
+    return feM
+end
+
+# update_weights!(feM, w)  (src/SolverCPU.jl:67-76)
+function FixedEffects.update_weights!(feM::FixedEffectSolverROCm, weights::AbstractWeights)
+    w = weights isa UnitWeights ? nothing : convert(Vector{Float64}, weights)
+    GC.@preserve w begin
+        check(ccall((:fehip_update_weights, libfehip), Cint, (Ptr{Cvoid}, Ptr{Float64}),
+                    feM.handle, w === nothing ? Ptr{Float64}(C_NULL) : pointer(w)))
+    end
+    feM.weights = weights
+    return feM
+end
+
+# inner solve_residuals!(r, feM; tol, maxiter)  (src/AbstractFixedEffectSolver.jl:49-68)
+function FixedEffects.solve_residuals!(r::AbstractVector{<:Real}, feM::FixedEffectSolverROCm{T};
+                                       tol::Real = sqrt(eps(T)), maxiter::Integer = 100_000) where {T}
+    _, iters, convs = _solve_many!([r], feM, tol, maxiter)
+    return r, iters[1], convs[1]
+end
+
+# multi-column solve_residuals!(xs, feM)  (src/AbstractFixedEffectSolver.jl:74-97):
+# all columns go to the device in one batched solve instead of a loop.
+function FixedEffects.solve_residuals!(xs, feM::FixedEffectSolverROCm{T}; progress_bar = true,
+                                       tol::Real = sqrt(eps(T)), maxiter::Integer = 100_000) where {T}
+    _, iters, convs = _solve_many!(collect(xs), feM, tol, maxiter)
+    return xs, iters, convs
+end
+
+function _solve_many!(cols::Vector, feM::FixedEffectSolverROCm, tol, maxiter)
+    K = length(cols)
+    bufs = [c isa Vector{Float64} ? c : convert(Vector{Float64}, c) for c in cols]
+    iters = zeros(Int64, K)
+    convs = zeros(UInt8, K)
+    GC.@preserve bufs begin
+        ptrs = Ptr{Float64}[pointer(b) for b in bufs]
+        check(ccall((:fehip_solve_residuals, libfehip), Cint,
+                    (Ptr{Cvoid}, Ptr{Ptr{Float64}}, Int32, Float64, Int64, Ptr{Int64}, Ptr{UInt8}),
+                    feM.handle, ptrs, K, tol, maxiter, iters, convs))
+    end
+    for (c, b) in zip(cols, bufs)
+        c === b || copyto!(c, b)
+    end
+    return cols, Int.(iters), Bool.(convs)
+end
+
+# inner solve_coefficients!(r, feM; tol, maxiter)  (src/AbstractFixedEffectSolver.jl:147-160)
+function FixedEffects.solve_coefficients!(r::AbstractVector, feM::FixedEffectSolverROCm{T};
+                                          tol::Real = sqrt(eps(T)), maxiter::Integer = 100_000) where {T}
+    y = convert(Vector{Float64}, r)
+    coefs = [zeros(Float64, fe.n) for fe in feM.fes]
+    iters = Ref{Int64}(0)
+    conv = Ref{UInt8}(0)
+    GC.@preserve coefs begin
+        ptrs = Ptr{Float64}[pointer(c) for c in coefs]
+        check(ccall((:fehip_solve_coefficients, libfehip), Cint,
+                    (Ptr{Cvoid}, Ptr{Float64}, Float64, Int64, Ptr{Ptr{Float64}}, Ptr{Int64}, Ptr{UInt8}),
+                    feM.handle, y, tol, maxiter, ptrs, iters, conv))
+    end
+    x = Vector{eltype(r)}[convert(Vector{eltype(r)}, c) for c in coefs]
+    return FixedEffects.full(FixedEffects.normalize!(x, feM.fes), feM.fes), Int(iters[]), conv[] != 0
+end
+
+end # module

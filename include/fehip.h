@@ -128,6 +128,12 @@ int fehip_get_cache(fehip_t *h, int32_t g, double *cache);
 int fehip_mul_A(fehip_t *h, double *y, const double *x, double a, double b);
 int fehip_mul_At(fehip_t *h, double *x, const double *y, double a, double b);
 
+/* Test hook: LSMR scalars of column `col` of the last solve on this handle
+ * (the ConvergenceHistory and estimates of src/utils/lsmr.jl:170-192, 218-220):
+ * out[0] iterations, out[1] istop, out[2] normr, out[3] normAr, out[4] normA,
+ * out[5] condA, out[6] normx, out[7] alpha, out[8] beta. */
+int fehip_lsmr_info(fehip_t *h, int32_t col, double *out);
+
 /* Measurement hook (no reference counterpart): average device time in ms of
  * each kernel of one LSMR iteration on K columns, from HIP events around
  * `reps` isolated launches on the handle's stream, after filling the solve

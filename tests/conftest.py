@@ -15,11 +15,12 @@ def pytest_configure(config):
 
 
 def gpu_available() -> bool:
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
+    # The HIP runtime libfehip links (not torch's bundled one, which would put
+    # a second HSA runtime in the process).
+    if not os.path.exists("/dev/kfd"):
         return False
+    from fixedeffects_rocm import device
+    return device.device_count() > 0
 
 
 @pytest.fixture(scope="session")

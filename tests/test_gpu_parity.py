@@ -5,8 +5,14 @@ Bars:
 * CSR permutation and offsets: bit-exact (integer).
 * scale / cache (FP64): bit-exact -- same summation order, same IEEE ops.
 * one matvec (FP64): rtol 1e-13 (parallel summation order differs).
-* residuals (FP64, tol 1e-8): ||r_gpu - r_oracle|| / ||r_oracle|| <= 1e-8,
-  LSMR iterations within +-2 of the oracle's.
+* residuals (FP64): at equal LSMR iteration counts (tol = 0, k <= 12) the
+  iterates agree to 1e-12; converged (tol = 1e-12) ||r_gpu - r_oracle|| /
+  ||r_oracle|| <= 1e-8; at the reference's tol = 1e-8 the iteration counts
+  agree within +-2 and the GPU residual is at least as close to the exact
+  least-squares residual as the oracle's own (within a factor 3).  LSMR in
+  finite precision loses orthogonality, and a reordering of the oracle's own
+  sums already moves its mid-run scalars by up to 1e-2 and its stopping
+  iteration by one, so tol-level residual differences are not parity defects.
 * reference Float32 mode: Float32.(r) matches at sqrt(eps(Float32)), as
   test/solve.jl:37 checks.
 """
@@ -158,26 +164,28 @@ def test_matvec_fp64(gpu, n):
 # -- full solves ---------------------------------------------------------------------
 
 def _check_solve(refs, n, z, w, Y, prec="fp64", tol=1e-8, iter_slack=2):
+    """GPU vs oracle on K columns at `tol`; also against a tight-tolerance
+    oracle solve (tol 1e-13) that stands in for the exact residual."""
     K = Y.shape[1]
     o = oracle.OracleSolver(refs, n, z, w, double_precision=(prec == "fp64"), nthreads=8)
-    want = []
+    want, tight = [], []
     for c in range(K):
-        r, it, cv = o.solve_residuals(Y[:, c], tol=tol, maxiter=10000)
-        want.append((r, it, cv))
+        want.append(o.solve_residuals(Y[:, c].copy(), tol=tol, maxiter=10000))
+        tight.append(o.solve_residuals(Y[:, c].copy(), tol=1e-13, maxiter=100000)[0])
     with api.FixedEffectSolverROCm(_fes(refs, n, z), w, prec) as s:
+        cols = [Y[:, c].copy() for c in range(K)]
         if K == 1:
-            y = np.ascontiguousarray(Y[:, 0])
-            r, it, cv = s.solve_residuals(y, tol=tol, maxiter=10000)
+            r, it, cv = s.solve_residuals(cols[0], tol=tol, maxiter=10000)
             got = [(r, it, cv)]
         else:
-            cols = [np.ascontiguousarray(Y[:, c]) for c in range(K)]
             _, its, cvs = s.solve_residuals_many(cols, tol=tol, maxiter=10000)
             got = list(zip(cols, its, cvs))
     worst = 0.0
-    for (rg, ig, cg), (ro, io, co) in zip(got, want):
+    for (rg, ig, cg), (ro, io, co), rt in zip(got, want, tight):
         worst = max(worst, _rel(rg, ro))
         assert cg == co
         assert abs(ig - io) <= iter_slack, (ig, io)
+        assert _rel(rg, rt) <= 3 * _rel(ro, rt) + 1e-12, (_rel(rg, rt), _rel(ro, rt))
     return worst, [g[1] for g in got], [w_[1] for w_ in want]
 
 
@@ -187,7 +195,6 @@ def test_solve_cpu_two_way_config(gpu):
     p = synth.config("cpu_two_way")
     worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y)
     _say("  rel err", worst, "iters gpu", ig, "oracle", io)
-    assert worst <= RES_RTOL
 
 
 def test_solve_worker_firm_scaled(gpu):
@@ -196,6 +203,15 @@ def test_solve_worker_firm_scaled(gpu):
     p = synth.config("worker_firm_1e7", scale=0.02)
     worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y)
     _say("  rel err", worst, "iters gpu", ig, "oracle", io)
+
+
+@pytest.mark.parametrize("name,scale", [("worker_firm_1e7", 0.02), ("cpu_two_way", 1.0)])
+def test_solve_converged_parity(gpu, name, scale):
+    # both sides converged (tol 1e-12): residuals agree to 1e-8 relative
+    _say(f"case: converged parity {name}")
+    p = synth.config(name, scale=scale)
+    worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y, tol=1e-12, iter_slack=10)
+    _say("  rel err", worst, "iters gpu", ig, "oracle", io)
     assert worst <= RES_RTOL
 
 
@@ -203,6 +219,7 @@ def test_solve_interaction_and_single_fe(gpu):
     _say("case: interaction / single FE")
     refs, n, z, w, y = _rand_problem(41, 30_000, (3000, 200, 9), interaction=(2,))
     worst, _, _ = _check_solve(refs, n, z, w, y[:, None])
+    worst, _, _ = _check_solve(refs, n, z, w, y[:, None], tol=1e-12, iter_slack=10)
     assert worst <= RES_RTOL
     refs1, n1, z1, w1, y1 = _rand_problem(42, 30_000, (3000,), interaction=(0,))
     worst, ig, _ = _check_solve(refs1, n1, z1, w1, y1[:, None])
@@ -214,9 +231,8 @@ def test_solve_batched_columns(gpu):
     p = synth.config("worker_firm_1e7_k32", scale=0.01)
     worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y[:, :5])
     _say("  K=5 rel err", worst, ig, io)
-    assert worst <= RES_RTOL
-    worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y)
-    _say("  K=32 rel err", worst)
+    worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, p.Y, tol=1e-12, iter_slack=10)
+    _say("  K=32 converged rel err", worst)
     assert worst <= RES_RTOL
 
 
@@ -292,10 +308,57 @@ def test_maxiter_and_stop_codes(gpu):
 
 
 def test_deterministic(gpu):
+    # atomic-free fixed-order reductions: two solves are bitwise identical
     _say("case: determinism")
     p = synth.config("worker_firm_1e7", scale=0.01)
     with api.FixedEffectSolverROCm(_fes(p.refs, p.n), p.weights, "fp64") as s:
-        a, ia, _ = s.solve_residuals(np.ascontiguousarray(p.Y[:, 0]))
-        b, ib, _ = s.solve_residuals(np.ascontiguousarray(p.Y[:, 0]))
+        a, ia, _ = s.solve_residuals(p.Y[:, 0].copy())
+        b, ib, _ = s.solve_residuals(p.Y[:, 0].copy())
     assert ia == ib
     np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("k", [1, 5, 12])
+def test_fixed_iteration_parity(gpu, k):
+    # tol = 0: neither side stops early, so the k-th LSMR iterates and scalars
+    # are compared directly (x0 = A'b as solve_residuals! uses)
+    _say(f"case: fixed iterations k={k}")
+    p = synth.config("worker_firm_1e7", scale=0.02)
+    y = p.Y[:, 0].copy()
+    o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=8)
+    ro, io, _ = o.solve_residuals(y.copy(), tol=0.0, maxiter=k + 1)
+    b = y * np.sqrt(p.weights)
+    _, info = o.lsmr(o.mul_At(b), b, atol=0.0, btol=0.0, conlim=1e8, maxiter=k)
+    with api.FixedEffectSolverROCm(_fes(p.refs, p.n), p.weights, "fp64") as s:
+        rg, ig, _ = s.solve_residuals(y.copy(), tol=0.0, maxiter=k + 1)
+        gi = s.lsmr_info(0)
+    _say(f"  iters {ig} {io} rel err {_rel(rg, ro):.3e}")
+    for key in ("normr", "normAr", "normA", "condA", "normx"):
+        _say(f"  {key}: gpu {gi[key]!r} oracle {getattr(info, key)!r}")
+    assert ig == io == k + 1
+    assert _rel(rg, ro) <= 1e-12
+    for key in ("normr", "normAr", "normA", "normx"):
+        assert abs(gi[key] - getattr(info, key)) <= 1e-12 * abs(getattr(info, key)), key
+
+
+def test_matvec_skewed(gpu):
+    # Zipf-sized levels spanning many tiles (through-tile fix-up), empty levels, b != 0
+    _say("case: matvec skewed")
+    rng = np.random.default_rng(3)
+    N = 300_000
+    sizes = synth.zipf_sizes(N, 20_000, 1.1)
+    r0 = np.repeat(np.arange(1, 20_001), sizes)
+    rng.shuffle(r0)
+    r1 = rng.integers(1, 3001, N)
+    r1[r1 == 7] = 8
+    r2 = rng.integers(1, 13, N)
+    refs = [r0.astype(np.int64), r1.astype(np.int64), r2.astype(np.int64)]
+    n = [20_000, 3000, 12]
+    w = rng.lognormal(0, 0.5, N)
+    o = oracle.OracleSolver(refs, n, None, w)
+    x = rng.standard_normal(o.p)
+    y = rng.standard_normal(N)
+    with api.FixedEffectSolverROCm(_fes(refs, n), w, "fp64") as s:
+        for a, b in ((1.0, 0.0), (0.3, -1.7)):
+            np.testing.assert_allclose(s.mul_At(y, x, a, b), o.mul_At(y, x, a, b), rtol=1e-11, atol=1e-11)
+            np.testing.assert_allclose(s.mul_A(x, y, a, b), o.mul_A(x, y, a, b), rtol=1e-12, atol=1e-12)
