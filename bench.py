@@ -160,9 +160,9 @@ def main():
     neq = float(np.abs(g).max() / np.linalg.norm(sw * panel.Y[:, 0]))
 
     # per-kernel durations (HIP events on the solver's stream)
-    ms = np.zeros(5)
+    ms = np.zeros(8)
     check(lib.fehip_time_kernels(feM._h, K, 20, ms.ctypes.data_as(C.POINTER(C.c_double))))
-    names = ["k_db", "k_dtx_seg", "k_dtx_dense", "k_dtx_fixup", "k_update"]
+    names = ["k_db", "k_fin_db", "k_dtx_seg", "k_dtx_dense", "k_dtx_fixup", "k_fin_dtx", "k_update", "k_fin_update"]
     kinds = []
     s_max = int(np.argmax(panel.n))
     for gi, k in enumerate(panel.n):
@@ -171,7 +171,7 @@ def main():
     kt = {nm: float(m) for nm, m in zip(names, ms)}
     dom = max(["k_db", "k_dtx_seg", "k_dtx_dense", "k_update"], key=lambda q: kt[q])
     achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
-    pair_ms = kt["k_db"] + kt["k_dtx_seg"] + kt["k_dtx_dense"] + kt["k_dtx_fixup"]
+    pair_ms = kt["k_db"] + kt["k_fin_db"] + kt["k_dtx_seg"] + kt["k_dtx_dense"] + kt["k_dtx_fixup"] + kt["k_fin_dtx"]
     pair_gbs = kb["pair"] / (pair_ms * 1e-3) / 1e9
     iters = int(np.max(its)) - 1  # LSMR iterations (solve_residuals! reports lsmr + 1)
 

@@ -11,6 +11,7 @@ constexpr int kBlock = 256;               // 4 waves per workgroup
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxG = 16;                 // fixed effects per handle
 constexpr int kMaxK = 32;                 // columns per batched launch
+constexpr int kFinBlock = 1024;           // threads of a finalize launch
 
 // What a reduction's last workgroup does with the column norms it forms.
 enum Mode : int {
@@ -81,19 +82,30 @@ __device__ __forceinline__ double block_column_sum(double v, double *red) {
     return s;
 }
 
-// Deterministic sum of nparts per-column partials laid out [part*K + c]
-// (several arrays concatenated through two spans).  Every thread of the block
-// takes part; thread c < K returns column c's total.  Fixed order: thread t
-// sums parts t/K, t/K + 256/K, ... of column t%K, then a fixed tree.
-template <int K>
+// Deterministic sum of per-column partials laid out [part*K + c] (two spans
+// of parts).  All NT threads of the block take part; thread c < K returns
+// column c's total.  Fixed order: thread t owns column t%K and parts
+// r = t/K, r + R, r + 2R, ... (R = NT/K) accumulated in four interleaved
+// chains, then a fixed tree over the R threads of the column.
+template <int K, int NT>
 __device__ double block_sum_partials(const double *p0, int64_t n0, const double *p1, int64_t n1,
-                                     double *red /* kBlock doubles */) {
-    constexpr int R = kBlock / K;  // threads per column
+                                     double *red /* NT doubles */) {
+    constexpr int R = NT / K;
     const int t = threadIdx.x, c = t % K, r = t / K;
-    double s = 0.0;
-    for (int64_t j = r; j < n0; j += R) s += p0[j * K + c];
-    for (int64_t j = r; j < n1; j += R) s += p1[j * K + c];
-    red[t] = s;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int span = 0; span < 2; span++) {
+        const double *p = span ? p1 : p0;
+        const int64_t n = span ? n1 : n0;
+        int64_t j = r;
+        for (; j + 3 * R < n; j += 4 * R) {
+            s0 += p[j * K + c];
+            s1 += p[(j + R) * K + c];
+            s2 += p[(j + 2 * R) * K + c];
+            s3 += p[(j + 3 * R) * K + c];
+        }
+        for (; j < n; j += R) s0 += p[j * K + c];
+    }
+    red[t] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     for (int w = R / 2; w >= 1; w >>= 1) {
         if (r < w) red[t] += red[t + w * K];
@@ -102,26 +114,6 @@ __device__ double block_sum_partials(const double *p0, int64_t n0, const double 
     double out = (t < K) ? red[t] : 0.0;
     __syncthreads();
     return out;
-}
-
-// Last-workgroup election: every block publishes its partials, then bumps a
-// device-scope counter; the block that sees count == gridDim.x - 1 is last and
-// may read every partial.  The counter is re-armed by the last block, so the
-// pattern replays inside a captured graph without a memset.
-__device__ __forceinline__ bool elect_last_block(unsigned int *counter, int *flag_lds) {
-    __threadfence();  // release this block's partials (device scope)
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int prev = atomicAdd(counter, 1u);
-        *flag_lds = (prev == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    bool last = *flag_lds != 0;
-    if (last) {
-        __threadfence();  // acquire the other blocks' partials
-        if (threadIdx.x == 0) atomicExch(counter, 0u);
-    }
-    return last;
 }
 
 }  // namespace fehip

@@ -284,16 +284,12 @@ struct DbParams {
     const T *coef;  // p x K
     T *target;      // N x K
     const ColOps *ops;
-    double *partials;  // gridDim.x x K
-    unsigned int *counter;
+    double *partials;  // gridDim.x x K (||target||^2 per column, MODE_INIT/ITER)
     const int32_t *all_done;  // early exit when set (iterating launches)
-    int32_t *all_done_w;
     int mode;
-    void *state;      // LsmrState<Tr>[K]
-    ColOps *ops_next; // written by the finalize (for the following D'x)
 };
 
-template <typename T, typename Tr, int K>
+template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
     if (p.all_done && *p.all_done) return;
     constexpr int P = kWave / K;
@@ -326,15 +322,8 @@ __global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
     }
     if (p.mode == MODE_NONE) return;
     __shared__ double red[kBlock];
-    __shared__ int lastflag;
     const double s = block_column_sum<K>(sq, red);
     if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
-    if (!elect_last_block(p.counter, &lastflag)) return;
-    const double tot = block_sum_partials<K>(p.partials, gridDim.x, nullptr, 0, red);
-    if (threadIdx.x < K) {
-        LsmrState<Tr> *st = (LsmrState<Tr> *)p.state;
-        fin_after_db<Tr>(st[threadIdx.x], tot, p.mode, threadIdx.x, p.ops_next);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -445,12 +434,13 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
 // FEs with few levels: one wave per chunk of observations (internal order,
 // fully sequential reads); every lane owns a private LDS accumulator column
 // per level, so adds never collide.  Per-chunk level sums go to dpart,
-// level-major, and are combined by the fix-up launch.
+// level-major ([level][chunk] rows of K), and are combined by the fix-up
+// launch in chunk order.  WPB waves share a workgroup, each with its own LDS.
 template <typename T>
 struct DenseFE {
     const int32_t *refs;
     const T *cache;
-    int32_t nlev, blk0, nblk, pad;
+    int32_t nlev, chunk0, nchunk, pad;
     int64_t coef_off;
     int64_t dpart_row0;  // first row (of K doubles) of this FE in dpart
 };
@@ -458,7 +448,7 @@ struct DenseFE {
 template <typename T>
 struct DtxDenseParams {
     int64_t N;
-    int32_t ndense, total_blocks, chunk, pad;
+    int32_t ndense, total_chunks, chunk, wpb;
     DenseFE<T> fe[kMaxG];
     const T *u;
     double *dpart;
@@ -467,17 +457,19 @@ struct DtxDenseParams {
 };
 
 template <typename T, typename Acc, int K>
-__global__ __launch_bounds__(kWave) void k_dtx_dense(DtxDenseParams<T> p) {
+__global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
     if (p.all_done && *p.all_done) return;
     extern __shared__ __align__(16) unsigned char dense_smem[];
-    Acc *acc = reinterpret_cast<Acc *>(dense_smem);
     constexpr int P = kWave / K;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int psub = lane / K, c = lane % K;
+    const int ch = blockIdx.x * p.wpb + wave;  // global chunk
+    if (wave >= p.wpb || ch >= p.total_chunks) return;
     int f = 0;
-    while (f + 1 < p.ndense && (int)blockIdx.x >= p.fe[f + 1].blk0) f++;
+    while (f + 1 < p.ndense && ch >= p.fe[f + 1].chunk0) f++;
     const DenseFE<T> fe = p.fe[f];
-    const int bl = blockIdx.x - fe.blk0;
+    Acc *acc = reinterpret_cast<Acc *>(dense_smem) + (size_t)wave * fe.nlev * kWave;
+    const int bl = ch - fe.chunk0;
     for (int k = 0; k < fe.nlev; k++) acc[k * kWave + lane] = Acc(0.f);
     const int on = p.ops->on[c];
     const int64_t start = (int64_t)bl * p.chunk;
@@ -492,12 +484,12 @@ __global__ __launch_bounds__(kWave) void k_dtx_dense(DtxDenseParams<T> p) {
             }
         }
     }
-    __syncthreads();
+    // each lane reads back only its own LDS column: no barrier is needed
     double *out = p.dpart + fe.dpart_row0 * K;
     for (int k = 0; k < fe.nlev; k++) {
         Acc x = acc[k * kWave + lane];
         x = column_allreduce<K>(x);
-        if (psub == 0) out[((int64_t)k * fe.nblk + bl) * K + c] = acc_to_double(x);
+        if (psub == 0) out[((int64_t)k * fe.nchunk + bl) * K + c] = acc_to_double(x);
     }
 }
 
@@ -520,18 +512,12 @@ struct FixParams {
     const double *tileA, *tileB, *dpart;
     T *v;
     const ColOps *ops;
-    double *partials;             // gridDim.x x K
-    const double *main_partials;  // from k_dtx_seg
-    int64_t n_main;
-    unsigned int *counter;
+    double *partials;  // gridDim.x x K
     const int32_t *all_done;
-    int32_t *all_done_w;
     int mode;
-    void *state;
-    ColOps *ops_next_db;
 };
 
-template <typename T, typename Tr, int K>
+template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
     if (p.all_done && *p.all_done) return;
     constexpr int P = kWave / K;
@@ -545,12 +531,21 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
     if (item < p.nitems) {
         const FixItem it = p.items[item];
         double s = 0.0;
-        if (on) {
-            if (it.kind == 0) {
-                for (int q = 1 + psub; q <= it.count; q += P) s += p.tileA[(it.first + q) * K + c];
-            } else if (it.kind == 1) {
-                for (int q = psub; q < it.count; q += P) s += p.dpart[(it.first + q) * K + c];
+        if (on && it.kind != 2) {
+            // rows [lo, hi) of `src`, lane-strided, four independent chains
+            const double *src = it.kind == 0 ? p.tileA : p.dpart;
+            const int64_t lo = it.kind == 0 ? it.first + 1 : it.first;
+            const int64_t hi = lo + it.count;
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int64_t q = lo + psub;
+            for (; q + 3 * P < hi; q += 4 * P) {
+                s0 += src[q * K + c];
+                s1 += src[(q + P) * K + c];
+                s2 += src[(q + 2 * P) * K + c];
+                s3 += src[(q + 3 * P) * K + c];
             }
+            for (; q < hi; q += P) s0 += src[q * K + c];
+            s = (s0 + s1) + (s2 + s3);
         }
         s = column_allreduce<K>(s);
         if (on && psub == 0) {
@@ -564,15 +559,8 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
     }
     if (p.mode == MODE_NONE) return;
     __shared__ double red[kBlock];
-    __shared__ int lastflag;
     const double s = block_column_sum<K>(sq, red);
     if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
-    if (!elect_last_block(p.counter, &lastflag)) return;
-    const double tot = block_sum_partials<K>(p.main_partials, p.n_main, p.partials, gridDim.x, red);
-    LsmrState<Tr> *st = (LsmrState<Tr> *)p.state;
-    if (threadIdx.x < K) fin_after_dtx<Tr>(st[threadIdx.x], tot, p.mode, threadIdx.x, p.ops_next_db);
-    __syncthreads();
-    if (p.mode == MODE_INIT) publish_all_done<K, Tr>(st, p.all_done_w);
 }
 
 // ---------------------------------------------------------------------------
@@ -587,17 +575,14 @@ struct UpdParams {
     int64_t pK;
     T *v, *h, *hbar, *x;
     double *partials;
-    unsigned int *counter;
     const int32_t *all_done;
-    int32_t *all_done_w;
-    void *state;
-    ColOps *ops_next_db;
+    const void *state;
 };
 
 template <typename T, typename Tr, int K>
 __global__ __launch_bounds__(kBlock) void k_update(UpdParams<T> p) {
     if (*p.all_done) return;
-    LsmrState<Tr> *st = (LsmrState<Tr> *)p.state;
+    const LsmrState<Tr> *st = (const LsmrState<Tr> *)p.state;
     const int c = threadIdx.x % K;  // stride is a multiple of K
     const bool act = !st[c].done;
     const bool vs = st[c].vskip != 0;
@@ -620,14 +605,50 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdParams<T> p) {
         }
     }
     __shared__ double red[kBlock];
-    __shared__ int lastflag;
     const double s = block_column_sum<K>(sq, red);
     if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
-    if (!elect_last_block(p.counter, &lastflag)) return;
-    const double tot = block_sum_partials<K>(p.partials, gridDim.x, nullptr, 0, red);
-    if (threadIdx.x < K) fin_after_update<Tr>(st[threadIdx.x], tot, threadIdx.x, p.ops_next_db);
+}
+
+// ---------------------------------------------------------------------------
+// Single-workgroup finalize launches: sum a reduction's per-workgroup partials
+// in index order and run the LSMR scalar step for each column.  A separate
+// launch (instead of a last-workgroup election inside the reduction) keeps
+// device-scope fences out of the streaming kernels: the kernel boundary
+// publishes the partials.
+// ---------------------------------------------------------------------------
+template <typename Tr, int K>
+__global__ __launch_bounds__(kFinBlock) void k_fin_db(const double *partials, int64_t nparts, int mode, void *state,
+                                                   ColOps *ops_next_dtx, const int32_t *all_done) {
+    if (all_done && *all_done) return;
+    __shared__ double red[kFinBlock];
+    const double tot = block_sum_partials<K, kFinBlock>(partials, nparts, nullptr, 0, red);
+    LsmrState<Tr> *st = (LsmrState<Tr> *)state;
+    if (threadIdx.x < K) fin_after_db<Tr>(st[threadIdx.x], tot, mode, threadIdx.x, ops_next_dtx);
+}
+
+template <typename Tr, int K>
+__global__ __launch_bounds__(kFinBlock) void k_fin_dtx(const double *p0, int64_t n0, const double *p1, int64_t n1,
+                                                    int mode, void *state, ColOps *ops_next_db, int32_t *all_done_w,
+                                                    const int32_t *all_done) {
+    if (all_done && *all_done) return;
+    __shared__ double red[kFinBlock];
+    const double tot = block_sum_partials<K, kFinBlock>(p0, n0, p1, n1, red);
+    LsmrState<Tr> *st = (LsmrState<Tr> *)state;
+    if (threadIdx.x < K) fin_after_dtx<Tr>(st[threadIdx.x], tot, mode, threadIdx.x, ops_next_db);
     __syncthreads();
-    publish_all_done<K, Tr>(st, p.all_done_w);
+    if (mode == MODE_INIT) publish_all_done<K, Tr>(st, all_done_w);
+}
+
+template <typename Tr, int K>
+__global__ __launch_bounds__(kFinBlock) void k_fin_update(const double *partials, int64_t nparts, void *state,
+                                                       ColOps *ops_next_db, int32_t *all_done) {
+    if (*all_done) return;
+    __shared__ double red[kFinBlock];
+    const double tot = block_sum_partials<K, kFinBlock>(partials, nparts, nullptr, 0, red);
+    LsmrState<Tr> *st = (LsmrState<Tr> *)state;
+    if (threadIdx.x < K) fin_after_update<Tr>(st[threadIdx.x], tot, threadIdx.x, ops_next_db);
+    __syncthreads();
+    publish_all_done<K, Tr>(st, all_done);
 }
 
 // LSMR set-up of the p-length vectors (:68, :77-78): v /= alpha; h = v; hbar = 0.

@@ -135,14 +135,17 @@ int fehip_mul_At(fehip_t *h, double *x, const double *y, double a, double b);
 int fehip_lsmr_info(fehip_t *h, int32_t col, double *out);
 
 /* Measurement hook (no reference counterpart): average device time in ms of
- * each kernel of one LSMR iteration on K columns, from HIP events around
- * `reps` isolated launches on the handle's stream, after filling the solve
- * vectors with constants:
+ * each launch of one LSMR iteration on K columns, from HIP events around
+ * `reps` isolated launches on the handle's stream, in the launch
+ * configuration the solve uses, with the solve vectors holding filler:
  *   out_ms[0] D*v          (the reference's scatter!, src/SolverCPU.jl:29-34)
- *   out_ms[1] D'u, FEs with many levels (gather!, src/SolverCPU.jl:21-27)
- *   out_ms[2] D'u, FEs with few levels
- *   out_ms[3] D'u fix-up of segments split across tiles
- *   out_ms[4] LSMR vector update (src/utils/lsmr.jl:138-142, :192) */
+ *   out_ms[1] beta = ||u|| finalize
+ *   out_ms[2] D'u, FEs with many levels (gather!, src/SolverCPU.jl:21-27)
+ *   out_ms[3] D'u, FEs with few levels
+ *   out_ms[4] D'u fix-up of segments split across tiles
+ *   out_ms[5] alpha = ||v|| and plane-rotation finalize
+ *   out_ms[6] LSMR vector update (src/utils/lsmr.jl:138-142, :192)
+ *   out_ms[7] ||x|| and stopping-rule finalize (one launch) */
 int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms);
 
 /* Dimensions of a handle. */
