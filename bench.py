@@ -32,6 +32,7 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     "worker_firm_1e7": ("worker_firm_1e7", 1),
     "worker_firm_1e7_k32": ("worker_firm_1e7_k32", 32),
+    "worker_firm_1e8_k16": ("worker_firm_1e8_k16", 16),
 }
 
 
@@ -107,6 +108,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--maxiter", type=int, default=10000)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the solve is not sharded across GPUs "
@@ -126,7 +128,7 @@ def main():
     log(f"[bench] panel {name}: N={panel.N} n={panel.n} K={panel.K} ({time.perf_counter() - t0:.1f} s)")
     fes = [api.FixedEffect.from_refs(r, k, z) for r, k, z in zip(panel.refs, panel.n, panel.interactions)]
     t0 = time.perf_counter()
-    feM = api.FixedEffectSolverROCm(fes, panel.weights, "fp64")
+    feM = api.FixedEffectSolverROCm(fes, panel.weights, args.precision)
     setup_s = time.perf_counter() - t0
     log(f"[bench] solver set-up {setup_s:.2f} s")
 
@@ -167,7 +169,7 @@ def main():
     s_max = int(np.argmax(panel.n))
     for gi, k in enumerate(panel.n):
         kinds.append("sorted" if gi == s_max else ("dense" if k <= 64 else "csr"))
-    kb = kernel_bytes(N, len(panel.n), panel.n, kinds, K, 8)
+    kb = kernel_bytes(N, len(panel.n), panel.n, kinds, K, 8 if args.precision == "fp64" else 4)
     kt = {nm: float(m) for nm, m in zip(names, ms)}
     dom = max(["k_db", "k_dtx_seg", "k_dtx_dense", "k_update"], key=lambda q: kt[q])
     achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
@@ -192,11 +194,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (seeded numpy PCG64 worker/firm/year panel)",
         "config": {"workload": name, "N": N, "levels": [int(k) for k in panel.n], "K": K, "tol": args.tol,
-                   "weights": "LogNormal(0,0.5)", "precision": "fp64", "mode": "plain LSMR, reference start"},
+                   "weights": "LogNormal(0,0.5)", "precision": args.precision,
+                   "mode": "plain LSMR, reference start" if args.precision == "fp64"
+                   else "FP32-stream LSMR phase, then FP64 LSMR iterations from its solution"},
         "lsmr_iters": [int(i) - 1 for i in its],
+        "normal_eq_note": "max_level |A'(sqrt(w) r)| / ||sqrt(w) y|| of column 0",
         "converged": bool(all(cvs)),
         "normal_eq_rel": neq,
         "setup_s": setup_s,

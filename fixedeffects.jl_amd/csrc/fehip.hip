@@ -41,6 +41,11 @@ static int fail(int code, const std::string &msg) {
     return code;
 }
 
+namespace fehip {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+int mlp_predict_device(fehip_mlp *m, const float *X_dev, int64_t n, float *y_dev);  // mlp.hip
+}  // namespace fehip
+
 #define HIPCHK(expr)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (expr);                                                               \
@@ -72,10 +77,11 @@ struct FeInfo {
     int32_t *seg_level = nullptr;
     int64_t nwords = 0;
     double *z_orig = nullptr;      // interaction, caller's order (nullptr: UnitWeights)
-    void *scale_T = nullptr;       // n, storage precision
-    double *scale_d = nullptr;     // n, widened
-    void *cache_int = nullptr;     // N, internal order
-    void *cache_csr = nullptr;     // N, CSR order (== cache_int for KIND_SORTED)
+    // per storage precision: [0] double, [1] float (both for FEHIP_MIXED)
+    void *scale_T[2] = {nullptr, nullptr};    // n
+    double *scale_d = nullptr;                // n, widened
+    void *cache_int[2] = {nullptr, nullptr};  // N, internal order
+    void *cache_csr[2] = {nullptr, nullptr};  // N, CSR order (== cache_int for KIND_SORTED)
     std::vector<int32_t> h_ioff;   // host copy of segment offsets (tile schedule)
 };
 
@@ -86,7 +92,8 @@ struct fehip_solver {
     int32_t G = 0;
     int64_t p = 0;
     int precision = FEHIP_FP64;
-    size_t tsize = 8;
+    bool has64 = true, has32 = false;  // storage precisions present
+    double mixed_tol = 1e-6;           // FEHIP_MIXED: tolerance of the FP32-stream phase
     int s = 0;  // FE defining the internal order
     std::vector<int64_t> n, off;
     hipStream_t stream = nullptr;
@@ -105,6 +112,13 @@ struct fehip_solver {
     int poll_iters = 8;
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
     std::vector<double> last_info;  // LSMR scalars of the last solve, 9 per column
+    // warm start: per-level degrees (lazy), feature / prediction scratch
+    int32_t *degree = nullptr;   // p
+    float *ws_feats = nullptr;   // p x 16
+    float *ws_pred = nullptr;    // p
+    double *ws_sig = nullptr;    // 1
+    double *ws_coef = nullptr;   // p (host-provided initial coefficients)
+    int t_sec = -1;              // FE giving the secondary order key (-1: none)
 };
 
 template <typename P>
@@ -291,6 +305,11 @@ __global__ void k_cache(const int32_t *__restrict__ perm_s, int64_t N, const int
     cache[j] = (T)((zi * sqrt(wi)) * (double)scale[ref_int[j]]);
 }
 
+__global__ void k_round_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = (float)a[e];
+}
+
 __global__ void k_sqrtw(const double *__restrict__ w, const int32_t *__restrict__ perm_s, int64_t N,
                         double *__restrict__ out) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -298,22 +317,26 @@ __global__ void k_sqrtw(const double *__restrict__ w, const int32_t *__restrict_
 }
 
 template <typename T>
+constexpr int pidx() { return sizeof(T) == 8 ? 0 : 1; }
+
+template <typename T>
 static int compute_scale_cache(fehip_solver *h) {
+    constexpr int q = pidx<T>();
     // scale! sums each level in the caller's observation order (the
     // reference's order), through the caller-order bucketing of every FE
     for (int g = 0; g < h->G; g++) {
         FeInfo &f = h->fe[g];
         hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(f.n, 64)), dim3(64), 0, h->stream, f.ooff, f.operm, f.n,
-                           h->w_orig, f.z_orig, (T *)f.scale_T, f.scale_d);
+                           h->w_orig, f.z_orig, (T *)f.scale_T[q], f.scale_d);
         HIPCHK(hipGetLastError());
     }
     for (int g = 0; g < h->G; g++) {
         FeInfo &f = h->fe[g];
         hipLaunchKernelGGL(k_cache<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, h->perm_s, h->N,
-                           f.ref_int, h->w_orig, f.z_orig, (const T *)f.scale_T, (T *)f.cache_int);
+                           f.ref_int, h->w_orig, f.z_orig, (const T *)f.scale_T[q], (T *)f.cache_int[q]);
         if (f.kind == KIND_CSR)
             hipLaunchKernelGGL(k_gather_perm<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
-                               (const T *)f.cache_int, f.iperm, h->N, (T *)f.cache_csr);
+                               (const T *)f.cache_int[q], f.iperm, h->N, (T *)f.cache_csr[q]);
         HIPCHK(hipGetLastError());
     }
     if (h->w_orig) {
@@ -340,8 +363,24 @@ static int do_update_weights(fehip_solver *h, const double *w) {
         h->w_orig = nullptr;
         h->sqrtw_int = nullptr;
     }
-    if (h->precision == FEHIP_FP64) return compute_scale_cache<double>(h);
-    return compute_scale_cache<float>(h);
+    if (h->precision == FEHIP_FP32) return compute_scale_cache<float>(h);
+    CHK(compute_scale_cache<double>(h));
+    if (h->precision == FEHIP_MIXED) {
+        // the FP32 streams hold the FP64 scales and caches rounded once
+        for (int g = 0; g < h->G; g++) {
+            FeInfo &f = h->fe[g];
+            hipLaunchKernelGGL(k_round_f32, dim3(grid_for(f.n, kBlock)), dim3(kBlock), 0, h->stream,
+                               (const double *)f.scale_T[0], (float *)f.scale_T[1], f.n);
+            hipLaunchKernelGGL(k_round_f32, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
+                               (const double *)f.cache_int[0], (float *)f.cache_int[1], h->N);
+            if (f.kind == KIND_CSR)
+                hipLaunchKernelGGL(k_round_f32, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
+                                   (const double *)f.cache_csr[0], (float *)f.cache_csr[1], h->N);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -374,7 +413,9 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
     h->N = N;
     h->G = G;
     h->precision = precision;
-    h->tsize = precision == FEHIP_FP64 ? 8 : 4;
+    h->has64 = precision != FEHIP_FP32;
+    h->has32 = precision != FEHIP_FP64;
+    if (const char *e = std::getenv("FEHIP_MIXED_TOL")) h->mixed_tol = std::atof(e);
     h->n.assign(nlevels, nlevels + G);
     h->off.resize(G + 1);
     h->off[0] = 0;
@@ -429,6 +470,7 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
     int t_sec = -1;
     for (int g = 0; g < G; g++)
         if (g != h->s && h->n[g] > dense_limit(precision) && (t_sec < 0 || h->n[g] > h->n[t_sec])) t_sec = g;
+    h->t_sec = t_sec;
     {
         FeInfo &f = h->fe[h->s];
         f.n = h->n[h->s];
@@ -499,11 +541,15 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
             CHK(dalloc(h, &f.z_orig, N));
             HIPCHK(hipMemcpyAsync(f.z_orig, interaction[g], sizeof(double) * N, hipMemcpyHostToDevice, h->stream));
         }
-        CHK(dalloc_bytes(h, &f.scale_T, h->tsize * f.n));
         CHK(dalloc(h, &f.scale_d, f.n));
-        CHK(dalloc_bytes(h, &f.cache_int, h->tsize * N));
-        if (f.kind == KIND_CSR) CHK(dalloc_bytes(h, &f.cache_csr, h->tsize * N));
-        else f.cache_csr = f.cache_int;
+        for (int q = 0; q < 2; q++) {
+            if (!(q == 0 ? h->has64 : h->has32)) continue;
+            const size_t ts = q == 0 ? 8 : 4;
+            CHK(dalloc_bytes(h, &f.scale_T[q], ts * f.n));
+            CHK(dalloc_bytes(h, &f.cache_int[q], ts * N));
+            if (f.kind == KIND_CSR) CHK(dalloc_bytes(h, &f.cache_csr[q], ts * N));
+            else f.cache_csr[q] = f.cache_int[q];
+        }
         if (f.kind != KIND_DENSE) {
             f.h_ioff.resize(f.n + 1);
             HIPCHK(hipMemcpyAsync(f.h_ioff.data(), f.ioff, sizeof(int32_t) * (f.n + 1), hipMemcpyDeviceToHost,
@@ -520,12 +566,12 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
 // ---------------------------------------------------------------------------
 // per-K plan: tile schedule, fix-up items, buffers
 // ---------------------------------------------------------------------------
-static int tile_positions(int K) { return K <= 8 ? 1024 / K : 128; }
+static int tile_positions(int) { return 1024; }
 
 struct Plan {
     int K = 0, TP = 0;
     int32_t total_tiles = 0, nseg = 0, ndense = 0, dense_blocks = 0, dense_chunk = 0, nitems = 0;
-    int32_t dense_chunks = 0, dense_wpb = 1, dense_lds_wave = 0;
+    int32_t dense_wpb = 1, dense_lds_wave = 0;
     std::vector<int> seg_fe, dense_fe;
     std::vector<int32_t> tile0, blk0;
     std::vector<int64_t> drow0;
@@ -542,8 +588,9 @@ struct Plan {
     int graph_iters = 0;
 };
 
-static int get_plan(fehip_solver *h, int K, Plan **out) {
-    auto it = h->plans.find(K);
+static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
+    const int key = K * 16 + (int)ts;
+    auto it = h->plans.find(key);
     if (it != h->plans.end()) {
         *out = it->second;
         return 0;
@@ -585,48 +632,48 @@ static int get_plan(fehip_solver *h, int K, Plan **out) {
     }
     P->nseg = (int32_t)P->seg_fe.size();
     P->total_tiles = t0;
-    // dense FEs: one wave per chunk of observations, ~8192 chunks per FE so
-    // every CU holds several waves; waves per workgroup bounded by 64 KB LDS
+    // dense FEs: one wave per chunk of observations, ~4096 chunks per FE;
+    // waves per workgroup bounded by 64 KB of LDS; each FE's chunk count is a
+    // multiple of the waves per workgroup, so a workgroup never spans two FEs
     const int Pq = kWave / K;
-    const int64_t target_chunks = std::max<int64_t>(1024, 8192 / K);
-    int64_t chunk = (N + target_chunks - 1) / target_chunks;
+    int64_t chunk = (N + 4095) / 4096;
     chunk = std::max<int64_t>(chunk, 16 * Pq);
     chunk = (chunk + Pq - 1) / Pq * Pq;
     P->dense_chunk = (int32_t)chunk;
-    int32_t c0 = 0;
-    int64_t row0 = 0;
     int maxlev = 1;
+    for (int g = 0; g < h->G; g++)
+        if (h->fe[g].kind == KIND_DENSE) maxlev = std::max<int>(maxlev, (int)h->fe[g].n);
+    P->dense_lds_wave = (int32_t)(ts == 8 ? sizeof(double) : sizeof(F32x2)) * kWave * maxlev;
+    P->dense_wpb = (int32_t)std::max<int64_t>(1, std::min<int64_t>(kWavesPerBlock, 65536 / P->dense_lds_wave));
+    int32_t b0 = 0;
+    int64_t row0 = 0;
     for (int g = 0; g < h->G; g++) {
         const FeInfo &f = h->fe[g];
         if (f.kind != KIND_DENSE) continue;
-        const int32_t nch = (int32_t)((N + chunk - 1) / chunk);
+        const int64_t nch = (N + chunk - 1) / chunk;
+        const int32_t nblk = (int32_t)((nch + P->dense_wpb - 1) / P->dense_wpb);
         P->dense_fe.push_back(g);
-        P->blk0.push_back(c0);
+        P->blk0.push_back(b0);
         P->drow0.push_back(row0);
-        maxlev = std::max<int>(maxlev, (int)f.n);
         for (int64_t k = 0; k < f.n; k++) {
             FixItem fi{};
             fi.kind = 1;
             fi.level = (int32_t)k;
             fi.coef_off = h->off[g];
-            fi.first = row0 + k * nch;
-            fi.count = nch;
+            fi.first = row0 + k * nblk;
+            fi.count = nblk;
             items.push_back(fi);
         }
-        c0 += nch;
-        row0 += f.n * nch;
+        b0 += nblk;
+        row0 += f.n * nblk;
     }
     P->ndense = (int32_t)P->dense_fe.size();
-    P->dense_chunks = c0;
-    P->dense_lds_wave = (int32_t)(h->tsize == 8 ? sizeof(double) : sizeof(F32x2)) * kWave * maxlev;
-    P->dense_wpb = (int32_t)std::max<int64_t>(1, std::min<int64_t>(kWavesPerBlock, 65536 / P->dense_lds_wave));
-    P->dense_blocks = (int32_t)((c0 + P->dense_wpb - 1) / P->dense_wpb);
+    P->dense_blocks = b0;
     P->nitems = (int32_t)items.size();
-    P->db_blocks = (unsigned)std::min<int64_t>(grid_for((N + Pq - 1) / Pq, kWavesPerBlock), 2048);
-    P->seg_blocks = grid_for(P->total_tiles, kWavesPerBlock);
-    P->fix_blocks = grid_for(P->nitems, kWavesPerBlock);
+    P->db_blocks = (unsigned)std::min<int64_t>(grid_for((N + Pq - 1) / Pq, 4 * kWavesPerBlock), 2048);
+    P->seg_blocks = (unsigned)std::min<int64_t>(grid_for(P->total_tiles, kWavesPerBlock), 2048);
+    P->fix_blocks = (unsigned)std::min<int64_t>(grid_for(P->nitems, kWavesPerBlock), 1024);
     P->upd_blocks = (unsigned)std::min<int64_t>(grid_for(h->p * K, kBlock), 1024);
-    const size_t ts = h->tsize;
     CHK(dalloc(h, &P->items, std::max<int32_t>(P->nitems, 1)));
     if (P->nitems)
         HIPCHK(hipMemcpyAsync(P->items, items.data(), sizeof(FixItem) * P->nitems, hipMemcpyHostToDevice, h->stream));
@@ -649,7 +696,7 @@ static int get_plan(fehip_solver *h, int K, Plan **out) {
     CHK(dalloc_bytes(h, &P->hh, ts * h->p * K));
     CHK(dalloc_bytes(h, &P->hbar, ts * h->p * K));
     HIPCHK(hipStreamSynchronize(h->stream));
-    h->plans[K] = pl.release();
+    h->plans[key] = pl.release();
     *out = P;
     return 0;
 }
@@ -758,11 +805,143 @@ __global__ void k_scale_coef(const T *__restrict__ x, const T *__restrict__ scal
 }
 
 // ---------------------------------------------------------------------------
+// warm-start support: per-level features and initial iterates
+// ---------------------------------------------------------------------------
+
+// degree of level k of FE s: distinct levels of FE t among its observations
+// (the internal order sorts each FE s run by FE t, so changes are counted)
+__global__ void k_degree_sorted(const int32_t *__restrict__ off, const int32_t *__restrict__ ref_t, int64_t nlev,
+                                int32_t *__restrict__ deg) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nlev) return;
+    const int32_t a = off[k], b = off[k + 1];
+    int32_t d = b > a ? 1 : 0;
+    if (ref_t)
+        for (int32_t q = a + 1; q < b; q++) d += ref_t[q] != ref_t[q - 1];
+    deg[k] = d;
+}
+
+// degree of level k of a CSR FE: distinct FE s levels among its observations
+// (its internal CSR lists observations in internal order, i.e. by FE s)
+__global__ void k_degree_csr(const int32_t *__restrict__ off, const int32_t *__restrict__ perm,
+                             const int32_t *__restrict__ ref_s, int64_t nlev, int32_t *__restrict__ deg) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nlev) return;
+    const int32_t a = off[k], b = off[k + 1];
+    int32_t d = b > a ? 1 : 0;
+    for (int32_t q = a + 1; q < b; q++) d += ref_s[perm[q]] != ref_s[perm[q - 1]];
+    deg[k] = d;
+}
+
+__global__ void k_degree_count(const int32_t *__restrict__ off, int64_t nlev, int32_t *__restrict__ deg) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nlev) deg[k] = off[k + 1] - off[k];
+}
+
+struct WsFE {
+    const double *scale;     // n
+    const int32_t *off;      // caller-order offsets (counts)
+    int64_t off_p;           // first level in p
+    int32_t n, interacted;
+};
+struct WsParams {
+    int32_t G, pad;
+    WsFE fe[kMaxG];
+    const int32_t *degree;   // p
+};
+
+// features of level e (FP32, 16 per level):
+//  0 log1p(count)   1 scale*xref/sigma (weighted group mean / sigma)
+//  2 log1p(degree)  3 scale*xap/sigma (one alternating-projection pass)
+//  4 interacted     5 -log(scale) = log(sum w z^2)/2   6..13 one-hot FE
+//  14 empty level   15 0
+template <typename T>
+__global__ void k_ws_features(WsParams prm, int64_t p, const T *__restrict__ xref, const T *__restrict__ xap,
+                              const double *__restrict__ sig, float *__restrict__ feats, double *__restrict__ coef_ap) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p) return;
+    int g = 0;
+    while (g + 1 < prm.G && e >= prm.fe[g + 1].off_p) g++;
+    const WsFE f = prm.fe[g];
+    const int64_t k = e - f.off_p;
+    const double sc = f.scale[k];
+    const double sigma = sig[0] > 0 ? sig[0] : 1.0;
+    const int32_t cnt = f.off[k + 1] - f.off[k];
+    float *o = feats + e * 16;
+    o[0] = log1pf((float)cnt);
+    o[1] = (float)(sc * (double)xref[e] / sigma);
+    o[2] = log1pf((float)prm.degree[e]);
+    o[3] = (float)(sc * (double)xap[e] / sigma);
+    o[4] = (float)f.interacted;
+    o[5] = sc > 0 ? (float)(-log(sc)) : 0.f;
+    for (int j = 0; j < 8; j++) o[6 + j] = (j == (g < 8 ? g : 7)) ? 1.f : 0.f;
+    o[14] = cnt == 0 ? 1.f : 0.f;
+    o[15] = 0.f;
+    if (coef_ap) coef_ap[e] = sc * (double)xap[e];
+}
+
+// x[e*K + c] = initial iterate in scaled coordinates from unscaled
+// coefficients b0 (x = b0 / scale; 0 on levels with scale 0)
+template <typename T>
+__global__ void k_x0_from_coef(const double *__restrict__ b0, WsParams prm, int64_t p, T *__restrict__ x, int K,
+                               int c) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p) return;
+    int g = 0;
+    while (g + 1 < prm.G && e >= prm.fe[g + 1].off_p) g++;
+    const double sc = prm.fe[g].scale[e - prm.fe[g].off_p];
+    x[e * K + c] = sc > 0 ? (T)(b0[e] / sc) : (T)0;
+}
+
+// x[e*K + c] = xap[e] + pred[e] * sigma / scale  (learned correction of the
+// alternating-projection start, in scaled coordinates)
+template <typename T>
+__global__ void k_x0_from_pred(const T *__restrict__ xap, const float *__restrict__ pred, const double *__restrict__ sig,
+                               WsParams prm, int64_t p, T *__restrict__ x, int K, int c) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p) return;
+    int g = 0;
+    while (g + 1 < prm.G && e >= prm.fe[g + 1].off_p) g++;
+    const double sc = prm.fe[g].scale[e - prm.fe[g].off_p];
+    const double sigma = sig[0] > 0 ? sig[0] : 1.0;
+    x[e * K + c] = sc > 0 ? (T)((double)xap[e] + (double)pred[e] * sigma / sc) : (T)0;
+}
+
+// sig[0] = sqrt(sum b^2 / n) over one column (single workgroup, fixed order)
+template <typename T>
+__global__ __launch_bounds__(kFinBlock) void k_col_rms(const T *__restrict__ b, int64_t n, double *__restrict__ sig) {
+    __shared__ double red[kFinBlock];
+    double s0 = 0, s1 = 0;
+    int64_t j = threadIdx.x;
+    for (; j + kFinBlock < n; j += 2 * kFinBlock) {
+        s0 += (double)b[j] * (double)b[j];
+        s1 += (double)b[j + kFinBlock] * (double)b[j + kFinBlock];
+    }
+    for (; j < n; j += kFinBlock) s0 += (double)b[j] * (double)b[j];
+    red[threadIdx.x] = s0 + s1;
+    __syncthreads();
+    for (int w = kFinBlock / 2; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sig[0] = n > 0 ? sqrt(red[0] / (double)n) : 0.0;
+}
+
+template <typename T>
+__global__ void k_copy_range(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = a[e];
+}
+
+// ---------------------------------------------------------------------------
 // engine: launches of one precision / column count
 // ---------------------------------------------------------------------------
 template <typename T, typename Acc, typename Tr, int K>
 struct Engine {
-    static constexpr int TP = K <= 8 ? 1024 / K : 128;
+    using TT = T;
+    using AccT = Acc;
+    using TrT = Tr;
+    static constexpr int TP = 1024;
 
     // target <- b*target + a*A*coef ; mode != NONE leaves ||target||^2 partials
     static void db(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *coef, T *target,
@@ -772,7 +951,7 @@ struct Engine {
         p.G = h->G;
         for (int g = 0; g < h->G; g++) {
             p.refs[g] = h->fe[g].ref_int;
-            p.cache[g] = (const T *)h->fe[g].cache_int;
+            p.cache[g] = (const T *)h->fe[g].cache_int[pidx<T>()];
             p.coef_off[g] = h->off[g];
         }
         p.coef = coef;
@@ -815,7 +994,7 @@ struct Engine {
             const FeInfo &f = h->fe[P->seg_fe[q]];
             SegFE<T> &s = p.fe[q];
             s.perm = f.kind == KIND_SORTED ? nullptr : f.iperm;
-            s.cache = (const T *)f.cache_csr;
+            s.cache = (const T *)f.cache_csr[pidx<T>()];
             s.heads = f.heads;
             s.wprefix = f.wprefix;
             s.seg_level = f.seg_level;
@@ -839,17 +1018,15 @@ struct Engine {
         DtxDenseParams<T> p{};
         p.N = h->N;
         p.ndense = P->ndense;
-        p.total_chunks = P->dense_chunks;
         p.chunk = P->dense_chunk;
-        p.wpb = P->dense_wpb;
         for (int q = 0; q < P->ndense; q++) {
             const FeInfo &f = h->fe[P->dense_fe[q]];
             DenseFE<T> &d = p.fe[q];
             d.refs = f.ref_int;
-            d.cache = (const T *)f.cache_int;
+            d.cache = (const T *)f.cache_int[pidx<T>()];
             d.nlev = (int32_t)f.n;
-            d.chunk0 = P->blk0[q];
-            d.nchunk = (int32_t)((h->N + P->dense_chunk - 1) / P->dense_chunk);
+            d.blk0 = P->blk0[q];
+            d.nblk = (q + 1 < P->ndense ? P->blk0[q + 1] : P->dense_blocks) - P->blk0[q];
             d.coef_off = h->off[P->dense_fe[q]];
             d.dpart_row0 = P->drow0[q];
         }
@@ -858,7 +1035,8 @@ struct Engine {
         p.ops = ops;
         p.all_done = early_exit;
         const size_t lds = (size_t)P->dense_lds_wave * P->dense_wpb;
-        hipLaunchKernelGGL((k_dtx_dense<T, Acc, K>), dim3(P->dense_blocks), dim3(kBlock), lds, h->stream, p);
+        hipLaunchKernelGGL((k_dtx_dense<T, Acc, K>), dim3(P->dense_blocks), dim3(kWave * P->dense_wpb), lds,
+                           h->stream, p);
     }
 
     static void dtx_fixup(fehip_solver *h, Plan *P, int mode, const ColOps *ops, T *coef,
@@ -888,7 +1066,7 @@ struct Engine {
     // isolated launches (HIP events on the handle's stream).
     static int time_kernels(fehip_solver *h, int reps, double *out_ms) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, &P));
+        CHK(get_plan(h, K, sizeof(T), &P));
         const int64_t NK = h->N * K, pK = h->p * K;
         hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->u, NK, (T)1);
         void *pv[] = {P->x, P->v, P->hh, P->hbar};
@@ -1056,20 +1234,38 @@ struct Engine {
         return 0;
     }
 
+    // r = sqrt(w) .* y ; b = copy(r) (:51-55) and x = A'b (:56)
+    static int start(fehip_solver *h, Plan *P, const double *dY, int Kreal, int64_t ldY, bool x_from_At) {
+        const int64_t NK = h->N * K;
+        hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
+                           h->perm_s, h->sqrtw_int, h->N, (T *)P->r, (T *)P->u);
+        if (x_from_At) {
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.0, K, Kreal);
+            dtx(h, P, MODE_NONE, P->ops_once, (const T *)P->r, (T *)P->x, nullptr);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
+    // r = r - A x ; r ./= sqrt(w) ; back to the caller's columns (:62-66)
+    static int finish(fehip_solver *h, Plan *P, double *dY, int Kreal, int64_t ldY) {
+        const int64_t NK = h->N * K;
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
+        db(h, P, MODE_NONE, P->ops_once, (const T *)P->x, (T *)P->r, nullptr);
+        hipLaunchKernelGGL(k_store_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
+                           h->perm_s, h->sqrtw_int, h->N, (const T *)P->r);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+
     // inner solve_residuals! (src/AbstractFixedEffectSolver.jl:49-68) on Kreal
     // device columns
     static int solve_residuals(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
                                int64_t *iters, uint8_t *converged) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, &P));
-        const int64_t NK = h->N * K;
-        // r = sqrt(w) .* y ; b = copy(r)   (:51-55)
-        hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
-                           h->perm_s, h->sqrtw_int, h->N, (T *)P->r, (T *)P->u);
-        // x = A'b  (:56)
-        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.0, K, Kreal);
-        dtx(h, P, MODE_NONE, P->ops_once, (const T *)P->r, (T *)P->x, nullptr);
-        HIPCHK(hipGetLastError());
+        CHK(get_plan(h, K, sizeof(T), &P));
+        CHK(start(h, P, dY, Kreal, ldY, true));
         std::vector<int64_t> it(Kreal, 0);
         std::vector<int> cv(Kreal, 1);
         if (h->G > 1) {  // :58-61
@@ -1079,13 +1275,116 @@ struct Engine {
         } else {
             for (int c = 0; c < Kreal; c++) it[c] = 1;
         }
-        // r = r - A x ; r ./= sqrt(w)  (:62-66)
-        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
-        db(h, P, MODE_NONE, P->ops_once, (const T *)P->x, (T *)P->r, nullptr);
-        hipLaunchKernelGGL(k_store_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
-                           h->perm_s, h->sqrtw_int, h->N, (const T *)P->r);
+        CHK(finish(h, P, dY, Kreal, ldY));
+        for (int c = 0; c < Kreal; c++) {
+            if (iters) iters[c] = it[c];
+            if (converged) converged[c] = (uint8_t)cv[c];
+        }
+        return 0;
+    }
+
+    static WsParams ws_params(fehip_solver *h) {
+        WsParams w{};
+        w.G = h->G;
+        for (int g = 0; g < h->G; g++) {
+            w.fe[g].scale = h->fe[g].scale_d;
+            w.fe[g].off = h->fe[g].ooff;
+            w.fe[g].off_p = h->off[g];
+            w.fe[g].n = (int32_t)h->n[g];
+            w.fe[g].interacted = h->fe[g].z_orig != nullptr;
+        }
+        w.degree = h->degree;
+        return w;
+    }
+
+    static int ensure_ws(fehip_solver *h) {
+        if (!h->degree) {
+            CHK(dalloc(h, &h->degree, h->p));
+            for (int g = 0; g < h->G; g++) {
+                const FeInfo &f = h->fe[g];
+                int32_t *d = h->degree + h->off[g];
+                if (g == h->s)
+                    hipLaunchKernelGGL(k_degree_sorted, dim3(grid_for(f.n, 64)), dim3(64), 0, h->stream, f.ioff,
+                                       h->t_sec >= 0 ? (const int32_t *)h->fe[h->t_sec].ref_int : nullptr, f.n, d);
+                else if (f.kind == KIND_CSR)
+                    hipLaunchKernelGGL(k_degree_csr, dim3(grid_for(f.n, 64)), dim3(64), 0, h->stream, f.ioff, f.iperm,
+                                       (const int32_t *)h->fe[h->s].ref_int, f.n, d);
+                else
+                    hipLaunchKernelGGL(k_degree_count, dim3(grid_for(f.n, 256)), dim3(256), 0, h->stream, f.ooff,
+                                       f.n, d);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        if (!h->ws_feats) CHK(dalloc(h, &h->ws_feats, h->p * 16));
+        if (!h->ws_pred) CHK(dalloc(h, &h->ws_pred, h->p));
+        if (!h->ws_sig) CHK(dalloc(h, &h->ws_sig, 1));
+        if (!h->ws_coef) CHK(dalloc(h, &h->ws_coef, h->p));
+        return 0;
+    }
+
+    // Features of one caller column dy (device, N doubles) on the K = 1 plan:
+    // x = A'b (the reference start), one alternating-projection pass
+    // (x_g = A_g' r, r -= A_g x_g for g = 1..G, left in P->hh), sigma = rms(b)
+    static int features(fehip_solver *h, const double *dy, float *feats, double *coef_ap) {
+        if (K != 1) return fail(FEHIP_EARG, "features run on single columns");
+        Plan *P = nullptr;
+        CHK(get_plan(h, 1, sizeof(T), &P));
+        CHK(ensure_ws(h));
+        CHK(start(h, P, dy, 1, h->N, true));
+        hipLaunchKernelGGL((k_col_rms<T>), dim3(1), dim3(kFinBlock), 0, h->stream, (const T *)P->r, h->N, h->ws_sig);
+        T *xap = (T *)P->hh, *mask = (T *)P->hbar, *tmp = (T *)P->v, *rw = (T *)P->u;
+        const unsigned gp = grid_for(h->p, kBlock);
+        hipLaunchKernelGGL(k_fill<T>, dim3(gp), dim3(kBlock), 0, h->stream, xap, h->p, (T)0);
+        for (int g = 0; g < h->G; g++) {
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.0, 1, 1);
+            dtx(h, P, MODE_NONE, P->ops_once, rw, tmp, nullptr);
+            hipLaunchKernelGGL(k_copy_range<T>, dim3(grid_for(h->n[g], kBlock)), dim3(kBlock), 0, h->stream,
+                               tmp + h->off[g], xap + h->off[g], h->n[g]);
+            hipLaunchKernelGGL(k_fill<T>, dim3(gp), dim3(kBlock), 0, h->stream, mask, h->p, (T)0);
+            hipLaunchKernelGGL(k_copy_range<T>, dim3(grid_for(h->n[g], kBlock)), dim3(kBlock), 0, h->stream,
+                               xap + h->off[g], mask + h->off[g], h->n[g]);
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, 1, 1);
+            db(h, P, MODE_NONE, P->ops_once, mask, rw, nullptr);
+        }
+        hipLaunchKernelGGL((k_ws_features<T>), dim3(gp), dim3(kBlock), 0, h->stream, ws_params(h), h->p,
+                           (const T *)P->x, (const T *)xap, (const double *)h->ws_sig, feats, coef_ap);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+
+    // solve_residuals! from given initial coefficients (unscaled, p per
+    // column; a NULL column keeps the reference start A'b), or from the
+    // network's start when net is given
+    static int solve_residuals_from(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
+                                    const double *const *coef0, fehip_mlp *net, int64_t *iters, uint8_t *converged) {
+        Plan *P = nullptr;
+        CHK(get_plan(h, K, sizeof(T), &P));
+        CHK(ensure_ws(h));
+        CHK(start(h, P, dY, Kreal, ldY, true));
+        const unsigned gp = grid_for(h->p, kBlock);
+        for (int c = 0; c < Kreal; c++) {
+            if (net) {
+                CHK((Engine<T, Acc, Tr, 1>::features(h, dY + (int64_t)c * ldY, h->ws_feats, nullptr)));
+                HIPCHK(hipStreamSynchronize(h->stream));
+                CHK(mlp_predict_device(net, h->ws_feats, h->p, h->ws_pred));
+                Plan *P1 = nullptr;
+                CHK(get_plan(h, 1, sizeof(T), &P1));
+                hipLaunchKernelGGL((k_x0_from_pred<T>), dim3(gp), dim3(kBlock), 0, h->stream, (const T *)P1->hh,
+                                   (const float *)h->ws_pred, (const double *)h->ws_sig, ws_params(h), h->p, (T *)P->x,
+                                   K, c);
+            } else if (coef0 && coef0[c]) {
+                HIPCHK(hipMemcpyAsync(h->ws_coef, coef0[c], sizeof(double) * h->p, hipMemcpyHostToDevice, h->stream));
+                hipLaunchKernelGGL((k_x0_from_coef<T>), dim3(gp), dim3(kBlock), 0, h->stream,
+                                   (const double *)h->ws_coef, ws_params(h), h->p, (T *)P->x, K, c);
+            }
+            HIPCHK(hipGetLastError());
+        }
+        std::vector<int64_t> it(Kreal, 1);
+        std::vector<int> cv(Kreal, 1);
+        CHK(lsmr(h, P, Kreal, tol, maxiter - 1));
+        CHK(read_state(h, P, Kreal, it, cv));
+        for (int c = 0; c < Kreal; c++) it[c] += 1;
+        CHK(finish(h, P, dY, Kreal, ldY));
         for (int c = 0; c < Kreal; c++) {
             if (iters) iters[c] = it[c];
             if (converged) converged[c] = (uint8_t)cv[c];
@@ -1095,21 +1394,28 @@ struct Engine {
 
     // inner solve_coefficients! up to the collect (:147-158)
     static int solve_coefficients(fehip_solver *h, double *dy, double tol, int64_t maxiter, double *dcoef,
-                                  int64_t *iters, uint8_t *converged) {
+                                  int64_t *iters, uint8_t *converged, const double *coef0 = nullptr) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, &P));
+        CHK(get_plan(h, K, sizeof(T), &P));
         const int64_t NK = h->N * K;
         hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, K, 1,
                            h->perm_s, h->sqrtw_int, h->N, (T *)P->u, (T *)nullptr);
         hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(h->p * K, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->x,
                            h->p * K, (T)0);
+        if (coef0) {
+            CHK(ensure_ws(h));
+            HIPCHK(hipMemcpyAsync(h->ws_coef, coef0, sizeof(double) * h->p, hipMemcpyHostToDevice, h->stream));
+            hipLaunchKernelGGL((k_x0_from_coef<T>), dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream,
+                               (const double *)h->ws_coef, ws_params(h), h->p, (T *)P->x, K, 0);
+        }
         CHK(lsmr(h, P, 1, tol, maxiter));
         std::vector<int64_t> it;
         std::vector<int> cv;
         CHK(read_state(h, P, 1, it, cv));
         for (int g = 0; g < h->G; g++)
             hipLaunchKernelGGL(k_scale_coef<T>, dim3(grid_for(h->n[g], kBlock)), dim3(kBlock), 0, h->stream,
-                               (const T *)P->x + h->off[g], (const T *)h->fe[g].scale_T, h->n[g], dcoef + h->off[g]);
+                               (const T *)P->x + h->off[g], (const T *)h->fe[g].scale_T[pidx<T>()], h->n[g],
+                               dcoef + h->off[g]);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(h->stream));
         if (iters) *iters = it[0] / 2;  // :159
@@ -1120,7 +1426,7 @@ struct Engine {
     // test hooks: one mul! on K = 1 (caller's order in and out)
     static int mul_A(fehip_solver *h, double *dy, const double *dx, double a, double b) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, &P));
+        CHK(get_plan(h, K, sizeof(T), &P));
         hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, 1, 1,
                            h->perm_s, (const double *)nullptr, h->N, (T *)P->r, (T *)nullptr);
         hipLaunchKernelGGL(k_to_T<T>, dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream, dx, (T *)P->x, h->p);
@@ -1135,7 +1441,7 @@ struct Engine {
 
     static int mul_At(fehip_solver *h, double *dx, const double *dy, double a, double b) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, &P));
+        CHK(get_plan(h, K, sizeof(T), &P));
         hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, 1, 1,
                            h->perm_s, (const double *)nullptr, h->N, (T *)P->r, (T *)nullptr);
         hipLaunchKernelGGL(k_to_T<T>, dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream, (const double *)dx,
@@ -1171,6 +1477,94 @@ static int dispatch(int precision, int K, F &&f) {
     return for_K<float, F32x2, double>(K, f);
 }
 
+// FEHIP_MIXED: LSMR with FP32 streams (compensated segment sums, FP64
+// scalars) to a loose tolerance, then FP64 LSMR iterations started from that
+// solution (x handed over in FP64) under the caller's stopping rule; the
+// residual is formed in FP64.  The FP32 phase alone cannot reach 1e-8: the
+// caches rounded to FP32 perturb the operator and bias the least-squares
+// fixed point by about eps32 * cond(A).
+template <int K>
+struct Mixed {
+    using E32 = Engine<float, F32x2, double, K>;
+    using E64 = Engine<double, double, double, K>;
+
+    // phase 1 from x0 (A'b for residuals, 0 for coefficients); returns the
+    // largest phase-1 iteration count
+    static int phase1(fehip_solver *h, Plan *P32, Plan *P64, const double *dY, int Kreal, int64_t ldY, bool x_from_At,
+                      double tol, int64_t maxiter, std::vector<int64_t> &it1) {
+        CHK(E32::start(h, P32, dY, Kreal, ldY, x_from_At));
+        if (!x_from_At)
+            hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(h->p * K, kBlock)), dim3(kBlock), 0, h->stream,
+                               (float *)P32->x, h->p * K, 0.f);
+        CHK(E32::lsmr(h, P32, Kreal, std::max(tol, h->mixed_tol), maxiter));
+        std::vector<int> cv;
+        CHK(E32::read_state(h, P32, Kreal, it1, cv));
+        hipLaunchKernelGGL(k_to_double<float>, dim3(grid_for(h->p * K, kBlock)), dim3(kBlock), 0, h->stream,
+                           (const float *)P32->x, (double *)P64->x, h->p * K);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
+    static int solve_residuals(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
+                               int64_t *iters, uint8_t *converged) {
+        if (h->G == 1) return E64::solve_residuals(h, dY, Kreal, ldY, tol, maxiter, iters, converged);
+        Plan *P32 = nullptr, *P64 = nullptr;
+        CHK(get_plan(h, K, 4, &P32));
+        CHK(get_plan(h, K, 8, &P64));
+        std::vector<int64_t> it1, it2;
+        std::vector<int> cv;
+        CHK(phase1(h, P32, P64, dY, Kreal, ldY, true, tol, maxiter - 1, it1));
+        const int64_t used = *std::max_element(it1.begin(), it1.end());
+        CHK(E64::start(h, P64, dY, Kreal, ldY, false));
+        CHK(E64::lsmr(h, P64, Kreal, tol, std::max<int64_t>(1, maxiter - 1 - used)));
+        CHK(E64::read_state(h, P64, Kreal, it2, cv));
+        CHK(E64::finish(h, P64, dY, Kreal, ldY));
+        for (int c = 0; c < Kreal; c++) {
+            h->last_info[(size_t)c * 9] = (double)(it1[c] + it2[c]);
+            if (iters) iters[c] = it1[c] + it2[c] + 1;
+            if (converged) converged[c] = (uint8_t)cv[c];
+        }
+        return 0;
+    }
+
+    static int solve_coefficients(fehip_solver *h, double *dy, double tol, int64_t maxiter, double *dcoef,
+                                  int64_t *iters, uint8_t *converged) {
+        Plan *P32 = nullptr, *P64 = nullptr;
+        CHK(get_plan(h, K, 4, &P32));
+        CHK(get_plan(h, K, 8, &P64));
+        std::vector<int64_t> it1, it2;
+        std::vector<int> cv;
+        CHK(phase1(h, P32, P64, dy, 1, h->N, false, tol, maxiter, it1));
+        CHK(E64::start(h, P64, dy, 1, h->N, false));
+        CHK(E64::lsmr(h, P64, 1, tol, std::max<int64_t>(1, maxiter - it1[0])));
+        CHK(E64::read_state(h, P64, 1, it2, cv));
+        for (int g = 0; g < h->G; g++)
+            hipLaunchKernelGGL(k_scale_coef<double>, dim3(grid_for(h->n[g], kBlock)), dim3(kBlock), 0, h->stream,
+                               (const double *)P64->x + h->off[g], (const double *)h->fe[g].scale_T[0], h->n[g],
+                               dcoef + h->off[g]);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+        const int64_t tot = it1[0] + it2[0];
+        h->last_info[0] = (double)tot;
+        if (iters) *iters = tot / 2;
+        if (converged) *converged = (uint8_t)cv[0];
+        return 0;
+    }
+};
+
+template <typename F>
+static int for_K_mixed(int K, F &&f) {
+    switch (K) {
+        case 1: return f(Mixed<1>{});
+        case 2: return f(Mixed<2>{});
+        case 4: return f(Mixed<4>{});
+        case 8: return f(Mixed<8>{});
+        case 16: return f(Mixed<16>{});
+        case 32: return f(Mixed<32>{});
+        default: return fail(FEHIP_EARG, "unsupported batch width");
+    }
+}
+
 static int padded_K(int K) {
     int k = 1;
     while (k < K) k <<= 1;
@@ -1179,6 +1573,10 @@ static int padded_K(int K) {
 
 static int solve_residuals_dev(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
                                int64_t *iters, uint8_t *converged) {
+    if (h->precision == FEHIP_MIXED)
+        return for_K_mixed(padded_K(Kreal), [&](auto e) {
+            return decltype(e)::solve_residuals(h, dY, Kreal, ldY, tol, maxiter, iters, converged);
+        });
     return dispatch(h->precision, padded_K(Kreal), [&](auto e) {
         return decltype(e)::solve_residuals(h, dY, Kreal, ldY, tol, maxiter, iters, converged);
     });
@@ -1271,13 +1669,131 @@ int fehip_solve_coefficients(fehip_t *h, const double *y, double tol, int64_t ma
         hipError_t e = hipMemcpyAsync(dy, y, sizeof(double) * h->N, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
     }
-    if (rc == 0)
+    if (rc == 0 && h->precision == FEHIP_MIXED)
+        rc = for_K_mixed(1, [&](auto e) {
+            return decltype(e)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged);
+        });
+    else if (rc == 0)
         rc = dispatch(h->precision, 1, [&](auto e) {
             return decltype(e)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged);
         });
     for (int g = 0; g < h->G && rc == 0; g++) {
         hipError_t e = hipMemcpy(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    dfree(h, dy);
+    dfree(h, dc);
+    return rc;
+}
+
+int fehip_features(fehip_t *h, const double *y, float *feats, double *coef_ap, double *sigma) {
+    g_err.clear();
+    if (!h || !y || !feats) return fail(FEHIP_EARG, "bad arguments");
+    double *dy = nullptr, *dca = nullptr;
+    CHK(dalloc(h, &dy, h->N));
+    CHK(dalloc(h, &dca, h->p));
+    int rc = 0;
+    hipError_t e = hipMemcpyAsync(dy, y, sizeof(double) * h->N, hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    const int prec = h->precision == FEHIP_MIXED ? FEHIP_FP64 : h->precision;
+    if (rc == 0)
+        rc = dispatch(prec, 1, [&](auto eng) -> int {
+            using E1 = Engine<typename decltype(eng)::TT, typename decltype(eng)::AccT, typename decltype(eng)::TrT, 1>;
+            CHK(E1::ensure_ws(h));
+            return E1::features(h, dy, h->ws_feats, dca);
+        });
+    if (rc == 0) {
+        HIPCHK(hipMemcpyAsync(feats, h->ws_feats, sizeof(float) * h->p * 16, hipMemcpyDeviceToHost, h->stream));
+        if (coef_ap) HIPCHK(hipMemcpyAsync(coef_ap, dca, sizeof(double) * h->p, hipMemcpyDeviceToHost, h->stream));
+        if (sigma) HIPCHK(hipMemcpyAsync(sigma, h->ws_sig, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    dfree(h, dy);
+    dfree(h, dca);
+    return rc;
+}
+
+static int solve_from_common(fehip_t *h, double *const *cols, int32_t K, double tol, int64_t maxiter,
+                             const double *const *coef0, fehip_mlp_t *net, int64_t *iters, uint8_t *converged) {
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (K < 1 || K > kMaxK || !cols) return fail(FEHIP_EARG, "need 1..32 columns");
+    if (maxiter < 1) return fail(FEHIP_EARG, "maxiter must be >= 1");
+    double *dY = nullptr;
+    CHK(dalloc(h, &dY, (size_t)h->N * K));
+    int rc = 0;
+    for (int c = 0; c < K && rc == 0; c++) {
+        hipError_t e = hipMemcpyAsync(dY + (int64_t)c * h->N, cols[c], sizeof(double) * h->N, hipMemcpyHostToDevice,
+                                      h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    const int prec = h->precision == FEHIP_MIXED ? FEHIP_FP64 : h->precision;
+    if (rc == 0)
+        rc = dispatch(prec, padded_K(K), [&](auto eng) {
+            return decltype(eng)::solve_residuals_from(h, dY, K, h->N, tol, maxiter, coef0, net, iters, converged);
+        });
+    for (int c = 0; c < K && rc == 0; c++) {
+        hipError_t e = hipMemcpyAsync(cols[c], dY + (int64_t)c * h->N, sizeof(double) * h->N, hipMemcpyDeviceToHost,
+                                      h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    if (rc == 0) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    dfree(h, dY);
+    return rc;
+}
+
+int fehip_solve_residuals_x0(fehip_t *h, double *const *cols, int32_t K, double tol, int64_t maxiter,
+                             const double *const *coef0, int64_t *iters, uint8_t *converged) {
+    g_err.clear();
+    return solve_from_common(h, cols, K, tol, maxiter, coef0, nullptr, iters, converged);
+}
+
+int fehip_solve_residuals_learned(fehip_t *h, double *const *cols, int32_t K, double tol, int64_t maxiter,
+                                  fehip_mlp_t *net, int64_t *iters, uint8_t *converged) {
+    g_err.clear();
+    if (!net) return fail(FEHIP_EARG, "network is NULL");
+    return solve_from_common(h, cols, K, tol, maxiter, nullptr, net, iters, converged);
+}
+
+int fehip_solve_residuals_learned_device(fehip_t *h, double *dY, int32_t K, int64_t ldY, double tol,
+                                         int64_t maxiter, fehip_mlp_t *net, int64_t *iters, uint8_t *converged) {
+    g_err.clear();
+    if (!h || !net || !dY) return fail(FEHIP_EARG, "bad arguments");
+    if (K < 1 || K > kMaxK) return fail(FEHIP_EARG, "need 1..32 columns");
+    if (ldY < h->N) return fail(FEHIP_EDIM, "ldY < N");
+    if (maxiter < 1) return fail(FEHIP_EARG, "maxiter must be >= 1");
+    const int prec = h->precision == FEHIP_MIXED ? FEHIP_FP64 : h->precision;
+    return dispatch(prec, padded_K(K), [&](auto eng) {
+        return decltype(eng)::solve_residuals_from(h, dY, K, ldY, tol, maxiter, nullptr, net, iters, converged);
+    });
+}
+
+int fehip_solve_coefficients_x0(fehip_t *h, const double *y, double tol, int64_t maxiter, const double *coef0,
+                                double *const *coef, int64_t *iters, uint8_t *converged) {
+    g_err.clear();
+    if (!h || !y || !coef) return fail(FEHIP_EARG, "bad arguments");
+    if (maxiter < 0) return fail(FEHIP_EARG, "maxiter must be >= 0");
+    double *dy = nullptr, *dc = nullptr;
+    CHK(dalloc(h, &dy, h->N));
+    CHK(dalloc(h, &dc, h->p));
+    int rc = 0;
+    hipError_t e = hipMemcpyAsync(dy, y, sizeof(double) * h->N, hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    const int prec = h->precision == FEHIP_MIXED ? FEHIP_FP64 : h->precision;
+    if (rc == 0)
+        rc = dispatch(prec, 1, [&](auto eng) {
+            return decltype(eng)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged, coef0);
+        });
+    for (int g = 0; g < h->G && rc == 0; g++) {
+        hipError_t e2 = hipMemcpyAsync(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost,
+                                       h->stream);
+        if (e2 != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e2));
+    }
+    if (rc == 0) {
+        hipError_t e2 = hipStreamSynchronize(h->stream);
+        if (e2 != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e2));
     }
     dfree(h, dy);
     dfree(h, dc);
@@ -1309,12 +1825,12 @@ int fehip_get_cache(fehip_t *h, int32_t g, double *cache) {
     double *tmp = nullptr, *tmp2 = nullptr;
     CHK(dalloc(h, &tmp, h->N));
     CHK(dalloc(h, &tmp2, h->N));
-    if (h->precision == FEHIP_FP64)
+    if (h->has64)
         hipLaunchKernelGGL(k_to_double<double>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
-                           (const double *)h->fe[g].cache_int, tmp, h->N);
+                           (const double *)h->fe[g].cache_int[0], tmp, h->N);
     else
         hipLaunchKernelGGL(k_to_double<float>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
-                           (const float *)h->fe[g].cache_int, tmp, h->N);
+                           (const float *)h->fe[g].cache_int[1], tmp, h->N);
     hipLaunchKernelGGL(k_scatter_perm<double>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, tmp,
                        h->perm_s, h->N, tmp2);
     HIPCHK(hipGetLastError());
@@ -1336,7 +1852,7 @@ static int mul_common(fehip_t *h, double *io, const double *other, double a, dou
     double *x_host = adjoint ? io : (double *)other;
     HIPCHK(hipMemcpy(dy, y_host, sizeof(double) * h->N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dx, x_host, sizeof(double) * h->p, hipMemcpyHostToDevice));
-    const int rc = dispatch(h->precision, 1, [&](auto e) {
+    const int rc = dispatch(h->precision == FEHIP_MIXED ? FEHIP_FP64 : h->precision, 1, [&](auto e) {
         using E = decltype(e);
         return adjoint ? E::mul_At(h, dx, dy, a, b) : E::mul_A(h, dy, dx, a, b);
     });

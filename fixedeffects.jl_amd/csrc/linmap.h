@@ -293,6 +293,7 @@ template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
     if (p.all_done && *p.all_done) return;
     constexpr int P = kWave / K;
+    constexpr int U = 4;  // wave steps in flight per lane
     const int lane = threadIdx.x & (kWave - 1);
     const int psub = lane / K, c = lane % K;
     const int on = p.ops->on[c];
@@ -303,20 +304,38 @@ __global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     double sq = 0.0;
     if (on) {
-        for (int64_t rr = gw; rr < nrounds; rr += nw) {
-            const int64_t i = rr * P + psub;
-            if (i < p.N) {
-                T acc = (T)0;
-                for (int g = 0; g < p.G; g++) {
-                    const int32_t r = __builtin_nontemporal_load(p.refs[g] + i);
-                    const T cv = __builtin_nontemporal_load(p.cache[g] + i);
-                    acc += cv * p.coef[(p.coef_off[g] + r) * K + c];
+        for (int64_t r0 = gw * U; r0 < nrounds; r0 += nw * U) {
+            int64_t idx[U];
+            T acc[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                idx[q] = (r0 + q) * P + psub;
+                acc[q] = (T)0;
+            }
+            for (int g = 0; g < p.G; g++) {
+                int32_t r[U];
+                T cv[U];
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    if (idx[q] < p.N) {
+                        r[q] = __builtin_nontemporal_load(p.refs[g] + idx[q]);
+                        cv[q] = __builtin_nontemporal_load(p.cache[g] + idx[q]);
+                    }
                 }
-                T *tp = p.target + i * K + c;
-                T out = a * acc;
-                if (keep) out = b * (*tp) + out;
-                *tp = out;
-                sq += (double)out * (double)out;
+                const T *cf = p.coef + p.coef_off[g] * K + c;
+#pragma unroll
+                for (int q = 0; q < U; q++)
+                    if (idx[q] < p.N) acc[q] += cv[q] * cf[(int64_t)r[q] * K];
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                if (idx[q] < p.N) {
+                    T *tp = p.target + idx[q] * K + c;
+                    T out = a * acc[q];
+                    if (keep) out = b * (*tp) + out;
+                    *tp = out;
+                    sq += (double)out * (double)out;
+                }
             }
         }
     }
@@ -365,14 +384,16 @@ template <typename T, typename Acc, int K, int TP>
 __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
     if (p.all_done && *p.all_done) return;
     constexpr int P = kWave / K;
+    constexpr int U = 4;  // wave steps whose loads are issued together
     const int lane = threadIdx.x & (kWave - 1);
     const int psub = lane / K, c = lane % K;
-    const int t = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
     const int on = p.ops->on[c];
     const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
     const bool keep = p.ops->b[c] != 0.0;
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    const int nw = gridDim.x * kWavesPerBlock;
     double sq = 0.0;
-    if (t < p.total_tiles) {
+    for (int t = gw; t < p.total_tiles; t += nw) {
         int f = 0;
         while (f + 1 < p.nseg && t >= p.fe[f + 1].tile0) f++;
         const SegFE<T> fe = p.fe[f];
@@ -382,43 +403,55 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
         Acc carry = Acc(0.f);
         bool cont = true;    // no segment head seen yet in this tile
         bool open = false;   // a run is still open after the last step
-        for (int64_t base = start; base < end; base += P) {
-            const int64_t pos = base + psub;
-            const bool valid = pos < end;
-            Acc val = Acc(0.f);
-            if (valid && on) {
-                const int64_t i = fe.perm ? (int64_t)__builtin_nontemporal_load(fe.perm + pos) : pos;
-                const T cv = __builtin_nontemporal_load(fe.cache + pos);
-                val = mul_acc<Acc, T>(cv, p.u[i * K + c]);
-            }
-            const uint64_t w = fe.heads[base >> 6];
-            const int sh = (int)(base & 63);
-            const uint64_t hb = (P == 64) ? w : ((w >> sh) & ((1ull << (P & 63)) - 1ull));
-            const uint64_t nb = (sh + P < 64) ? ((w >> (sh + P)) & 1ull) : (fe.heads[(base >> 6) + 1] & 1ull);
-            const uint64_t hle = hb & lanes_le(psub);
-            const int rs = hle ? (63 - __clzll(hle)) : 0;
-            if (psub == 0 && !(hb & 1ull)) val += carry;
-            val = seg_scan<K>(val, psub, rs);
-            const bool isend = (psub == P - 1) ? (nb != 0) : (((hb >> (psub + 1)) & 1ull) != 0);
-            const bool mycont = cont && (hle == 0);
-            if (valid && on && isend) {
-                if (mycont) {
-                    p.tileA[(int64_t)t * K + c] = acc_to_double(val);
-                } else {
-                    const uint32_t ord = fe.wprefix[pos >> 6] +
-                                         (uint32_t)__popcll(w & lanes_le((int)(pos & 63))) - 1u;
-                    const int32_t lev = fe.seg_level[ord];
-                    T *dst = v + (int64_t)lev * K + c;
-                    T out = a * acc_to_T<T, Acc>(val);
-                    if (keep) out = b * (*dst) + out;
-                    *dst = out;
-                    sq += (double)out * (double)out;
+        for (int64_t base0 = start; base0 < end; base0 += U * P) {
+            Acc val[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t pos = base0 + q * P + psub;
+                val[q] = Acc(0.f);
+                if (pos < end && on) {
+                    const int64_t i = fe.perm ? (int64_t)__builtin_nontemporal_load(fe.perm + pos) : pos;
+                    const T cv = __builtin_nontemporal_load(fe.cache + pos);
+                    val[q] = mul_acc<Acc, T>(cv, p.u[i * K + c]);
                 }
             }
-            const Acc last = __shfl(val, (P - 1) * K + c, kWave);
-            open = (nb == 0);
-            carry = open ? last : Acc(0.f);
-            cont = cont && (hb == 0);
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t base = base0 + q * P;
+                if (base >= end) break;  // wave-uniform
+                const int64_t pos = base + psub;
+                const bool valid = pos < end;
+                const uint64_t w = fe.heads[base >> 6];
+                const int sh = (int)(base & 63);
+                const uint64_t hb = (P == 64) ? w : ((w >> sh) & ((1ull << (P & 63)) - 1ull));
+                const uint64_t nb =
+                    (sh + P < 64) ? ((w >> (sh + P)) & 1ull) : (fe.heads[(base >> 6) + 1] & 1ull);
+                const uint64_t hle = hb & lanes_le(psub);
+                const int rs = hle ? (63 - __clzll(hle)) : 0;
+                Acc x = val[q];
+                if (psub == 0 && !(hb & 1ull)) x += carry;
+                x = seg_scan<K>(x, psub, rs);
+                const bool isend = (psub == P - 1) ? (nb != 0) : (((hb >> (psub + 1)) & 1ull) != 0);
+                const bool mycont = cont && (hle == 0);
+                if (valid && on && isend) {
+                    if (mycont) {
+                        p.tileA[(int64_t)t * K + c] = acc_to_double(x);
+                    } else {
+                        const uint32_t ord = fe.wprefix[pos >> 6] +
+                                             (uint32_t)__popcll(w & lanes_le((int)(pos & 63))) - 1u;
+                        const int32_t lev = fe.seg_level[ord];
+                        T *dst = v + (int64_t)lev * K + c;
+                        T out = a * acc_to_T<T, Acc>(x);
+                        if (keep) out = b * (*dst) + out;
+                        *dst = out;
+                        sq += (double)out * (double)out;
+                    }
+                }
+                const Acc last = __shfl(x, (P - 1) * K + c, kWave);
+                open = (nb == 0);
+                carry = open ? last : Acc(0.f);
+                cont = cont && (hb == 0);
+            }
         }
         if (open && on && psub == 0) {
             if (cont) p.tileA[(int64_t)t * K + c] = acc_to_double(carry);
@@ -433,14 +466,15 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
 
 // FEs with few levels: one wave per chunk of observations (internal order,
 // fully sequential reads); every lane owns a private LDS accumulator column
-// per level, so adds never collide.  Per-chunk level sums go to dpart,
-// level-major ([level][chunk] rows of K), and are combined by the fix-up
-// launch in chunk order.  WPB waves share a workgroup, each with its own LDS.
+// per level, so adds never collide.  The WPB waves of a workgroup cover
+// consecutive chunks; their level sums are combined in wave order in LDS and
+// written as one row per workgroup to dpart ([level][workgroup] rows of K),
+// which the fix-up launch sums in workgroup order.
 template <typename T>
 struct DenseFE {
     const int32_t *refs;
     const T *cache;
-    int32_t nlev, chunk0, nchunk, pad;
+    int32_t nlev, blk0, nblk, pad;
     int64_t coef_off;
     int64_t dpart_row0;  // first row (of K doubles) of this FE in dpart
 };
@@ -448,7 +482,7 @@ struct DenseFE {
 template <typename T>
 struct DtxDenseParams {
     int64_t N;
-    int32_t ndense, total_chunks, chunk, wpb;
+    int32_t ndense, chunk, pad0, pad1;
     DenseFE<T> fe[kMaxG];
     const T *u;
     double *dpart;
@@ -462,34 +496,56 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
     extern __shared__ __align__(16) unsigned char dense_smem[];
     constexpr int P = kWave / K;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int wpb = blockDim.x / kWave;
     const int psub = lane / K, c = lane % K;
-    const int ch = blockIdx.x * p.wpb + wave;  // global chunk
-    if (wave >= p.wpb || ch >= p.total_chunks) return;
     int f = 0;
-    while (f + 1 < p.ndense && ch >= p.fe[f + 1].chunk0) f++;
+    while (f + 1 < p.ndense && (int)blockIdx.x >= p.fe[f + 1].blk0) f++;
     const DenseFE<T> fe = p.fe[f];
+    const int bl = blockIdx.x - fe.blk0;
     Acc *acc = reinterpret_cast<Acc *>(dense_smem) + (size_t)wave * fe.nlev * kWave;
-    const int bl = ch - fe.chunk0;
     for (int k = 0; k < fe.nlev; k++) acc[k * kWave + lane] = Acc(0.f);
     const int on = p.ops->on[c];
-    const int64_t start = (int64_t)bl * p.chunk;
+    const int64_t start = ((int64_t)bl * wpb + wave) * p.chunk;
     const int64_t end = start + p.chunk < p.N ? start + p.chunk : p.N;
     if (on) {
-        for (int64_t base = start; base < end; base += P) {
-            const int64_t pos = base + psub;
-            if (pos < end) {
-                const int32_t r = __builtin_nontemporal_load(fe.refs + pos);
-                const T cv = __builtin_nontemporal_load(fe.cache + pos);
-                acc[r * kWave + lane] += mul_acc<Acc, T>(cv, p.u[pos * K + c]);
+        constexpr int U = 4;
+        for (int64_t base0 = start; base0 < end; base0 += U * P) {
+            int32_t r[U];
+            Acc x[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t pos = base0 + q * P + psub;
+                r[q] = -1;
+                if (pos < end) {
+                    r[q] = __builtin_nontemporal_load(fe.refs + pos);
+                    const T cv = __builtin_nontemporal_load(fe.cache + pos);
+                    x[q] = mul_acc<Acc, T>(cv, p.u[pos * K + c]);
+                }
             }
+#pragma unroll
+            for (int q = 0; q < U; q++)
+                if (r[q] >= 0) acc[r[q] * kWave + lane] += x[q];
         }
     }
-    // each lane reads back only its own LDS column: no barrier is needed
-    double *out = p.dpart + fe.dpart_row0 * K;
+    // Per-wave level sums over the lanes of each column.  The double result
+    // of level k goes to slot k*K + c of the wave's own region, which only
+    // overlaps accumulators of levels <= k, already consumed (K <= 32).
+    double *wsum = reinterpret_cast<double *>(acc);
     for (int k = 0; k < fe.nlev; k++) {
         Acc x = acc[k * kWave + lane];
         x = column_allreduce<K>(x);
-        if (psub == 0) out[((int64_t)k * fe.nchunk + bl) * K + c] = acc_to_double(x);
+        if (psub == 0) wsum[k * K + c] = acc_to_double(x);
+    }
+    __syncthreads();
+    // combine the waves in order; thread t handles entries t, t + blockDim, ...
+    double *out = p.dpart + fe.dpart_row0 * K;
+    const size_t region = (size_t)fe.nlev * kWave * sizeof(Acc) / sizeof(double);
+    const double *base_ws = reinterpret_cast<const double *>(dense_smem);
+    for (int e = threadIdx.x; e < fe.nlev * K; e += blockDim.x) {
+        double s = 0.0;
+        for (int w = 0; w < wpb; w++) s += base_ws[w * region + e];
+        const int k = e / K, cc = e % K;
+        out[((int64_t)k * fe.nblk + bl) * K + cc] = s;
     }
 }
 
@@ -523,12 +579,13 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
     constexpr int P = kWave / K;
     const int lane = threadIdx.x & (kWave - 1);
     const int psub = lane / K, c = lane % K;
-    const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    const int nw = gridDim.x * kWavesPerBlock;
     const int on = p.ops->on[c];
     const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
     const bool keep = p.ops->b[c] != 0.0;
     double sq = 0.0;
-    if (item < p.nitems) {
+    for (int item = gw; item < p.nitems; item += nw) {
         const FixItem it = p.items[item];
         double s = 0.0;
         if (on && it.kind != 2) {
@@ -554,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
             T out = a * (T)s;
             if (keep) out = b * (*dst) + out;
             *dst = out;
-            sq = (double)out * (double)out;
+            sq += (double)out * (double)out;
         }
     }
     if (p.mode == MODE_NONE) return;

@@ -23,6 +23,19 @@ FEHIP_FP32 = 1
 FEHIP_MIXED = 2
 
 EXPORTS = (
+    "fehip_solve_residuals_x0",
+    "fehip_solve_coefficients_x0",
+    "fehip_features",
+    "fehip_solve_residuals_learned",
+    "fehip_solve_residuals_learned_device",
+    "fehip_mlp_create",
+    "fehip_mlp_nparams",
+    "fehip_mlp_get_params",
+    "fehip_mlp_set_params",
+    "fehip_mlp_gradient",
+    "fehip_mlp_fit",
+    "fehip_mlp_predict",
+    "fehip_mlp_destroy",
     "fehip_create",
     "fehip_create_i32",
     "fehip_update_weights",
@@ -82,8 +95,26 @@ def _load() -> C.CDLL:
     L.fehip_last_error.argtypes = []
     L.fehip_destroy.argtypes = [_vp]
     L.fehip_destroy.restype = None
+    _fp = C.POINTER(C.c_float)
+    L.fehip_solve_residuals_x0.argtypes = [_vp, C.POINTER(_dp), C.c_int32, C.c_double, C.c_int64, C.POINTER(_dp),
+                                           _i64p, _u8p]
+    L.fehip_solve_coefficients_x0.argtypes = [_vp, _dp, C.c_double, C.c_int64, _dp, C.POINTER(_dp), _i64p, _u8p]
+    L.fehip_features.argtypes = [_vp, _dp, _fp, _dp, _dp]
+    L.fehip_solve_residuals_learned.argtypes = [_vp, C.POINTER(_dp), C.c_int32, C.c_double, C.c_int64, _vp, _i64p,
+                                                _u8p]
+    L.fehip_solve_residuals_learned_device.argtypes = [_vp, _vp, C.c_int32, C.c_int64, C.c_double, C.c_int64, _vp,
+                                                       _i64p, _u8p]
+    L.fehip_mlp_create.argtypes = [C.POINTER(_vp), C.c_uint64]
+    L.fehip_mlp_nparams.argtypes = [_vp, _i64p]
+    L.fehip_mlp_get_params.argtypes = [_vp, _fp]
+    L.fehip_mlp_set_params.argtypes = [_vp, _fp]
+    L.fehip_mlp_gradient.argtypes = [_vp, _fp, _fp, C.c_int64, _fp, _dp]
+    L.fehip_mlp_fit.argtypes = [_vp, _fp, _fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _dp]
+    L.fehip_mlp_predict.argtypes = [_vp, _fp, C.c_int64, _fp]
+    L.fehip_mlp_destroy.argtypes = [_vp]
+    L.fehip_mlp_destroy.restype = None
     for name in EXPORTS:
-        if name not in ("fehip_destroy", "fehip_last_error"):
+        if name not in ("fehip_destroy", "fehip_last_error", "fehip_mlp_destroy"):
             getattr(L, name).restype = C.c_int
     return L
 

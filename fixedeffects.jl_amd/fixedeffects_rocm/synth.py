@@ -126,6 +126,22 @@ def worker_firm(N: int = 10_000_000, n_workers: int = 1_000_000, n_firms: int = 
     return Panel([r1, r2, r3], [m1, m2, m3], [None, None, z], w, Y, info)
 
 
+def uneven_panel(seed: int, n_min: float = 1e4, n_max: float = 2e6) -> Panel:
+    """One panel of the uneven-batch workload: N log-uniform in [n_min, n_max],
+    2 or 3 FEs with N/10 Zipf(1.1) workers and N/100 firms (+ 20 years when
+    there are 3), mobility ~ U(0.01, 0.1), 1..8 columns built as above."""
+    rng = np.random.default_rng(seed)
+    N = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+    G = int(rng.integers(2, 4))
+    mob = float(rng.uniform(0.01, 0.1))
+    K = int(rng.integers(1, 9))
+    p = worker_firm(N, max(2, N // 10), max(2, N // 100), 20, mob, 1.1, K, seed=seed + 7_919)
+    if G == 2:
+        p.refs, p.n, p.interactions = p.refs[:2], p.n[:2], p.interactions[:2]
+    p.info.update({"panel_seed": seed, "G": G})
+    return p
+
+
 def config(name: str, scale: float = 1.0, seed: int = 1) -> Panel:
     """The named workloads; ``scale`` shrinks N and the level counts together
     (used for parity cases the CPU oracle finishes in seconds)."""

@@ -36,6 +36,7 @@ extern "C" {
 #endif
 
 typedef struct fehip_solver fehip_t;
+typedef struct fehip_mlp fehip_mlp_t;  /* the warm-start network */
 
 /* status codes */
 #define FEHIP_OK 0
@@ -105,6 +106,66 @@ int fehip_solve_residuals_device(fehip_t *h, double *dY, int32_t K,
 int fehip_solve_coefficients(fehip_t *h, const double *y, double tol,
                              int64_t maxiter, double *const *coef,
                              int64_t *iters, uint8_t *converged);
+
+/* Initial iterate (no reference counterpart as an argument; the reference's
+ * lsmr! takes it through its x argument, src/utils/lsmr.jl:42-43, and
+ * solve_residuals! always passes x0 = A'b, src/AbstractFixedEffectSolver.jl:56).
+ * Same as fehip_solve_residuals with LSMR started from the unscaled
+ * coefficients coef0[c] (sum(nlevels) doubles, FE blocks concatenated) for
+ * column c; coef0 == NULL or coef0[c] == NULL keeps the reference start. */
+int fehip_solve_residuals_x0(fehip_t *h, double *const *cols, int32_t K,
+                             double tol, int64_t maxiter,
+                             const double *const *coef0, int64_t *iters,
+                             uint8_t *converged);
+
+/* fehip_solve_coefficients with LSMR started from coef0 (unscaled,
+ * sum(nlevels) doubles) instead of 0; coef0 == NULL is the reference start. */
+int fehip_solve_coefficients_x0(fehip_t *h, const double *y, double tol,
+                                int64_t maxiter, const double *coef0,
+                                double *const *coef, int64_t *iters,
+                                uint8_t *converged);
+
+/* Warm-start features of one column y (N doubles), computed on the device:
+ * per level (sum(nlevels) rows x 16 floats) log1p(count), weighted group mean
+ * / sigma, log1p(degree in the FE graph), one alternating-projection pass /
+ * sigma, interaction flag, log(sum w z^2)/2, one-hot FE index, empty flag.
+ * coef_ap (optional) receives the alternating-projection coefficients
+ * (unscaled), sigma (optional) the rms of sqrt(w) .* y. */
+int fehip_features(fehip_t *h, const double *y, float *feats, double *coef_ap,
+                   double *sigma);
+
+/* Residualizes K columns with LSMR started from the network's prediction:
+ * coefficients = alternating-projection pass + net(features) * sigma.
+ * Otherwise as fehip_solve_residuals (host columns) /
+ * fehip_solve_residuals_device (device columns). */
+int fehip_solve_residuals_learned(fehip_t *h, double *const *cols, int32_t K,
+                                  double tol, int64_t maxiter,
+                                  fehip_mlp_t *net, int64_t *iters,
+                                  uint8_t *converged);
+int fehip_solve_residuals_learned_device(fehip_t *h, double *dY, int32_t K,
+                                         int64_t ldY, double tol,
+                                         int64_t maxiter, fehip_mlp_t *net,
+                                         int64_t *iters, uint8_t *converged);
+
+/* The warm-start network (no reference counterpart): 16 -> 128 -> 128 -> 128
+ * -> 1 perceptron with ReLU, bf16 MFMA products with FP32 accumulation on the
+ * GPU, FP32 master weights, Adam.  Parameters are one flat FP32 vector:
+ * W0 [16x128], b0 [128], W1 [128x128], b1, W2 [128x128], b2, W3 [128x16]
+ * (column 0 live), b3 [16]. */
+int fehip_mlp_create(fehip_mlp_t **out, uint64_t seed);
+int fehip_mlp_nparams(const fehip_mlp_t *m, int64_t *n);
+int fehip_mlp_get_params(fehip_mlp_t *m, float *out);
+int fehip_mlp_set_params(fehip_mlp_t *m, const float *in);
+/* Gradient of 0.5 * mean (net(X) - t)^2 over n rows (X: n x 16 floats), in
+ * the parameter layout; loss (optional) receives the loss.  No update. */
+int fehip_mlp_gradient(fehip_mlp_t *m, const float *X, const float *t,
+                       int64_t n, float *grad, double *loss);
+/* Adam training over `epochs` shuffled passes of mini-batches. */
+int fehip_mlp_fit(fehip_mlp_t *m, const float *X, const float *t, int64_t n,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                  double *loss_per_epoch);
+int fehip_mlp_predict(fehip_mlp_t *m, const float *X, int64_t n, float *y);
+void fehip_mlp_destroy(fehip_mlp_t *m);
 
 /* Test hook: the CSR bucketing of FE g over the caller's observation order,
  * as bucketize_refs builds it (ext/MetalExt.jl:40-64) but 0-based:

@@ -248,6 +248,40 @@ def test_solve_fp32_reference_mode(gpu):
     assert abs(it - io) <= max(3, io // 10)
 
 
+def test_mixed_precision_residuals(gpu):
+    # FP32 streams + FP64 scalars, then FP64 LSMR iterations: residuals as
+    # accurate as an FP64 solve at the same tol, and <= 1e-8 when converged
+    _say("case: mixed precision")
+    p = synth.config("worker_firm_1e7_k32", scale=0.02)
+    Y = p.Y[:, :4]
+    o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=8)
+    with api.FixedEffectSolverROCm(_fes(p.refs, p.n), p.weights, "mixed") as s:
+        for tol in (1e-8, 1e-12):
+            cols = [Y[:, c].copy() for c in range(4)]
+            _, its, cvs = s.solve_residuals_many(cols, tol=tol, maxiter=10000)
+            for c in range(4):
+                ro, io, co = o.solve_residuals(Y[:, c].copy(), tol=tol, maxiter=10000)
+                rt = o.solve_residuals(Y[:, c].copy(), tol=1e-13, maxiter=100000)[0]
+                _say(f"  tol {tol:g} col {c}: iters {its[c]} (oracle {io}) err vs oracle {_rel(cols[c], ro):.2e} "
+                     f"vs exact {_rel(cols[c], rt):.2e} (oracle {_rel(ro, rt):.2e})")
+                assert cvs[c]
+                if tol == 1e-12:
+                    assert _rel(cols[c], ro) <= RES_RTOL
+                else:
+                    assert _rel(cols[c], rt) <= 3 * _rel(ro, rt) + 1e-12
+
+
+def test_mixed_precision_coefficients(gpu):
+    _say("case: mixed coefficients")
+    refs, n, z, w, y = _rand_problem(62, 20_000, (2000, 150))
+    o = oracle.OracleSolver(refs, n, z, w)
+    co, io, cvo = o.solve_coefficients(y, tol=1e-12, maxiter=10000)
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, "mixed") as s:
+        cg, ig, cvg = s.solve_coefficients(y, tol=1e-12, maxiter=10000)
+    assert cvg and cvo
+    assert _rel(sum(cg), sum(co)) <= 1e-8
+
+
 def test_coefficients_match_oracle(gpu):
     _say("case: coefficients")
     refs, n, z, w, y = _rand_problem(61, 20_000, (2000, 150))
