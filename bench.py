@@ -41,22 +41,28 @@ def log(*a):
 
 
 def kernel_bytes(N, G, n, kinds, K, s):
-    """Algorithmic HBM bytes per launch of each kernel (one LSMR iteration)."""
+    """Algorithmic HBM bytes per launch of each kernel of one LSMR iteration
+    (each array element counted once; gathers counted as their distinct
+    elements)."""
     p = sum(n)
+    gs = kinds.index("sorted")
+    ndense_lev = sum(k for k, kd in zip(n, kinds) if kd == "dense")
+    fused_dense = 0 < ndense_lev <= 32
     out = {}
-    # D*v: refs + cache per FE, u read + write, each coefficient read once
-    out["k_db"] = N * G * (4 + s) + K * N * 2 * s + K * p * s
-    seg = 0
+    # fused pass: refs + cache of every FE, u read + write, each coefficient
+    # once, FE s head bits, raw FE s sums (double) out
+    out["k_fused"] = N * G * (4 + s) + K * N * 2 * s + K * p * s + N // 8 + 8 * K * n[gs]
+    csr = 0
     dense = 0
     for g in range(G):
-        if kinds[g] == "dense":
+        if kinds[g] == "csr":
+            csr += 4 * N + N * s + K * N * s + 2 * K * n[g] * s + N // 8 + 4 * n[g]
+        elif kinds[g] == "dense" and not fused_dense:
             dense += N * (4 + s) + K * N * s
-        else:
-            seg += N * s + K * N * s + 2 * K * n[g] * s + N // 8 + 4 * n[g]
-            if kinds[g] == "csr":
-                seg += 4 * N
-    out["k_dtx_seg"] = seg
+    out["k_dtx_csr"] = csr
     out["k_dtx_dense"] = dense
+    # FE s levels: raw sum (double) + v read + write, one flag byte
+    out["k_dtx_fixup"] = K * n[gs] * (8 + 2 * s) + n[gs]
     out["k_update"] = 8 * K * p * s
     # the pair formula of the survey: D*v + D'u over all G FEs with perm+cache each
     out["pair"] = 2 * N * G * (4 + s) + K * (2 * N * s + N * G * s + 3 * p * s) + 4 * (p + G)
@@ -164,16 +170,16 @@ def main():
     # per-kernel durations (HIP events on the solver's stream)
     ms = np.zeros(8)
     check(lib.fehip_time_kernels(feM._h, K, 20, ms.ctypes.data_as(C.POINTER(C.c_double))))
-    names = ["k_db", "k_fin_db", "k_dtx_seg", "k_dtx_dense", "k_dtx_fixup", "k_fin_dtx", "k_update", "k_fin_update"]
+    names = ["k_fused", "k_fin_db", "k_dtx_csr", "k_dtx_dense", "k_dtx_fixup", "k_fin_dtx", "k_update", "k_fin_update"]
     kinds = []
     s_max = int(np.argmax(panel.n))
     for gi, k in enumerate(panel.n):
         kinds.append("sorted" if gi == s_max else ("dense" if k <= 64 else "csr"))
     kb = kernel_bytes(N, len(panel.n), panel.n, kinds, K, 8 if args.precision == "fp64" else 4)
     kt = {nm: float(m) for nm, m in zip(names, ms)}
-    dom = max(["k_db", "k_dtx_seg", "k_dtx_dense", "k_update"], key=lambda q: kt[q])
+    dom = max(["k_fused", "k_dtx_csr", "k_dtx_dense", "k_dtx_fixup", "k_update"], key=lambda q: kt[q])
     achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
-    pair_ms = kt["k_db"] + kt["k_fin_db"] + kt["k_dtx_seg"] + kt["k_dtx_dense"] + kt["k_dtx_fixup"] + kt["k_fin_dtx"]
+    pair_ms = sum(kt[q] for q in names[:6])
     pair_gbs = kb["pair"] / (pair_ms * 1e-3) / 1e9
     iters = int(np.max(its)) - 1  # LSMR iterations (solve_residuals! reports lsmr + 1)
 

@@ -566,19 +566,43 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
 // ---------------------------------------------------------------------------
 // per-K plan: tile schedule, fix-up items, buffers
 // ---------------------------------------------------------------------------
-static int tile_positions(int) { return 1024; }
+// Tiles: one wave per tile, about one tile per resident wave, so every wave
+// of a launch does the same amount of work.  FE s (and the fused pass over
+// it) uses ~4096 tiles, the CSR FEs ~8192 (their gathers want more waves in
+// flight).  Both are multiples of 256 positions.  FEHIP_TILES_S and
+// FEHIP_TILES_C override the counts.
+static int64_t env_int(const char *name, int64_t dflt, int64_t lo, int64_t hi) {
+    const char *e = getenv(name);
+    if (!e || !*e) return dflt;
+    const long long v = atoll(e);
+    return v < lo ? lo : (v > hi ? hi : (int64_t)v);
+}
+static int64_t tile_size(int64_t N, int64_t ntiles) {
+    const int64_t t = (N + ntiles - 1) / ntiles;
+    return std::max<int64_t>(256, (t + 255) / 256 * 256);
+}
 
 struct Plan {
-    int K = 0, TP = 0;
+    int K = 0;
     int32_t total_tiles = 0, nseg = 0, ndense = 0, dense_blocks = 0, dense_chunk = 0, nitems = 0;
     int32_t dense_wpb = 1, dense_lds_wave = 0;
-    std::vector<int> seg_fe, dense_fe;
+    std::vector<int> seg_fe, dense_fe;  // seg_fe[0] is FE s
+    std::vector<int64_t> seg_tp;        // positions per tile, per segmented FE
     std::vector<int32_t> tile0, blk0;
     std::vector<int64_t> drow0;
     FixItem *items = nullptr;
     double *tileA = nullptr, *tileB = nullptr, *dpart = nullptr;
-    unsigned db_blocks = 0, seg_blocks = 0, fix_blocks = 0, upd_blocks = 0;
+    unsigned db_blocks = 0, seg_blocks = 0, csr_blocks = 0, fix_blocks = 0, upd_blocks = 0;
     double *part_db = nullptr, *part_seg = nullptr, *part_fix = nullptr, *part_upd = nullptr;
+    // fused pass (k_fused): FE s tiles of R positions, one per wave
+    int64_t R = 0;
+    int32_t ntiles_s = 0, nslot = 0;
+    bool fused_dense = false;
+    std::vector<int32_t> slot0;
+    unsigned fused_blocks = 0;
+    size_t fused_lds = 0;
+    double *part_fused = nullptr, *wsum = nullptr;
+    uint8_t *sdirect = nullptr;
     void *state = nullptr;
     ColOps *ops_db = nullptr, *ops_dtx = nullptr, *ops_once = nullptr;
     int32_t *all_done = nullptr;
@@ -598,15 +622,22 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
     std::unique_ptr<Plan> pl(new Plan());
     Plan *P = pl.get();
     P->K = K;
-    P->TP = tile_positions(K);
     const int64_t N = h->N;
-    const int64_t ntile_fe = (N + P->TP - 1) / P->TP;
+    P->R = tile_size(N, env_int("FEHIP_TILES_S", 4096, 1, 1 << 20));
+    const int64_t Rc = tile_size(N, env_int("FEHIP_TILES_C", 8192, 1, 1 << 20));
+    P->ntiles_s = (int32_t)((N + P->R - 1) / P->R);
+    P->fused_blocks = grid_for(P->ntiles_s, kWavesPerBlock);
     std::vector<FixItem> items;
+    std::vector<uint8_t> sdirect(h->fe[h->s].n, 0);
     int32_t t0 = 0;
-    for (int g = 0; g < h->G; g++) {
+    std::vector<int> order{h->s};
+    for (int g = 0; g < h->G; g++)
+        if (g != h->s && h->fe[g].kind == KIND_CSR) order.push_back(g);
+    for (int g : order) {
         const FeInfo &f = h->fe[g];
-        if (f.kind == KIND_DENSE) continue;
+        const int64_t tp = g == h->s ? P->R : Rc;
         P->seg_fe.push_back(g);
+        P->seg_tp.push_back(tp);
         P->tile0.push_back(t0);
         for (int64_t k = 0; k < f.n; k++) {
             const int64_t st = f.h_ioff[k], en = f.h_ioff[k + 1];
@@ -618,33 +649,52 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
                 items.push_back(fi);
                 continue;
             }
-            const int64_t ts = st / P->TP, te = (en - 1) / P->TP;
-            if (ts == te) continue;
+            const int64_t tfirst = st / tp, tlast = (en - 1) / tp;
+            if (tfirst == tlast) {
+                if (g == h->s) sdirect[k] = 1;
+                continue;
+            }
             FixItem fi{};
             fi.kind = 0;
             fi.level = (int32_t)k;
             fi.coef_off = h->off[g];
-            fi.first = t0 + ts;
-            fi.count = (int32_t)(te - ts);
+            fi.first = t0 + tfirst;
+            fi.count = (int32_t)(tlast - tfirst);
             items.push_back(fi);
         }
-        t0 += (int32_t)ntile_fe;
+        t0 += (int32_t)((N + tp - 1) / tp);
     }
     P->nseg = (int32_t)P->seg_fe.size();
     P->total_tiles = t0;
-    // dense FEs: one wave per chunk of observations, ~4096 chunks per FE;
-    // waves per workgroup bounded by 64 KB of LDS; each FE's chunk count is a
-    // multiple of the waves per workgroup, so a workgroup never spans two FEs
+    // dense FEs.  If their levels fit the fused pass's LDS (<= 64 KB per
+    // workgroup of 4 waves), the fused pass accumulates them and k_dtx_dense
+    // (used outside the iteration) runs on the same partition: chunk = R,
+    // 4 waves per workgroup.  Otherwise k_dtx_dense does them in every
+    // iteration with ~4096 chunks per FE and waves per workgroup bounded by
+    // 64 KB of LDS.
     const int Pq = kWave / K;
-    int64_t chunk = (N + 4095) / 4096;
-    chunk = std::max<int64_t>(chunk, 16 * Pq);
-    chunk = (chunk + Pq - 1) / Pq * Pq;
-    P->dense_chunk = (int32_t)chunk;
+    const size_t accsz = ts == 8 ? sizeof(double) : sizeof(F32x2);
     int maxlev = 1;
     for (int g = 0; g < h->G; g++)
-        if (h->fe[g].kind == KIND_DENSE) maxlev = std::max<int>(maxlev, (int)h->fe[g].n);
-    P->dense_lds_wave = (int32_t)(ts == 8 ? sizeof(double) : sizeof(F32x2)) * kWave * maxlev;
-    P->dense_wpb = (int32_t)std::max<int64_t>(1, std::min<int64_t>(kWavesPerBlock, 65536 / P->dense_lds_wave));
+        if (h->fe[g].kind == KIND_DENSE) {
+            maxlev = std::max<int>(maxlev, (int)h->fe[g].n);
+            P->slot0.push_back(P->nslot);
+            P->nslot += (int32_t)h->fe[g].n;
+        }
+    P->fused_dense = P->nslot > 0 && (size_t)P->nslot * kWave * accsz * kWavesPerBlock <= 65536;
+    P->dense_lds_wave = (int32_t)(accsz * kWave * maxlev);
+    int64_t chunk;
+    if (P->fused_dense) {
+        chunk = P->R;
+        P->dense_wpb = kWavesPerBlock;
+        P->fused_lds = (size_t)P->nslot * kWave * accsz * kWavesPerBlock;
+    } else {
+        chunk = (N + 4095) / 4096;
+        chunk = std::max<int64_t>(chunk, 16 * Pq);
+        chunk = (chunk + Pq - 1) / Pq * Pq;
+        P->dense_wpb = (int32_t)std::max<int64_t>(1, std::min<int64_t>(kWavesPerBlock, 65536 / P->dense_lds_wave));
+    }
+    P->dense_chunk = (int32_t)chunk;
     int32_t b0 = 0;
     int64_t row0 = 0;
     for (int g = 0; g < h->G; g++) {
@@ -652,6 +702,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
         if (f.kind != KIND_DENSE) continue;
         const int64_t nch = (N + chunk - 1) / chunk;
         const int32_t nblk = (int32_t)((nch + P->dense_wpb - 1) / P->dense_wpb);
+        if (P->fused_dense && (unsigned)nblk != P->fused_blocks) return fail(FEHIP_EHIP, "dense partition mismatch");
         P->dense_fe.push_back(g);
         P->blk0.push_back(b0);
         P->drow0.push_back(row0);
@@ -672,15 +723,24 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
     P->nitems = (int32_t)items.size();
     P->db_blocks = (unsigned)std::min<int64_t>(grid_for((N + Pq - 1) / Pq, 4 * kWavesPerBlock), 2048);
     P->seg_blocks = (unsigned)std::min<int64_t>(grid_for(P->total_tiles, kWavesPerBlock), 2048);
-    P->fix_blocks = (unsigned)std::min<int64_t>(grid_for(P->nitems, kWavesPerBlock), 1024);
+    P->csr_blocks = P->nseg > 1 ? (unsigned)std::min<int64_t>(grid_for(P->total_tiles - P->ntiles_s, kWavesPerBlock),
+                                                             2048)
+                                : 0u;
+    P->fix_blocks = (unsigned)std::min<int64_t>(grid_for(std::max<int64_t>(P->nitems, h->n[h->s] * K / 64),
+                                                         kWavesPerBlock),
+                                                1024);
     P->upd_blocks = (unsigned)std::min<int64_t>(grid_for(h->p * K, kBlock), 1024);
     CHK(dalloc(h, &P->items, std::max<int32_t>(P->nitems, 1)));
     if (P->nitems)
         HIPCHK(hipMemcpyAsync(P->items, items.data(), sizeof(FixItem) * P->nitems, hipMemcpyHostToDevice, h->stream));
+    CHK(dalloc(h, &P->sdirect, sdirect.size()));
+    HIPCHK(hipMemcpyAsync(P->sdirect, sdirect.data(), sdirect.size(), hipMemcpyHostToDevice, h->stream));
     CHK(dalloc(h, &P->tileA, (size_t)std::max<int32_t>(P->total_tiles, 1) * K));
     CHK(dalloc(h, &P->tileB, (size_t)std::max<int32_t>(P->total_tiles, 1) * K));
     CHK(dalloc(h, &P->dpart, (size_t)std::max<int64_t>(row0, 1) * K));
+    CHK(dalloc(h, &P->wsum, (size_t)h->n[h->s] * K));
     CHK(dalloc(h, &P->part_db, (size_t)P->db_blocks * K));
+    CHK(dalloc(h, &P->part_fused, (size_t)P->fused_blocks * K));
     CHK(dalloc(h, &P->part_seg, (size_t)P->seg_blocks * K));
     CHK(dalloc(h, &P->part_fix, (size_t)P->fix_blocks * K));
     CHK(dalloc(h, &P->part_upd, (size_t)P->upd_blocks * K));
@@ -704,7 +764,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
 static void destroy_plan(fehip_solver *h, Plan *P) {
     if (P->graph) (void)hipGraphExecDestroy(P->graph);
     void *ptrs[] = {P->items, P->tileA, P->tileB, P->dpart, P->part_db, P->part_seg, P->part_fix, P->part_upd,
-                    P->state, P->ops_db, P->ops_dtx, P->ops_once, P->all_done, P->u, P->r, P->x,
+                    P->part_fused, P->wsum, P->sdirect, P->state, P->ops_db, P->ops_dtx, P->ops_once, P->all_done, P->u, P->r, P->x,
                     P->v, P->hh, P->hbar, P->dY};
     for (void *q : ptrs) dfree(h, q);
     delete P;
@@ -941,7 +1001,6 @@ struct Engine {
     using TT = T;
     using AccT = Acc;
     using TrT = Tr;
-    static constexpr int TP = 1024;
 
     // target <- b*target + a*A*coef ; mode != NONE leaves ||target||^2 partials
     static void db(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *coef, T *target,
@@ -963,17 +1022,20 @@ struct Engine {
         hipLaunchKernelGGL((k_db<T, K>), dim3(P->db_blocks), dim3(kBlock), 0, h->stream, p);
     }
 
-    // beta = ||u|| and the D'u coefficients
-    static void fin_db(fehip_solver *h, Plan *P, int mode, const int32_t *early_exit) {
-        hipLaunchKernelGGL((k_fin_db<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream, (const double *)P->part_db,
-                           (int64_t)P->db_blocks, mode, P->state, P->ops_dtx, early_exit);
+    // beta = ||u|| and the D'u coefficients (partials of k_db, or of k_fused)
+    static void fin_db(fehip_solver *h, Plan *P, int mode, const int32_t *early_exit, bool fused = false) {
+        hipLaunchKernelGGL((k_fin_db<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream,
+                           (const double *)(fused ? P->part_fused : P->part_db),
+                           (int64_t)(fused ? P->fused_blocks : P->db_blocks), mode, P->state, P->ops_dtx, early_exit);
     }
 
-    // alpha = ||v|| and the plane rotations
-    static void fin_dtx(fehip_solver *h, Plan *P, int mode, const int32_t *early_exit) {
+    // alpha = ||v|| and the plane rotations (partials of k_dtx_seg over all
+    // segmented FEs, or over the CSR FEs only after k_fused, and the fix-up)
+    static void fin_dtx(fehip_solver *h, Plan *P, int mode, const int32_t *early_exit, bool fused = false) {
+        const int64_t n0 = fused ? P->csr_blocks : (P->nseg ? P->seg_blocks : 0);
         hipLaunchKernelGGL((k_fin_dtx<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream, (const double *)P->part_seg,
-                           (int64_t)(P->nseg ? P->seg_blocks : 0), (const double *)P->part_fix,
-                           (int64_t)P->fix_blocks, mode, P->state, P->ops_db, P->all_done, early_exit);
+                           n0, (const double *)P->part_fix, (int64_t)P->fix_blocks, mode, P->state, P->ops_db,
+                           P->all_done, early_exit);
     }
 
     // ||x|| and the stopping rules
@@ -982,13 +1044,15 @@ struct Engine {
                            (int64_t)P->upd_blocks, P->state, P->ops_db, P->all_done);
     }
 
-    // coef <- b*coef + a*A'y : segmented tiles, few-level chunks, fix-up
+    // coef <- b*coef + a*A'y : segmented tiles, few-level chunks, fix-up.
+    // csr_only: the tiles of the CSR FEs only (FE s is done by k_fused).
     static void dtx_seg(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
-                        const int32_t *early_exit) {
-        if (!P->nseg) return;
+                        const int32_t *early_exit, bool csr_only = false) {
+        if (!P->nseg || (csr_only && !P->csr_blocks)) return;
         DtxSegParams<T> p{};
         p.N = h->N;
         p.nseg = P->nseg;
+        p.first_tile = csr_only ? P->ntiles_s : 0;
         p.total_tiles = P->total_tiles;
         for (int q = 0; q < P->nseg; q++) {
             const FeInfo &f = h->fe[P->seg_fe[q]];
@@ -999,8 +1063,9 @@ struct Engine {
             s.wprefix = f.wprefix;
             s.seg_level = f.seg_level;
             s.coef_off = h->off[P->seg_fe[q]];
+            s.tp = P->seg_tp[q];
             s.tile0 = P->tile0[q];
-            s.ntiles = (int32_t)((h->N + P->TP - 1) / P->TP);
+            s.ntiles = (int32_t)((h->N + s.tp - 1) / s.tp);
         }
         p.u = y;
         p.v = coef;
@@ -1010,7 +1075,49 @@ struct Engine {
         p.partials = P->part_seg;
         p.all_done = early_exit;
         p.mode = mode;
-        hipLaunchKernelGGL((k_dtx_seg<T, Acc, K, TP>), dim3(P->seg_blocks), dim3(kBlock), 0, h->stream, p);
+        hipLaunchKernelGGL((k_dtx_seg<T, Acc, K>), dim3(csr_only ? P->csr_blocks : P->seg_blocks), dim3(kBlock), 0,
+                           h->stream, p);
+    }
+
+    // u <- b*u + a*A v with the raw D'u sums of FE s and of the dense FEs
+    static void fused(fehip_solver *h, Plan *P, const ColOps *ops, const T *coef, T *target) {
+        FusedParams<T> p{};
+        p.N = h->N;
+        p.R = P->R;
+        p.G = h->G;
+        p.s = h->s;
+        p.ntiles = P->ntiles_s;
+        for (int g = 0; g < h->G; g++) {
+            p.refs[g] = h->fe[g].ref_int;
+            p.cache[g] = (const T *)h->fe[g].cache_int[pidx<T>()];
+            p.coef_off[g] = h->off[g];
+        }
+        if (P->fused_dense) {
+            p.ndense = P->ndense;
+            p.nslot = P->nslot;
+            for (int q = 0; q < P->ndense; q++) {
+                p.dense_g[q] = P->dense_fe[q];
+                p.dense_slot0[q] = P->slot0[q];
+                p.dense_row0[q] = P->drow0[q];
+            }
+        }
+        const FeInfo &fs = h->fe[h->s];
+        p.heads = fs.heads;
+        p.wprefix = fs.wprefix;
+        p.seg_level = fs.seg_level;
+        p.coef = coef;
+        p.target = target;
+        p.wsum = P->wsum;
+        p.tileA = P->tileA;
+        p.tileB = P->tileB;
+        p.dpart = P->dpart;
+        p.ops = ops;
+        p.partials = P->part_fused;
+        p.all_done = P->all_done;
+        const dim3 gr(P->fused_blocks), bl(kBlock);
+        if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3>), gr, bl, P->fused_lds, h->stream, p);
+        else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2>), gr, bl, P->fused_lds, h->stream, p);
+        else hipLaunchKernelGGL((k_fused<T, Acc, K, 0>), gr, bl, P->fused_lds, h->stream, p);
     }
 
     static void dtx_dense(fehip_solver *h, Plan *P, const ColOps *ops, const T *y, const int32_t *early_exit) {
@@ -1040,7 +1147,7 @@ struct Engine {
     }
 
     static void dtx_fixup(fehip_solver *h, Plan *P, int mode, const ColOps *ops, T *coef,
-                          const int32_t *early_exit) {
+                          const int32_t *early_exit, bool fused = false) {
         FixParams<T> f{};
         f.nitems = P->nitems;
         f.items = P->items;
@@ -1052,6 +1159,12 @@ struct Engine {
         f.partials = P->part_fix;
         f.all_done = early_exit;
         f.mode = mode;
+        if (fused) {
+            f.wsum = P->wsum;
+            f.sdirect = P->sdirect;
+            f.ns = h->n[h->s];
+            f.s_off = h->off[h->s];
+        }
         hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
     }
 
@@ -1091,13 +1204,16 @@ struct Engine {
         };
         // the launches of one LSMR iteration, in their iterating mode; state
         // and vectors hold finite filler
-        out_ms[0] = timed([&] { db(h, P, MODE_ITER, P->ops_once, (const T *)P->v, (T *)P->u, nullptr); });
-        out_ms[1] = timed([&] { fin_db(h, P, MODE_ITER, nullptr); });
-        out_ms[2] = P->nseg ? timed([&] { dtx_seg(h, P, MODE_ITER, P->ops_once, (const T *)P->u, (T *)P->v, nullptr); })
-                            : 0.0;
-        out_ms[3] = P->ndense ? timed([&] { dtx_dense(h, P, P->ops_once, (const T *)P->u, nullptr); }) : 0.0;
-        out_ms[4] = timed([&] { dtx_fixup(h, P, MODE_ITER, P->ops_once, (T *)P->v, nullptr); });
-        out_ms[5] = timed([&] { fin_dtx(h, P, MODE_ITER, nullptr); });
+        out_ms[0] = timed([&] { fused(h, P, P->ops_once, (const T *)P->v, (T *)P->u); });
+        out_ms[1] = timed([&] { fin_db(h, P, MODE_ITER, nullptr, true); });
+        out_ms[2] = P->csr_blocks
+                        ? timed([&] { dtx_seg(h, P, MODE_ITER, P->ops_once, (const T *)P->u, (T *)P->v, nullptr, true); })
+                        : 0.0;
+        out_ms[3] = (P->ndense && !P->fused_dense)
+                        ? timed([&] { dtx_dense(h, P, P->ops_once, (const T *)P->u, nullptr); })
+                        : 0.0;
+        out_ms[4] = timed([&] { dtx_fixup(h, P, MODE_ITER, P->ops_once, (T *)P->v, nullptr, true); });
+        out_ms[5] = timed([&] { fin_dtx(h, P, MODE_ITER, nullptr, true); });
         hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, K, 0.0,
                            0.0, 0.0, (int64_t)1 << 40, P->all_done);
         out_ms[6] = timed([&] { update(h, P); });
@@ -1133,12 +1249,17 @@ struct Engine {
         hipLaunchKernelGGL((k_update<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, p);
     }
 
-    // one LSMR iteration (src/utils/lsmr.jl:103-215) as 8 stream-ordered launches
+    // one LSMR iteration (src/utils/lsmr.jl:103-215) as stream-ordered
+    // launches: the fused D*x pass (with D'u of FE s and the dense FEs),
+    // beta, D'u of the CSR FEs, the fix-up, alpha and the rotations, the
+    // vector update, the stopping rules
     static void iteration(fehip_solver *h, Plan *P) {
-        db(h, P, MODE_ITER, P->ops_db, (const T *)P->v, (T *)P->u, P->all_done);
-        fin_db(h, P, MODE_ITER, P->all_done);
-        dtx(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done);
-        fin_dtx(h, P, MODE_ITER, P->all_done);
+        fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
+        fin_db(h, P, MODE_ITER, P->all_done, true);
+        if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done);
+        dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true);
+        dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
+        fin_dtx(h, P, MODE_ITER, P->all_done, true);
         update(h, P);
         fin_update(h, P);
     }

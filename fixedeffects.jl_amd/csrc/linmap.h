@@ -296,46 +296,44 @@ __global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
     constexpr int U = 4;  // wave steps in flight per lane
     const int lane = threadIdx.x & (kWave - 1);
     const int psub = lane / K, c = lane % K;
-    const int on = p.ops->on[c];
+    const bool on = p.ops->on[c] != 0;
     const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
     const bool keep = p.ops->b[c] != 0.0;
     const int64_t nrounds = (p.N + P - 1) / P;
     const int64_t gw = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
     const int64_t nw = (int64_t)gridDim.x * kWavesPerBlock;
     double sq = 0.0;
-    if (on) {
-        for (int64_t r0 = gw * U; r0 < nrounds; r0 += nw * U) {
-            int64_t idx[U];
-            T acc[U];
+    // loads are unconditional (positions past N read position 0), so no
+    // load waits inside a branch; only the stores are predicated
+    for (int64_t r0 = gw * U; r0 < nrounds; r0 += nw * U) {
+        int64_t idx[U], ic[U];
+        T acc[U], uo[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            idx[q] = (r0 + q) * P + psub;
+            ic[q] = idx[q] < p.N ? idx[q] : 0;
+            acc[q] = (T)0;
+            uo[q] = p.target[ic[q] * K + c];
+        }
+        for (int g = 0; g < p.G; g++) {
+            int32_t r[U];
+            T cv[U];
 #pragma unroll
             for (int q = 0; q < U; q++) {
-                idx[q] = (r0 + q) * P + psub;
-                acc[q] = (T)0;
+                r[q] = __builtin_nontemporal_load(p.refs[g] + ic[q]);
+                cv[q] = __builtin_nontemporal_load(p.cache[g] + ic[q]);
             }
-            for (int g = 0; g < p.G; g++) {
-                int32_t r[U];
-                T cv[U];
+            const T *cf = p.coef + p.coef_off[g] * K + c;
 #pragma unroll
-                for (int q = 0; q < U; q++) {
-                    if (idx[q] < p.N) {
-                        r[q] = __builtin_nontemporal_load(p.refs[g] + idx[q]);
-                        cv[q] = __builtin_nontemporal_load(p.cache[g] + idx[q]);
-                    }
-                }
-                const T *cf = p.coef + p.coef_off[g] * K + c;
+            for (int q = 0; q < U; q++) acc[q] += cv[q] * cf[(int64_t)r[q] * K];
+        }
 #pragma unroll
-                for (int q = 0; q < U; q++)
-                    if (idx[q] < p.N) acc[q] += cv[q] * cf[(int64_t)r[q] * K];
-            }
-#pragma unroll
-            for (int q = 0; q < U; q++) {
-                if (idx[q] < p.N) {
-                    T *tp = p.target + idx[q] * K + c;
-                    T out = a * acc[q];
-                    if (keep) out = b * (*tp) + out;
-                    *tp = out;
-                    sq += (double)out * (double)out;
-                }
+        for (int q = 0; q < U; q++) {
+            T out = a * acc[q];
+            out = keep ? b * uo[q] + out : out;
+            if (on && idx[q] < p.N) {
+                p.target[idx[q] * K + c] = out;
+                sq += (double)out * (double)out;
             }
         }
     }
@@ -349,8 +347,8 @@ __global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
 // D'y: coef[k,c] <- b[c]*coef[k,c] + a[c] * sum_{i in level k} cache[i]*y[i,c]
 // (reference gather!/mul!', src/SolverCPU.jl:21-27, src/AbstractFixedEffectLinearMap.jl:26-35)
 //
-// Segmented FEs (FE s and CSR FEs) are cut into tiles of TP CSR positions,
-// one wave per tile.  A segment that lies inside a tile is reduced and
+// Segmented FEs (FE s and CSR FEs) are cut into tiles of tp CSR positions
+// (one size per FE), one wave per tile.  A segment that lies inside a tile is reduced and
 // written by that wave.  A segment crossing tile boundaries leaves a partial
 // in its first tile (tileB) and in each later tile it covers (tileA); the
 // fix-up launch combines them in tile order.
@@ -363,13 +361,14 @@ struct SegFE {
     const uint32_t *wprefix;  // heads before each 64-position word
     const int32_t *seg_level; // ordinal of non-empty segment -> level
     int64_t coef_off;
+    int64_t tp;               // positions per tile (a multiple of 256)
     int32_t tile0, ntiles;
 };
 
 template <typename T>
 struct DtxSegParams {
     int64_t N;
-    int32_t nseg, total_tiles;
+    int32_t nseg, first_tile, total_tiles, pad;  // tiles [first_tile, total_tiles)
     SegFE<T> fe[kMaxG];
     const T *u;
     T *v;
@@ -380,7 +379,74 @@ struct DtxSegParams {
     int mode;
 };
 
-template <typename T, typename Acc, int K, int TP>
+// A wave's walk over one tile of segmented positions.
+template <typename Acc>
+struct SegCursor {
+    Acc carry;  // open run carried into the next step
+    bool cont;  // no segment head seen yet in this tile
+    bool open;  // a run is still open after the last step
+};
+
+// One wave step at `base` (P = 64/K positions, base a multiple of P): the
+// run carried in is added and the step is scanned in place (x).  Returns true
+// on the lane where a segment that started in this tile ends (the caller
+// writes it); a segment that began in an earlier tile ends in *tileA_tc.
+// w = heads[base >> 6], wnext = heads[(base >> 6) + 1], loaded by the caller
+// together with the step's data, so no load sits on the scan's chain.
+template <int K, typename Acc>
+__device__ __forceinline__ bool seg_step(uint64_t w, uint64_t wnext, int64_t base, int64_t end, int psub, int c,
+                                         bool on, Acc &x, SegCursor<Acc> &cur, double *tileA_tc) {
+    constexpr int P = kWave / K;
+    const int64_t pos = base + psub;
+    const bool valid = pos < end;
+    const int sh = (int)(base & 63);
+    const uint64_t hb = (P == 64) ? w : ((w >> sh) & ((1ull << (P & 63)) - 1ull));
+    const uint64_t nb = (sh + P < 64) ? ((w >> (sh + P)) & 1ull) : (wnext & 1ull);
+    const uint64_t hle = hb & lanes_le(psub);
+    const int rs = hle ? (63 - __clzll(hle)) : 0;
+    if (psub == 0 && !(hb & 1ull)) x += cur.carry;
+    x = seg_scan<K>(x, psub, rs);
+    const bool isend = (psub == P - 1) ? (nb != 0) : (((hb >> (psub + 1)) & 1ull) != 0);
+    const bool mycont = cur.cont && (hle == 0);
+    bool em = false;
+    if (valid && on && isend) {
+        if (mycont) *tileA_tc = acc_to_double(x);
+        else em = true;
+    }
+    const Acc last = __shfl(x, (P - 1) * K + c, kWave);
+    cur.open = (nb == 0);
+    cur.carry = cur.open ? last : Acc(0.f);
+    cur.cont = cur.cont && (hb == 0);
+    return em;
+}
+
+// acc[slot[q]*64 + lane] += x[q] for q = 0..U-1 in this order, with one LDS
+// round trip: all slots are read first, repeated slots are chained in
+// registers, and the writes go out in order (the last write to a repeated
+// slot carries every addition, in the same order as one-by-one updates).
+template <int U, typename Acc>
+__device__ __forceinline__ void lds_accumulate(Acc *acc, const int32_t (&slot)[U], const Acc (&x)[U], int lane) {
+    Acc old[U], nv[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) old[q] = acc[slot[q] * kWave + lane];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+        Acc base = old[q];
+#pragma unroll
+        for (int q2 = 0; q2 < q; q2++) base = slot[q2] == slot[q] ? nv[q2] : base;
+        nv[q] = base + x[q];
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) acc[slot[q] * kWave + lane] = nv[q];
+}
+
+// level of the segment holding position pos (w = head word of pos,
+// wp = wprefix of that word)
+__device__ __forceinline__ int32_t seg_level_of(uint32_t wp, const int32_t *seg_level, int64_t pos, uint64_t w) {
+    return seg_level[wp + (uint32_t)__popcll(w & lanes_le((int)(pos & 63))) - 1u];
+}
+
+template <typename T, typename Acc, int K>
 __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
     if (p.all_done && *p.all_done) return;
     constexpr int P = kWave / K;
@@ -393,69 +459,75 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
     const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
     const int nw = gridDim.x * kWavesPerBlock;
     double sq = 0.0;
-    for (int t = gw; t < p.total_tiles; t += nw) {
+    for (int t = p.first_tile + gw; t < p.total_tiles; t += nw) {
         int f = 0;
         while (f + 1 < p.nseg && t >= p.fe[f + 1].tile0) f++;
         const SegFE<T> fe = p.fe[f];
-        const int64_t start = (int64_t)(t - fe.tile0) * TP;
-        const int64_t end = start + TP < p.N ? start + TP : p.N;
+        const int64_t start = (int64_t)(t - fe.tile0) * fe.tp;
+        const int64_t end = start + fe.tp < p.N ? start + fe.tp : p.N;
         T *v = p.v + fe.coef_off * K;
-        Acc carry = Acc(0.f);
-        bool cont = true;    // no segment head seen yet in this tile
-        bool open = false;   // a run is still open after the last step
+        SegCursor<Acc> cur{Acc(0.f), true, false};
         for (int64_t base0 = start; base0 < end; base0 += U * P) {
             Acc val[U];
+            uint64_t hw[U], hn[U];
+            uint32_t wp[U];
+            // unconditional loads (positions past the tile read its first
+            // position): every load of the step is in flight together
 #pragma unroll
             for (int q = 0; q < U; q++) {
-                const int64_t pos = base0 + q * P + psub;
-                val[q] = Acc(0.f);
-                if (pos < end && on) {
-                    const int64_t i = fe.perm ? (int64_t)__builtin_nontemporal_load(fe.perm + pos) : pos;
-                    const T cv = __builtin_nontemporal_load(fe.cache + pos);
-                    val[q] = mul_acc<Acc, T>(cv, p.u[i * K + c]);
+                const int64_t base = base0 + q * P;
+                const int64_t pos = base + psub;
+                hw[q] = hn[q] = 0;
+                wp[q] = 0;
+                if (base < end) {  // wave-uniform
+                    hw[q] = fe.heads[base >> 6];
+                    hn[q] = fe.heads[(base >> 6) + 1];
+                    wp[q] = fe.wprefix[base >> 6];
                 }
+                const int64_t pc = pos < end ? pos : start;
+                const int64_t i = fe.perm ? (int64_t)__builtin_nontemporal_load(fe.perm + pc) : pc;
+                const T cv = __builtin_nontemporal_load(fe.cache + pc);
+                const Acc pr = mul_acc<Acc, T>(cv, p.u[i * K + c]);
+                val[q] = (pos < end && on) ? pr : Acc(0.f);
             }
+            // every em[q] is defined before the scan, which stops at the
+            // first step past the tile's end
+            bool em[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) em[q] = false;
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const int64_t base = base0 + q * P;
                 if (base >= end) break;  // wave-uniform
-                const int64_t pos = base + psub;
-                const bool valid = pos < end;
-                const uint64_t w = fe.heads[base >> 6];
-                const int sh = (int)(base & 63);
-                const uint64_t hb = (P == 64) ? w : ((w >> sh) & ((1ull << (P & 63)) - 1ull));
-                const uint64_t nb =
-                    (sh + P < 64) ? ((w >> (sh + P)) & 1ull) : (fe.heads[(base >> 6) + 1] & 1ull);
-                const uint64_t hle = hb & lanes_le(psub);
-                const int rs = hle ? (63 - __clzll(hle)) : 0;
-                Acc x = val[q];
-                if (psub == 0 && !(hb & 1ull)) x += carry;
-                x = seg_scan<K>(x, psub, rs);
-                const bool isend = (psub == P - 1) ? (nb != 0) : (((hb >> (psub + 1)) & 1ull) != 0);
-                const bool mycont = cont && (hle == 0);
-                if (valid && on && isend) {
-                    if (mycont) {
-                        p.tileA[(int64_t)t * K + c] = acc_to_double(x);
-                    } else {
-                        const uint32_t ord = fe.wprefix[pos >> 6] +
-                                             (uint32_t)__popcll(w & lanes_le((int)(pos & 63))) - 1u;
-                        const int32_t lev = fe.seg_level[ord];
-                        T *dst = v + (int64_t)lev * K + c;
-                        T out = a * acc_to_T<T, Acc>(x);
-                        if (keep) out = b * (*dst) + out;
-                        *dst = out;
-                        sq += (double)out * (double)out;
-                    }
+                em[q] = seg_step<K>(hw[q], hn[q], base, end, psub, c, on, val[q], cur, &p.tileA[(int64_t)t * K + c]);
+            }
+            // the segments that ended in these steps: levels, then v, then stores
+            int32_t lev[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t pos = base0 + q * P + psub;
+                const uint32_t ord = wp[q] + (uint32_t)__popcll(hw[q] & lanes_le((int)(pos & 63))) - 1u;
+                lev[q] = fe.seg_level[em[q] ? ord : 0u];
+            }
+            T old[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const T ov = v[(int64_t)lev[q] * K + c];
+                old[q] = keep ? ov : (T)0;
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                if (em[q]) {
+                    T out = a * acc_to_T<T, Acc>(val[q]);
+                    if (keep) out = b * old[q] + out;
+                    v[(int64_t)lev[q] * K + c] = out;
+                    sq += (double)out * (double)out;
                 }
-                const Acc last = __shfl(x, (P - 1) * K + c, kWave);
-                open = (nb == 0);
-                carry = open ? last : Acc(0.f);
-                cont = cont && (hb == 0);
             }
         }
-        if (open && on && psub == 0) {
-            if (cont) p.tileA[(int64_t)t * K + c] = acc_to_double(carry);
-            else p.tileB[(int64_t)t * K + c] = acc_to_double(carry);
+        if (cur.open && on && psub == 0) {
+            if (cur.cont) p.tileA[(int64_t)t * K + c] = acc_to_double(cur.carry);
+            else p.tileB[(int64_t)t * K + c] = acc_to_double(cur.carry);
         }
     }
     if (p.mode == MODE_NONE) return;
@@ -507,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
     const int on = p.ops->on[c];
     const int64_t start = ((int64_t)bl * wpb + wave) * p.chunk;
     const int64_t end = start + p.chunk < p.N ? start + p.chunk : p.N;
-    if (on) {
+    if (on && start < end) {
         constexpr int U = 4;
         for (int64_t base0 = start; base0 < end; base0 += U * P) {
             int32_t r[U];
@@ -515,16 +587,13 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const int64_t pos = base0 + q * P + psub;
-                r[q] = -1;
-                if (pos < end) {
-                    r[q] = __builtin_nontemporal_load(fe.refs + pos);
-                    const T cv = __builtin_nontemporal_load(fe.cache + pos);
-                    x[q] = mul_acc<Acc, T>(cv, p.u[pos * K + c]);
-                }
+                const int64_t pc = pos < end ? pos : start;
+                r[q] = __builtin_nontemporal_load(fe.refs + pc);
+                const T cv = __builtin_nontemporal_load(fe.cache + pc);
+                const T uv = p.u[pc * K + c];
+                x[q] = pos < end ? mul_acc<Acc, T>(cv, uv) : Acc(0.f);
             }
-#pragma unroll
-            for (int q = 0; q < U; q++)
-                if (r[q] >= 0) acc[r[q] * kWave + lane] += x[q];
+            lds_accumulate<U, Acc>(acc, r, x, lane);
         }
     }
     // Per-wave level sums over the lanes of each column.  The double result
@@ -549,6 +618,238 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// One pass over the internal order for an LSMR iteration:
+//   u <- b*u + a*A v                       (D*x of every FE, as k_db)
+//   wsum[k] = sum_{i in level k of FE s} cache_s[i]*u[i]   (raw D_s'u)
+//   dense FEs: per-workgroup level sums of cache_g*u  -> dpart rows
+//   ||u||^2 per workgroup                  -> partials
+// D'u is linear, so its raw sums can be formed before beta = ||u|| is known;
+// the fix-up launch applies v <- a*wsum + b*v once beta is.  This reads the
+// index data, caches and u once per iteration for FE s and the dense FEs
+// instead of three times.  One wave per tile of R positions (tile t = wave t
+// of the grid); tiles crossing FE s segments leave tileA/tileB pieces exactly
+// as k_dtx_seg with tp = R does, and the dense rows are those k_dtx_dense
+// writes with chunk = R and 4 waves per workgroup, so one set of fix-up items
+// serves both paths.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct FusedParams {
+    int64_t N, R;
+    int32_t G, s, ntiles, ndense, nslot, pad;
+    const int32_t *refs[kMaxG];
+    const T *cache[kMaxG];  // internal order
+    int64_t coef_off[kMaxG];
+    int32_t dense_g[kMaxG];      // dense FE ordinal -> FE index
+    int32_t dense_slot0[kMaxG];  // its first LDS slot
+    int64_t dense_row0[kMaxG];   // its first dpart row
+    const uint64_t *heads;       // FE s
+    const uint32_t *wprefix;
+    const int32_t *seg_level;
+    const T *coef;     // v
+    T *target;         // u
+    double *wsum;      // p_s x K
+    double *tileA, *tileB;
+    double *dpart;
+    const ColOps *ops;  // the D*x coefficients
+    double *partials;   // gridDim.x x K
+    const int32_t *all_done;
+};
+
+#ifndef FUSED_U64
+#define FUSED_U64 2
+#endif
+// GT > 0: the FE count is the compile-time GT and every load of a wave step
+// (all FEs' refs and caches, u, the head words) is issued before the first
+// coefficient gather, so a step waits on memory twice.  GT = 0: any FE count,
+// one FE at a time.
+template <typename T, typename Acc, int K, int GT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(FusedParams<T> p) {
+    if (*p.all_done) return;
+    extern __shared__ __align__(16) unsigned char fused_smem[];
+    constexpr int P = kWave / K;
+    // wave steps per loop step: all their loads are live at once, so FP64
+    // takes fewer to stay within 128 VGPRs (4 waves per SIMD) without spills
+    constexpr int U = sizeof(T) == 8 ? FUSED_U64 : 4;
+    // head words one loop step touches: steps q use word (q*P)/64 and the next
+    constexpr int NW = (U * P) / kWave + 1 > 2 ? (U * P) / kWave + 1 : 2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
+    const int psub = lane / K, c = lane % K;
+    const bool on = p.ops->on[c] != 0;
+    const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
+    const bool keep = p.ops->b[c] != 0.0;
+    Acc *acc = reinterpret_cast<Acc *>(fused_smem) + (size_t)wave * p.nslot * kWave;
+    for (int k = 0; k < p.nslot; k++) acc[k * kWave + lane] = Acc(0.f);
+    const int t = blockIdx.x * kWavesPerBlock + wave;
+    double sq = 0.0;
+    if (t < p.ntiles) {  // wave-uniform
+        const int64_t start = (int64_t)t * p.R;
+        const int64_t end = start + p.R < p.N ? start + p.R : p.N;
+        SegCursor<Acc> cur{Acc(0.f), true, false};
+        for (int64_t base0 = start; base0 < end; base0 += U * P) {
+            // loads are unconditional (positions past the tile read its
+            // first position), so every load of the step is in flight at
+            // once and no wait sits inside a branch; stores are predicated
+            int64_t idx[U], ic[U];
+            bool ok[U];
+            T sum[U], uo[U], cs[U], cd[U];
+            int32_t rs[U], rd[U];
+            uint64_t hwv[NW];
+            // base0 is a multiple of U*P and of 64 when U*P >= 64; the words
+            // up to (end-1)/64 + 1 exist (the head bits run one word past N)
+#pragma unroll
+            for (int j = 0; j < NW; j++) {
+                const int64_t wi = (base0 >> 6) + j;
+                hwv[j] = wi <= ((end - 1) >> 6) + 1 ? p.heads[wi] : 0ull;
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                idx[q] = base0 + q * P + psub;
+                ok[q] = on && idx[q] < end;
+                ic[q] = idx[q] < end ? idx[q] : start;
+                sum[q] = (T)0;
+                uo[q] = p.target[ic[q] * K + c];
+                cs[q] = cd[q] = (T)0;
+                rs[q] = rd[q] = 0;
+            }
+            if constexpr (GT > 0) {
+                int32_t r[GT][U];
+                T cv[GT][U];
+#pragma unroll
+                for (int g = 0; g < GT; g++)
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        r[g][q] = p.refs[g][ic[q]];
+                        cv[g][q] = p.cache[g][ic[q]];
+                    }
+#pragma unroll
+                for (int g = 0; g < GT; g++) {
+                    const T *cf = p.coef + p.coef_off[g] * K + c;
+#pragma unroll
+                    for (int q = 0; q < U; q++) sum[q] += cv[g][q] * cf[(int64_t)r[g][q] * K];
+                }
+#pragma unroll
+                for (int g = 0; g < GT; g++) {
+                    if (g == p.s) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) {
+                            cs[q] = cv[g][q];
+                            rs[q] = r[g][q];
+                        }
+                    }
+                    if (p.ndense && g == p.dense_g[0]) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) {
+                            cd[q] = cv[g][q];
+                            rd[q] = r[g][q];
+                        }
+                    }
+                }
+            } else {
+                for (int g = 0; g < p.G; g++) {
+                    int32_t r[U];
+                    T cv[U];
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        r[q] = p.refs[g][ic[q]];
+                        cv[q] = p.cache[g][ic[q]];
+                    }
+                    const T *cf = p.coef + p.coef_off[g] * K + c;
+#pragma unroll
+                    for (int q = 0; q < U; q++) sum[q] += cv[q] * cf[(int64_t)r[q] * K];
+                    if (g == p.s) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) {
+                            cs[q] = cv[q];
+                            rs[q] = r[q];
+                        }
+                    }
+                    if (p.ndense && g == p.dense_g[0]) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) {
+                            cd[q] = cv[q];
+                            rd[q] = r[q];
+                        }
+                    }
+                }
+            }
+            T un[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                T out = a * sum[q];
+                out = keep ? b * uo[q] + out : out;
+                un[q] = ok[q] ? out : (T)0;
+                if (ok[q]) p.target[idx[q] * K + c] = out;
+                sq += (double)un[q] * (double)un[q];
+            }
+            // FE s: segmented sums of cache_s * u; a segment ending at a
+            // position belongs to that position's level
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t base = base0 + q * P;
+                if (base >= end) break;  // wave-uniform
+                Acc x = mul_acc<Acc, T>(cs[q], un[q]);
+                const int jw = (q * P) / kWave;  // constant after unrolling
+                if (seg_step<K>(hwv[jw], hwv[jw + 1], base, end, psub, c, on, x, cur, &p.tileA[(int64_t)t * K + c]))
+                    p.wsum[(int64_t)rs[q] * K + c] = acc_to_double(x);
+            }
+            // dense FEs: lane-private LDS accumulators
+            if (p.ndense) {
+                const int s0 = p.dense_slot0[0];
+                int32_t sl[U];
+                Acc x[U];
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    sl[q] = s0 + rd[q];
+                    x[q] = mul_acc<Acc, T>(cd[q], un[q]);
+                }
+                lds_accumulate<U, Acc>(acc, sl, x, lane);
+                for (int d = 1; d < p.ndense; d++) {
+                    const int32_t *rf = p.refs[p.dense_g[d]];
+                    const T *cf = p.cache[p.dense_g[d]];
+                    const int sd = p.dense_slot0[d];
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        sl[q] = sd + rf[ic[q]];
+                        x[q] = mul_acc<Acc, T>(cf[ic[q]], un[q]);
+                    }
+                    lds_accumulate<U, Acc>(acc, sl, x, lane);
+                }
+            }
+        }
+        if (cur.open && on && psub == 0) {
+            if (cur.cont) p.tileA[(int64_t)t * K + c] = acc_to_double(cur.carry);
+            else p.tileB[(int64_t)t * K + c] = acc_to_double(cur.carry);
+        }
+    }
+    if (p.ndense) {
+        // per-wave slot sums (written in place, see k_dtx_dense), then the
+        // waves of the workgroup in order -> one dpart row per workgroup
+        double *ws = reinterpret_cast<double *>(acc);
+        for (int k = 0; k < p.nslot; k++) {
+            Acc x = acc[k * kWave + lane];
+            x = column_allreduce<K>(x);
+            if (psub == 0) ws[k * K + c] = acc_to_double(x);
+        }
+        __syncthreads();
+        const size_t region = (size_t)p.nslot * kWave * sizeof(Acc) / sizeof(double);
+        const double *base_ws = reinterpret_cast<const double *>(fused_smem);
+        for (int e = threadIdx.x; e < p.nslot * K; e += blockDim.x) {
+            double s = 0.0;
+            for (int w = 0; w < kWavesPerBlock; w++) s += base_ws[w * region + e];
+            const int slot = e / K, cc = e % K;
+            int d = 0;
+            while (d + 1 < p.ndense && slot >= p.dense_slot0[d + 1]) d++;
+            const int k = slot - p.dense_slot0[d];
+            p.dpart[(p.dense_row0[d] + (int64_t)k * gridDim.x + blockIdx.x) * K + cc] = s;
+        }
+    }
+    __shared__ double red[kBlock];
+    const double s = block_column_sum<K>(sq, red);
+    if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+}
+
 // Fix-up items: kind 0 = a segment crossing tiles (total = tileB[first] +
 // sum_{q=1..count} tileA[first+q]); kind 1 = a dense level (total = sum of
 // count dpart rows from row first); kind 2 = a level no observation has
@@ -571,6 +872,11 @@ struct FixParams {
     double *partials;  // gridDim.x x K
     const int32_t *all_done;
     int mode;
+    // after k_fused: v_s[k] <- b*v_s[k] + a*wsum[k] for the levels of FE s
+    // whose segment lies inside one tile (sdirect[k] = 1); nullptr = none
+    const double *wsum;
+    const uint8_t *sdirect;
+    int64_t ns, s_off;
 };
 
 template <typename T, int K>
@@ -585,6 +891,18 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
     const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
     const bool keep = p.ops->b[c] != 0.0;
     double sq = 0.0;
+    if (p.wsum) {
+        // stride is a multiple of K, so element e belongs to column c
+        const int64_t nsK = p.ns * K;
+        for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nsK; e += (int64_t)gridDim.x * kBlock) {
+            if (!on || !p.sdirect[e / K]) continue;
+            T *dst = p.v + p.s_off * K + e;
+            T out = a * (T)p.wsum[e];
+            if (keep) out = b * (*dst) + out;
+            *dst = out;
+            sq += (double)out * (double)out;
+        }
+    }
     for (int item = gw; item < p.nitems; item += nw) {
         const FixItem it = p.items[item];
         double s = 0.0;

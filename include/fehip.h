@@ -199,11 +199,14 @@ int fehip_lsmr_info(fehip_t *h, int32_t col, double *out);
  * each launch of one LSMR iteration on K columns, from HIP events around
  * `reps` isolated launches on the handle's stream, in the launch
  * configuration the solve uses, with the solve vectors holding filler:
- *   out_ms[0] D*v          (the reference's scatter!, src/SolverCPU.jl:29-34)
+ *   out_ms[0] one pass: u = A v - alpha u (scatter!, src/SolverCPU.jl:29-34)
+ *             with the raw D'u sums of the FE with most levels and of the
+ *             few-level FEs (gather!, src/SolverCPU.jl:21-27)
  *   out_ms[1] beta = ||u|| finalize
- *   out_ms[2] D'u, FEs with many levels (gather!, src/SolverCPU.jl:21-27)
- *   out_ms[3] D'u, FEs with few levels
- *   out_ms[4] D'u fix-up of segments split across tiles
+ *   out_ms[2] D'u of the other many-level FEs (CSR-ordered gathers)
+ *   out_ms[3] D'u of few-level FEs when they do not fit the pass (else 0)
+ *   out_ms[4] fix-up: v = A'u/beta - beta v for the first FE's levels, and
+ *             the segments split across tiles
  *   out_ms[5] alpha = ||v|| and plane-rotation finalize
  *   out_ms[6] LSMR vector update (src/utils/lsmr.jl:138-142, :192)
  *   out_ms[7] ||x|| and stopping-rule finalize (one launch) */
