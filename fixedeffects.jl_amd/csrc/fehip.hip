@@ -588,6 +588,7 @@ struct Plan {
     int32_t dense_wpb = 1, dense_lds_wave = 0;
     std::vector<int> seg_fe, dense_fe;  // seg_fe[0] is FE s
     std::vector<int64_t> seg_tp;        // positions per tile, per segmented FE
+    std::vector<int32_t> seg_nsegs;     // non-empty levels, per segmented FE
     std::vector<int32_t> tile0, blk0;
     std::vector<int64_t> drow0;
     FixItem *items = nullptr;
@@ -612,7 +613,9 @@ struct Plan {
     int graph_iters = 0;
 };
 
-static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
+// waves_f / waves_c: how many waves of the fused pass / of the CSR D'u launch
+// the device holds at once; each gets one tile
+static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves_f, int64_t waves_c) {
     const int key = K * 16 + (int)ts;
     auto it = h->plans.find(key);
     if (it != h->plans.end()) {
@@ -623,8 +626,8 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
     Plan *P = pl.get();
     P->K = K;
     const int64_t N = h->N;
-    P->R = tile_size(N, env_int("FEHIP_TILES_S", 4096, 1, 1 << 20));
-    const int64_t Rc = tile_size(N, env_int("FEHIP_TILES_C", 8192, 1, 1 << 20));
+    P->R = tile_size(N, env_int("FEHIP_TILES_S", waves_f, 1, 1 << 20));
+    const int64_t Rc = tile_size(N, env_int("FEHIP_TILES_C", waves_c, 1, 1 << 20));
     P->ntiles_s = (int32_t)((N + P->R - 1) / P->R);
     P->fused_blocks = grid_for(P->ntiles_s, kWavesPerBlock);
     std::vector<FixItem> items;
@@ -639,6 +642,9 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
         P->seg_fe.push_back(g);
         P->seg_tp.push_back(tp);
         P->tile0.push_back(t0);
+        int32_t nonempty = 0;
+        for (int64_t k = 0; k < f.n; k++) nonempty += f.h_ioff[k + 1] > f.h_ioff[k] ? 1 : 0;
+        P->seg_nsegs.push_back(nonempty);
         for (int64_t k = 0; k < f.n; k++) {
             const int64_t st = f.h_ioff[k], en = f.h_ioff[k + 1];
             if (en <= st) {  // level without observations: coef <- b*coef only
@@ -759,6 +765,46 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out) {
     h->plans[key] = pl.release();
     *out = P;
     return 0;
+}
+
+// LDS of the fused pass: lane-private accumulators of the few-level FEs
+// when they fit 64 KB per workgroup (the rule get_plan applies)
+static size_t fused_lds_bytes(const fehip_solver *h, size_t ts) {
+    const size_t accsz = ts == 8 ? sizeof(double) : sizeof(F32x2);
+    int64_t nslot = 0;
+    for (int g = 0; g < h->G; g++)
+        if (h->fe[g].kind == KIND_DENSE) nslot += h->fe[g].n;
+    const size_t b = (size_t)nslot * kWave * accsz * kWavesPerBlock;
+    return b <= 65536 ? b : 0;
+}
+
+// resident waves of a kernel on this device (all CUs, kBlock threads)
+static int64_t resident_waves(const void *kernel, size_t lds) {
+    int dev = 0, cus = 0, nb = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, lds) != hipSuccess || nb < 1 || cus < 1) {
+        (void)hipGetLastError();
+        return 4096;
+    }
+    return (int64_t)nb * cus * kWavesPerBlock;
+}
+
+// the plan of one engine: tiles sized by the occupancy of its kernels
+template <typename T, typename Acc, int K>
+static int get_plan_for(fehip_solver *h, Plan **out) {
+    const int key = K * 16 + (int)sizeof(T);
+    auto it = h->plans.find(key);
+    if (it != h->plans.end()) {
+        *out = it->second;
+        return 0;
+    }
+    const void *kf = h->G == 3   ? (const void *)k_fused<T, Acc, K, 3>
+                     : h->G == 2 ? (const void *)k_fused<T, Acc, K, 2>
+                                 : (const void *)k_fused<T, Acc, K, 0>;
+    const int64_t wf = resident_waves(kf, fused_lds_bytes(h, sizeof(T)));
+    const int64_t wc = resident_waves((const void *)k_dtx_seg<T, Acc, K>, 0);
+    return get_plan(h, K, sizeof(T), out, wf, wc);
 }
 
 static void destroy_plan(fehip_solver *h, Plan *P) {
@@ -1061,7 +1107,8 @@ struct Engine {
             s.cache = (const T *)f.cache_csr[pidx<T>()];
             s.heads = f.heads;
             s.wprefix = f.wprefix;
-            s.seg_level = f.seg_level;
+            s.nsegs = std::max<int32_t>(P->seg_nsegs[q], 1);
+            s.seg_level = P->seg_nsegs[q] == f.n ? nullptr : f.seg_level;
             s.coef_off = h->off[P->seg_fe[q]];
             s.tp = P->seg_tp[q];
             s.tile0 = P->tile0[q];
@@ -1179,7 +1226,7 @@ struct Engine {
     // isolated launches (HIP events on the handle's stream).
     static int time_kernels(fehip_solver *h, int reps, double *out_ms) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, sizeof(T), &P));
+        CHK((get_plan_for<T, Acc, K>(h, &P)));
         const int64_t NK = h->N * K, pK = h->p * K;
         hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->u, NK, (T)1);
         void *pv[] = {P->x, P->v, P->hh, P->hbar};
@@ -1385,7 +1432,7 @@ struct Engine {
     static int solve_residuals(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
                                int64_t *iters, uint8_t *converged) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, sizeof(T), &P));
+        CHK((get_plan_for<T, Acc, K>(h, &P)));
         CHK(start(h, P, dY, Kreal, ldY, true));
         std::vector<int64_t> it(Kreal, 0);
         std::vector<int> cv(Kreal, 1);
@@ -1449,7 +1496,7 @@ struct Engine {
     static int features(fehip_solver *h, const double *dy, float *feats, double *coef_ap) {
         if (K != 1) return fail(FEHIP_EARG, "features run on single columns");
         Plan *P = nullptr;
-        CHK(get_plan(h, 1, sizeof(T), &P));
+        CHK((get_plan_for<T, Acc, 1>(h, &P)));
         CHK(ensure_ws(h));
         CHK(start(h, P, dy, 1, h->N, true));
         hipLaunchKernelGGL((k_col_rms<T>), dim3(1), dim3(kFinBlock), 0, h->stream, (const T *)P->r, h->N, h->ws_sig);
@@ -1479,7 +1526,7 @@ struct Engine {
     static int solve_residuals_from(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
                                     const double *const *coef0, fehip_mlp *net, int64_t *iters, uint8_t *converged) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, sizeof(T), &P));
+        CHK((get_plan_for<T, Acc, K>(h, &P)));
         CHK(ensure_ws(h));
         CHK(start(h, P, dY, Kreal, ldY, true));
         const unsigned gp = grid_for(h->p, kBlock);
@@ -1489,7 +1536,7 @@ struct Engine {
                 HIPCHK(hipStreamSynchronize(h->stream));
                 CHK(mlp_predict_device(net, h->ws_feats, h->p, h->ws_pred));
                 Plan *P1 = nullptr;
-                CHK(get_plan(h, 1, sizeof(T), &P1));
+                CHK((get_plan_for<T, Acc, 1>(h, &P1)));
                 hipLaunchKernelGGL((k_x0_from_pred<T>), dim3(gp), dim3(kBlock), 0, h->stream, (const T *)P1->hh,
                                    (const float *)h->ws_pred, (const double *)h->ws_sig, ws_params(h), h->p, (T *)P->x,
                                    K, c);
@@ -1517,7 +1564,7 @@ struct Engine {
     static int solve_coefficients(fehip_solver *h, double *dy, double tol, int64_t maxiter, double *dcoef,
                                   int64_t *iters, uint8_t *converged, const double *coef0 = nullptr) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, sizeof(T), &P));
+        CHK((get_plan_for<T, Acc, K>(h, &P)));
         const int64_t NK = h->N * K;
         hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, K, 1,
                            h->perm_s, h->sqrtw_int, h->N, (T *)P->u, (T *)nullptr);
@@ -1547,7 +1594,7 @@ struct Engine {
     // test hooks: one mul! on K = 1 (caller's order in and out)
     static int mul_A(fehip_solver *h, double *dy, const double *dx, double a, double b) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, sizeof(T), &P));
+        CHK((get_plan_for<T, Acc, K>(h, &P)));
         hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, 1, 1,
                            h->perm_s, (const double *)nullptr, h->N, (T *)P->r, (T *)nullptr);
         hipLaunchKernelGGL(k_to_T<T>, dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream, dx, (T *)P->x, h->p);
@@ -1562,7 +1609,7 @@ struct Engine {
 
     static int mul_At(fehip_solver *h, double *dx, const double *dy, double a, double b) {
         Plan *P = nullptr;
-        CHK(get_plan(h, K, sizeof(T), &P));
+        CHK((get_plan_for<T, Acc, K>(h, &P)));
         hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, dy, h->N, 1, 1,
                            h->perm_s, (const double *)nullptr, h->N, (T *)P->r, (T *)nullptr);
         hipLaunchKernelGGL(k_to_T<T>, dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream, (const double *)dx,
@@ -1630,8 +1677,8 @@ struct Mixed {
                                int64_t *iters, uint8_t *converged) {
         if (h->G == 1) return E64::solve_residuals(h, dY, Kreal, ldY, tol, maxiter, iters, converged);
         Plan *P32 = nullptr, *P64 = nullptr;
-        CHK(get_plan(h, K, 4, &P32));
-        CHK(get_plan(h, K, 8, &P64));
+        CHK((get_plan_for<float, F32x2, K>(h, &P32)));
+        CHK((get_plan_for<double, double, K>(h, &P64)));
         std::vector<int64_t> it1, it2;
         std::vector<int> cv;
         CHK(phase1(h, P32, P64, dY, Kreal, ldY, true, tol, maxiter - 1, it1));
@@ -1651,8 +1698,8 @@ struct Mixed {
     static int solve_coefficients(fehip_solver *h, double *dy, double tol, int64_t maxiter, double *dcoef,
                                   int64_t *iters, uint8_t *converged) {
         Plan *P32 = nullptr, *P64 = nullptr;
-        CHK(get_plan(h, K, 4, &P32));
-        CHK(get_plan(h, K, 8, &P64));
+        CHK((get_plan_for<float, F32x2, K>(h, &P32)));
+        CHK((get_plan_for<double, double, K>(h, &P64)));
         std::vector<int64_t> it1, it2;
         std::vector<int> cv;
         CHK(phase1(h, P32, P64, dy, 1, h->N, false, tol, maxiter, it1));

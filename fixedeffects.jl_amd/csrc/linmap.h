@@ -359,10 +359,11 @@ struct SegFE {
     const T *cache;           // cache in CSR order
     const uint64_t *heads;    // segment head bits, 1 beyond N
     const uint32_t *wprefix;  // heads before each 64-position word
-    const int32_t *seg_level; // ordinal of non-empty segment -> level
+    const int32_t *seg_level; // ordinal of non-empty segment -> level; nullptr = identity (no empty level)
     int64_t coef_off;
     int64_t tp;               // positions per tile (a multiple of 256)
     int32_t tile0, ntiles;
+    int32_t nsegs, pad;       // non-empty segments
 };
 
 template <typename T>
@@ -456,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
     const int on = p.ops->on[c];
     const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
     const bool keep = p.ops->b[c] != 0.0;
-    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
+    const int gw = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
     const int nw = gridDim.x * kWavesPerBlock;
     double sq = 0.0;
     for (int t = p.first_tile + gw; t < p.total_tiles; t += nw) {
@@ -467,27 +468,62 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
         const int64_t end = start + fe.tp < p.N ? start + fe.tp : p.N;
         T *v = p.v + fe.coef_off * K;
         SegCursor<Acc> cur{Acc(0.f), true, false};
-        for (int64_t base0 = start; base0 < end; base0 += U * P) {
-            Acc val[U];
-            uint64_t hw[U], hn[U];
-            uint32_t wp[U];
-            // unconditional loads (positions past the tile read its first
-            // position): every load of the step is in flight together
+        // per lane, one loop step ahead: the observation each position reads
+        // and the head-bit words, so that a step's u gathers, its cache loads
+        // and the v of the levels its positions belong to go out together
+        int32_t in[U];
+        uint64_t hwn[U], hnn[U];
+        uint32_t wpn[U];
+        auto meta = [&](int64_t b0) {
 #pragma unroll
             for (int q = 0; q < U; q++) {
-                const int64_t base = base0 + q * P;
+                const int64_t base = b0 + q * P;
                 const int64_t pos = base + psub;
-                hw[q] = hn[q] = 0;
-                wp[q] = 0;
-                if (base < end) {  // wave-uniform
-                    hw[q] = fe.heads[base >> 6];
-                    hn[q] = fe.heads[(base >> 6) + 1];
-                    wp[q] = fe.wprefix[base >> 6];
-                }
                 const int64_t pc = pos < end ? pos : start;
-                const int64_t i = fe.perm ? (int64_t)__builtin_nontemporal_load(fe.perm + pc) : pc;
-                const T cv = __builtin_nontemporal_load(fe.cache + pc);
-                const Acc pr = mul_acc<Acc, T>(cv, p.u[i * K + c]);
+                in[q] = fe.perm ? __builtin_nontemporal_load(fe.perm + pc) : (int32_t)pc;
+                hwn[q] = hnn[q] = 0;
+                wpn[q] = 0;
+                if (base < end) {  // wave-uniform
+                    hwn[q] = fe.heads[base >> 6];
+                    hnn[q] = fe.heads[(base >> 6) + 1];
+                    wpn[q] = fe.wprefix[base >> 6];
+                }
+            }
+        };
+        meta(start);
+        for (int64_t base0 = start; base0 < end; base0 += U * P) {
+            int32_t iq[U];
+            uint64_t hw[U], hn[U];
+            int32_t lev[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                iq[q] = in[q];
+                hw[q] = hwn[q];
+                hn[q] = hnn[q];
+                // ordinal of the segment holding this lane's position,
+                // clamped for lanes past the tile's end
+                const int64_t pos = base0 + q * P + psub;
+                uint32_t ord = wpn[q] + (uint32_t)__popcll(hw[q] & lanes_le((int)(pos & 63))) - 1u;
+                ord = ord < (uint32_t)fe.nsegs ? ord : (uint32_t)fe.nsegs - 1u;
+                lev[q] = fe.seg_level ? fe.seg_level[ord] : (int32_t)ord;
+            }
+            // unconditional loads (positions past the tile read its first
+            // position): every load of the step is in flight together
+            T cv[U], uv[U], ov[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t pos = base0 + q * P + psub;
+                const int64_t pc = pos < end ? pos : start;
+                cv[q] = __builtin_nontemporal_load(fe.cache + pc);
+                uv[q] = p.u[(int64_t)iq[q] * K + c];
+                ov[q] = v[(int64_t)lev[q] * K + c];
+            }
+            meta(base0 + U * P);
+            Acc val[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t pos = base0 + q * P + psub;
+                const Acc pr = mul_acc<Acc, T>(cv[q], uv[q]);
                 val[q] = (pos < end && on) ? pr : Acc(0.f);
             }
             // every em[q] is defined before the scan, which stops at the
@@ -501,25 +537,11 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
                 if (base >= end) break;  // wave-uniform
                 em[q] = seg_step<K>(hw[q], hn[q], base, end, psub, c, on, val[q], cur, &p.tileA[(int64_t)t * K + c]);
             }
-            // the segments that ended in these steps: levels, then v, then stores
-            int32_t lev[U];
-#pragma unroll
-            for (int q = 0; q < U; q++) {
-                const int64_t pos = base0 + q * P + psub;
-                const uint32_t ord = wp[q] + (uint32_t)__popcll(hw[q] & lanes_le((int)(pos & 63))) - 1u;
-                lev[q] = fe.seg_level[em[q] ? ord : 0u];
-            }
-            T old[U];
-#pragma unroll
-            for (int q = 0; q < U; q++) {
-                const T ov = v[(int64_t)lev[q] * K + c];
-                old[q] = keep ? ov : (T)0;
-            }
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 if (em[q]) {
                     T out = a * acc_to_T<T, Acc>(val[q]);
-                    if (keep) out = b * old[q] + out;
+                    if (keep) out = b * ov[q] + out;
                     v[(int64_t)lev[q] * K + c] = out;
                     sq += (double)out * (double)out;
                 }
@@ -687,6 +709,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int64_t start = (int64_t)t * p.R;
         const int64_t end = start + p.R < p.N ? start + p.R : p.N;
         SegCursor<Acc> cur{Acc(0.f), true, false};
+        // GT > 0: the refs of the next loop step are loaded one step ahead,
+        // so a step's coefficient gathers go out with its other loads
+        int32_t rn[GT > 0 ? GT : 1][U];
+        if constexpr (GT > 0) {
+#pragma unroll
+            for (int g = 0; g < GT; g++)
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    const int64_t i = start + q * P + psub;
+                    rn[g][q] = p.refs[g][i < end ? i : start];
+                }
+        }
         for (int64_t base0 = start; base0 < end; base0 += U * P) {
             // loads are unconditional (positions past the tile read its
             // first position), so every load of the step is in flight at
@@ -715,20 +749,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
             if constexpr (GT > 0) {
                 int32_t r[GT][U];
-                T cv[GT][U];
-#pragma unroll
-                for (int g = 0; g < GT; g++)
-#pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        r[g][q] = p.refs[g][ic[q]];
-                        cv[g][q] = p.cache[g][ic[q]];
-                    }
+                T cv[GT][U], gv[GT][U];
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
                     const T *cf = p.coef + p.coef_off[g] * K + c;
 #pragma unroll
-                    for (int q = 0; q < U; q++) sum[q] += cv[g][q] * cf[(int64_t)r[g][q] * K];
+                    for (int q = 0; q < U; q++) {
+                        r[g][q] = rn[g][q];
+                        gv[g][q] = cf[(int64_t)r[g][q] * K];
+                        cv[g][q] = p.cache[g][ic[q]];
+                    }
                 }
+#pragma unroll
+                for (int g = 0; g < GT; g++)
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        const int64_t i = base0 + (U + q) * P + psub;
+                        rn[g][q] = p.refs[g][i < end ? i : start];
+                    }
+#pragma unroll
+                for (int g = 0; g < GT; g++)
+#pragma unroll
+                    for (int q = 0; q < U; q++) sum[q] += cv[g][q] * gv[g][q];
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
                     if (g == p.s) {
