@@ -40,6 +40,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by a plain device copy of known size), by (workload, precision):
+# profiles/worker_firm_1e7_fp64_k1.md
+PMC_TRAFFIC = {
+    ("worker_firm_1e7", "fp64"): {"k_fused": 579.0e6, "k_dtx_csr": 352.6e6, "k_update": 70.5e6,
+                                  "k_dtx_fixup": 27.1e6},
+}
+
+
 def kernel_bytes(N, G, n, kinds, K, s):
     """Algorithmic HBM bytes per launch of each kernel of one LSMR iteration
     (each array element counted once; gathers counted as their distinct
@@ -214,7 +223,8 @@ def main():
         "kernel_ms": kt,
         "hbm_gbs_pair": pair_gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": PMC_TRAFFIC.get((name, args.precision), {}).get(dom),
                      "algorithmic_bytes": int(kb[dom])},
         "cpu_baseline": cpu,
     }

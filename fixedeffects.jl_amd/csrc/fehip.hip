@@ -83,6 +83,8 @@ struct FeInfo {
     void *cache_int[2] = {nullptr, nullptr};  // N, internal order
     void *cache_csr[2] = {nullptr, nullptr};  // N, CSR order (== cache_int for KIND_SORTED)
     std::vector<int32_t> h_ioff;   // host copy of segment offsets (tile schedule)
+    std::vector<int32_t> big_levels;  // levels with > kScaleBig observations
+    int32_t *d_big = nullptr;
 };
 
 struct Plan;  // per-K buffers and schedule
@@ -257,6 +259,9 @@ static int csr_products(fehip_solver *h, const int32_t *skeys, int64_t N, int64_
 // One thread per level, serial over the level's observations in ascending
 // caller order -- the reference's summation order (:81-83) -- in double,
 // rounded into T after every add exactly as the Julia promotion does.
+// Levels with more than kScaleBig observations are left to k_scale_big.
+constexpr int32_t kScaleBig = 2048;
+
 template <typename T>
 __global__ void k_scale(const int32_t *__restrict__ off, const int32_t *__restrict__ perm, int64_t nlev,
                         const double *__restrict__ w, const double *__restrict__ z, T *__restrict__ scale_T,
@@ -268,6 +273,7 @@ __global__ void k_scale(const int32_t *__restrict__ off, const int32_t *__restri
     T s = (T)0;
     const int32_t q1 = off[k + 1];
     int32_t q = off[k];
+    if (q1 - q > kScaleBig) return;
     for (; q + B <= q1; q += B) {
         double t[B];
 #pragma unroll
@@ -289,6 +295,55 @@ __global__ void k_scale(const int32_t *__restrict__ off, const int32_t *__restri
     const T r = s > (T)0 ? (T)1 / (T)sqrt((double)s) : (T)0;
     scale_T[k] = r;
     scale_d[k] = (double)r;
+}
+
+// scale! of one large level per 64-lane workgroup, in the same order and
+// rounding as k_scale: the wave loads 1024 terms (z_i^2 w_i) at a time into
+// LDS and lane 0 adds them in observation order; the next 1024 terms are in
+// flight while lane 0 adds.
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_scale_big(const int32_t *__restrict__ off, const int32_t *__restrict__ perm,
+                                                     const int32_t *__restrict__ levels, const double *__restrict__ w,
+                                                     const double *__restrict__ z, T *__restrict__ scale_T,
+                                                     double *__restrict__ scale_d) {
+#pragma clang fp contract(off)
+    constexpr int B = 16, CH = kWave * B;
+    __shared__ double buf[CH];
+    const int lane = threadIdx.x;
+    const int32_t k = levels[blockIdx.x];
+    const int32_t q0 = off[k], q1 = off[k + 1];
+    double t[B];
+    auto load = [&](int32_t base) {
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            const int32_t q = base + j * kWave + lane;
+            t[j] = 0.0;
+            if (q < q1) {
+                const int32_t i = perm ? perm[q] : q;
+                const double zi = z ? z[i] : 1.0;
+                const double wi = w ? w[i] : 1.0;
+                t[j] = (zi * zi) * wi;
+            }
+        }
+    };
+    T s = (T)0;
+    load(q0);
+    for (int32_t base = q0; base < q1; base += CH) {
+#pragma unroll
+        for (int j = 0; j < B; j++) buf[j * kWave + lane] = t[j];
+        __syncthreads();
+        if (base + CH < q1) load(base + CH);
+        if (lane == 0) {
+            const int32_t n = q1 - base < CH ? q1 - base : CH;
+            for (int32_t e = 0; e < n; e++) s = (T)((double)s + buf[e]);
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        const T r = s > (T)0 ? (T)1 / (T)sqrt((double)s) : (T)0;
+        scale_T[k] = r;
+        scale_d[k] = (double)r;
+    }
 }
 
 // cache[j] = z_i * sqrt(w_i) * scale[ref]  with i = perm_s[j]  (:93)
@@ -328,6 +383,15 @@ static int compute_scale_cache(fehip_solver *h) {
         FeInfo &f = h->fe[g];
         hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(f.n, 64)), dim3(64), 0, h->stream, f.ooff, f.operm, f.n,
                            h->w_orig, f.z_orig, (T *)f.scale_T[q], f.scale_d);
+        HIPCHK(hipGetLastError());
+        if (f.big_levels.empty()) continue;
+        if (!f.d_big) {
+            CHK(dalloc(h, &f.d_big, f.big_levels.size()));
+            HIPCHK(hipMemcpyAsync(f.d_big, f.big_levels.data(), sizeof(int32_t) * f.big_levels.size(),
+                                  hipMemcpyHostToDevice, h->stream));
+        }
+        hipLaunchKernelGGL(k_scale_big<T>, dim3((unsigned)f.big_levels.size()), dim3(kWave), 0, h->stream, f.ooff,
+                           f.operm, f.d_big, h->w_orig, f.z_orig, (T *)f.scale_T[q], f.scale_d);
         HIPCHK(hipGetLastError());
     }
     for (int g = 0; g < h->G; g++) {
@@ -557,6 +621,13 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
         }
     }
     HIPCHK(hipStreamSynchronize(h->stream));
+    for (int g = 0; g < G; g++) {
+        FeInfo &f = h->fe[g];
+        std::vector<int32_t> o(f.n + 1);
+        HIPCHK(hipMemcpy(o.data(), f.ooff, sizeof(int32_t) * (f.n + 1), hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < f.n; k++)
+            if (o[k + 1] - o[k] > kScaleBig) f.big_levels.push_back((int32_t)k);
+    }
     CHK(do_update_weights(h, weights));
     cleanup.armed = false;
     *out = hp.release();

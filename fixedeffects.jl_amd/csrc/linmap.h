@@ -77,18 +77,72 @@ __device__ __forceinline__ T acc_to_T(Acc a) {
     return (T)a;
 }
 
+// Cross-lane moves through DPP (no LDS round trip): CTRL is the DPP control
+// (row_shr:n = 0x110+n, row_bcast:15 = 0x142, row_bcast:31 = 0x143), RM the
+// row mask; lanes without a source read 0.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_i32(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, RM, 0xF, true);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_move(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp_i32<CTRL, RM>((int)(unsigned)u);
+    const unsigned hi = (unsigned)dpp_i32<CTRL, RM>((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __builtin_bit_cast(float, dpp_i32<CTRL, RM>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ F32x2 dpp_move(F32x2 v) {
+    return F32x2(dpp_move<CTRL, RM>(v.hi), dpp_move<CTRL, RM>(v.lo));
+}
+// value of lane l, uniform across the wave
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ F32x2 lane_value(F32x2 v, int l) { return F32x2(lane_value(v.hi, l), lane_value(v.lo, l)); }
+
 // Segmented inclusive scan over the positions of one wave step.  rs is the
 // psub of the latest segment head at or before this lane (0 if none); lanes
-// only add values of their own run.
+// only add values of their own run.  K = 1: scan within rows of 16 lanes
+// (DPP row shifts), then the rows' totals are carried across with
+// row_bcast:15 and row_bcast:31; a lane adds a moved value only if no head
+// lies between its source and itself.  K > 1: shuffles by multiples of K.
 template <int K, typename A>
 __device__ __forceinline__ A seg_scan(A v, int psub, int rs) {
-    constexpr int P = kWave / K;
+    if constexpr (K == 1) {
+        const int r16 = psub & 15;
+        A up = dpp_move<0x111, 0xF>(v);
+        if (r16 >= 1 && psub - 1 >= rs) v += up;
+        up = dpp_move<0x112, 0xF>(v);
+        if (r16 >= 2 && psub - 2 >= rs) v += up;
+        up = dpp_move<0x114, 0xF>(v);
+        if (r16 >= 4 && psub - 4 >= rs) v += up;
+        up = dpp_move<0x118, 0xF>(v);
+        if (r16 >= 8 && psub - 8 >= rs) v += up;
+        up = dpp_move<0x142, 0xA>(v);  // rows 1 and 3 read lanes 15 and 47
+        if ((psub & 16) && rs < (psub & ~15)) v += up;
+        up = dpp_move<0x143, 0xC>(v);  // rows 2 and 3 read lane 31
+        if (psub >= 32 && rs < 32) v += up;
+        return v;
+    } else {
+        constexpr int P = kWave / K;
 #pragma unroll
-    for (int d = 1; d < P; d <<= 1) {
-        A up = __shfl_up(v, (unsigned)(d * K), kWave);
-        if (psub - d >= rs) v += up;
+        for (int d = 1; d < P; d <<= 1) {
+            A up = __shfl_up(v, (unsigned)(d * K), kWave);
+            if (psub - d >= rs) v += up;
+        }
+        return v;
     }
-    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -414,7 +468,9 @@ __device__ __forceinline__ bool seg_step(uint64_t w, uint64_t wnext, int64_t bas
         if (mycont) *tileA_tc = acc_to_double(x);
         else em = true;
     }
-    const Acc last = __shfl(x, (P - 1) * K + c, kWave);
+    Acc last;
+    if constexpr (K == 1) last = lane_value(x, kWave - 1);
+    else last = __shfl(x, (P - 1) * K + c, kWave);
     cur.open = (nb == 0);
     cur.carry = cur.open ? last : Acc(0.f);
     cur.cont = cur.cont && (hb == 0);
