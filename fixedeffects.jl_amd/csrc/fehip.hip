@@ -758,7 +758,8 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             P->slot0.push_back(P->nslot);
             P->nslot += (int32_t)h->fe[g].n;
         }
-    P->fused_dense = P->nslot > 0 && (size_t)P->nslot * kWave * accsz * kWavesPerBlock <= 65536;
+    P->fused_dense = P->nslot > 0 && (size_t)P->nslot * kWave * accsz * kWavesPerBlock <= 65536 &&
+                     env_int("FEHIP_FUSE_DENSE", 1, 0, 1) != 0;
     P->dense_lds_wave = (int32_t)(accsz * kWave * maxlev);
     int64_t chunk;
     if (P->fused_dense) {
@@ -846,7 +847,7 @@ static size_t fused_lds_bytes(const fehip_solver *h, size_t ts) {
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) nslot += h->fe[g].n;
     const size_t b = (size_t)nslot * kWave * accsz * kWavesPerBlock;
-    return b <= 65536 ? b : 0;
+    return (b <= 65536 && env_int("FEHIP_FUSE_DENSE", 1, 0, 1) != 0) ? b : 0;
 }
 
 // resident waves of a kernel on this device (all CUs, kBlock threads)

@@ -737,20 +737,23 @@ struct FusedParams {
 #ifndef FUSED_U64
 #define FUSED_U64 2
 #endif
-// GT > 0: the FE count is the compile-time GT and every load of a wave step
-// (all FEs' refs and caches, u, the head words) is issued before the first
-// coefficient gather, so a step waits on memory twice.  GT = 0: any FE count,
-// one FE at a time.
+// GT > 0 (the FE count, compile-time): a three-stage software pipeline per
+// wave.  While loop step j is computed, the loads of step j+1 (u, caches,
+// head words and the coefficient gathers, whose refs arrived one step
+// earlier) and the refs of step j+2 are in flight.  GT = 0: any FE count,
+// one FE at a time, no pipelining.
 template <typename T, typename Acc, int K, int GT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
     extern __shared__ __align__(16) unsigned char fused_smem[];
     constexpr int P = kWave / K;
-    // wave steps per loop step: all their loads are live at once, so FP64
-    // takes fewer to stay within 128 VGPRs (4 waves per SIMD) without spills
-    constexpr int U = sizeof(T) == 8 ? FUSED_U64 : 4;
+    // wave steps per loop step; FP64 and batched columns take fewer to stay
+    // within 128 VGPRs (4 waves per SIMD) without spills
+    constexpr int U = (sizeof(T) == 8 || K > 1) ? FUSED_U64 : 4;
+    constexpr int UP = U * P;
     // head words one loop step touches: steps q use word (q*P)/64 and the next
-    constexpr int NW = (U * P) / kWave + 1 > 2 ? (U * P) / kWave + 1 : 2;
+    constexpr int NW = UP / kWave + 1 > 2 ? UP / kWave + 1 : 2;
+    constexpr int GA = GT > 0 ? GT : 1;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
     const int psub = lane / K, c = lane % K;
@@ -765,120 +768,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         const int64_t start = (int64_t)t * p.R;
         const int64_t end = start + p.R < p.N ? start + p.R : p.N;
         SegCursor<Acc> cur{Acc(0.f), true, false};
-        // GT > 0: the refs of the next loop step are loaded one step ahead,
-        // so a step's coefficient gathers go out with its other loads
-        int32_t rn[GT > 0 ? GT : 1][U];
-        if constexpr (GT > 0) {
-#pragma unroll
-            for (int g = 0; g < GT; g++)
-#pragma unroll
-                for (int q = 0; q < U; q++) {
-                    const int64_t i = start + q * P + psub;
-                    rn[g][q] = p.refs[g][i < end ? i : start];
-                }
-        }
-        for (int64_t base0 = start; base0 < end; base0 += U * P) {
-            // loads are unconditional (positions past the tile read its
-            // first position), so every load of the step is in flight at
-            // once and no wait sits inside a branch; stores are predicated
-            int64_t idx[U], ic[U];
-            bool ok[U];
+        // one loop step, ready to compute: D*v sums, old u, FE s and first
+        // dense FE cache/ref, head words
+        struct Step {
             T sum[U], uo[U], cs[U], cd[U];
             int32_t rs[U], rd[U];
-            uint64_t hwv[NW];
-            // base0 is a multiple of U*P and of 64 when U*P >= 64; the words
-            // up to (end-1)/64 + 1 exist (the head bits run one word past N)
+            uint64_t hw[NW];
+        };
+        // position of lane, step q, clamped into the tile (loads are
+        // unconditional, so no load waits inside a branch)
+        auto clampi = [&](int64_t b0, int q) -> int64_t {
+            const int64_t i = b0 + q * P + psub;
+            return i < end ? i : start;
+        };
+        // base0 is a multiple of U*P and of 64 when U*P >= 64; the words up
+        // to (end-1)/64 + 1 exist (the head bits run one word past N)
+        auto load_words = [&](int64_t b0, uint64_t (&hw)[NW]) {
 #pragma unroll
             for (int j = 0; j < NW; j++) {
-                const int64_t wi = (base0 >> 6) + j;
-                hwv[j] = wi <= ((end - 1) >> 6) + 1 ? p.heads[wi] : 0ull;
+                const int64_t wi = (b0 >> 6) + j;
+                hw[j] = (b0 < end && wi <= ((end - 1) >> 6) + 1) ? p.heads[wi] : 0ull;
             }
-#pragma unroll
-            for (int q = 0; q < U; q++) {
-                idx[q] = base0 + q * P + psub;
-                ok[q] = on && idx[q] < end;
-                ic[q] = idx[q] < end ? idx[q] : start;
-                sum[q] = (T)0;
-                uo[q] = p.target[ic[q] * K + c];
-                cs[q] = cd[q] = (T)0;
-                rs[q] = rd[q] = 0;
-            }
-            if constexpr (GT > 0) {
-                int32_t r[GT][U];
-                T cv[GT][U], gv[GT][U];
-#pragma unroll
-                for (int g = 0; g < GT; g++) {
-                    const T *cf = p.coef + p.coef_off[g] * K + c;
-#pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        r[g][q] = rn[g][q];
-                        gv[g][q] = cf[(int64_t)r[g][q] * K];
-                        cv[g][q] = p.cache[g][ic[q]];
-                    }
-                }
-#pragma unroll
-                for (int g = 0; g < GT; g++)
-#pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        const int64_t i = base0 + (U + q) * P + psub;
-                        rn[g][q] = p.refs[g][i < end ? i : start];
-                    }
-#pragma unroll
-                for (int g = 0; g < GT; g++)
-#pragma unroll
-                    for (int q = 0; q < U; q++) sum[q] += cv[g][q] * gv[g][q];
-#pragma unroll
-                for (int g = 0; g < GT; g++) {
-                    if (g == p.s) {
-#pragma unroll
-                        for (int q = 0; q < U; q++) {
-                            cs[q] = cv[g][q];
-                            rs[q] = r[g][q];
-                        }
-                    }
-                    if (p.ndense && g == p.dense_g[0]) {
-#pragma unroll
-                        for (int q = 0; q < U; q++) {
-                            cd[q] = cv[g][q];
-                            rd[q] = r[g][q];
-                        }
-                    }
-                }
-            } else {
-                for (int g = 0; g < p.G; g++) {
-                    int32_t r[U];
-                    T cv[U];
-#pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        r[q] = p.refs[g][ic[q]];
-                        cv[q] = p.cache[g][ic[q]];
-                    }
-                    const T *cf = p.coef + p.coef_off[g] * K + c;
-#pragma unroll
-                    for (int q = 0; q < U; q++) sum[q] += cv[q] * cf[(int64_t)r[q] * K];
-                    if (g == p.s) {
-#pragma unroll
-                        for (int q = 0; q < U; q++) {
-                            cs[q] = cv[q];
-                            rs[q] = r[q];
-                        }
-                    }
-                    if (p.ndense && g == p.dense_g[0]) {
-#pragma unroll
-                        for (int q = 0; q < U; q++) {
-                            cd[q] = cv[q];
-                            rd[q] = r[q];
-                        }
-                    }
-                }
-            }
+        };
+        auto compute = [&](const Step &st, int64_t base0) {
             T un[U];
 #pragma unroll
             for (int q = 0; q < U; q++) {
-                T out = a * sum[q];
-                out = keep ? b * uo[q] + out : out;
-                un[q] = ok[q] ? out : (T)0;
-                if (ok[q]) p.target[idx[q] * K + c] = out;
+                const int64_t idx = base0 + q * P + psub;
+                const bool ok = on && idx < end;
+                T out = a * st.sum[q];
+                out = keep ? b * st.uo[q] + out : out;
+                un[q] = ok ? out : (T)0;
+                if (ok) p.target[idx * K + c] = out;
                 sq += (double)un[q] * (double)un[q];
             }
             // FE s: segmented sums of cache_s * u; a segment ending at a
@@ -887,10 +808,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             for (int q = 0; q < U; q++) {
                 const int64_t base = base0 + q * P;
                 if (base >= end) break;  // wave-uniform
-                Acc x = mul_acc<Acc, T>(cs[q], un[q]);
+                Acc x = mul_acc<Acc, T>(st.cs[q], un[q]);
                 const int jw = (q * P) / kWave;  // constant after unrolling
-                if (seg_step<K>(hwv[jw], hwv[jw + 1], base, end, psub, c, on, x, cur, &p.tileA[(int64_t)t * K + c]))
-                    p.wsum[(int64_t)rs[q] * K + c] = acc_to_double(x);
+                if (seg_step<K>(st.hw[jw], st.hw[jw + 1], base, end, psub, c, on, x, cur,
+                                &p.tileA[(int64_t)t * K + c]))
+                    p.wsum[(int64_t)st.rs[q] * K + c] = acc_to_double(x);
             }
             // dense FEs: lane-private LDS accumulators
             if (p.ndense) {
@@ -899,8 +821,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 Acc x[U];
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    sl[q] = s0 + rd[q];
-                    x[q] = mul_acc<Acc, T>(cd[q], un[q]);
+                    sl[q] = s0 + st.rd[q];
+                    x[q] = mul_acc<Acc, T>(st.cd[q], un[q]);
                 }
                 lds_accumulate<U, Acc>(acc, sl, x, lane);
                 for (int d = 1; d < p.ndense; d++) {
@@ -909,11 +831,133 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     const int sd = p.dense_slot0[d];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        sl[q] = sd + rf[ic[q]];
-                        x[q] = mul_acc<Acc, T>(cf[ic[q]], un[q]);
+                        const int64_t i = clampi(base0, q);
+                        sl[q] = sd + rf[i];
+                        x[q] = mul_acc<Acc, T>(cf[i], un[q]);
                     }
                     lds_accumulate<U, Acc>(acc, sl, x, lane);
                 }
+            }
+        };
+        if constexpr (GT > 0) {
+            // stage A: refs of a step
+            auto issue_refs = [&](int64_t b0, int32_t (&rr)[GA][U]) {
+#pragma unroll
+                for (int g = 0; g < GT; g++)
+#pragma unroll
+                    for (int q = 0; q < U; q++) rr[g][q] = p.refs[g][clampi(b0, q)];
+            };
+            // stage B: every other load of a step (gathers use its refs)
+            struct InFlight {
+                T uo[U], cv[GA][U], gv[GA][U];
+                uint64_t hw[NW];
+            };
+            auto issue_data = [&](int64_t b0, const int32_t (&rr)[GA][U], InFlight &d) {
+                load_words(b0, d.hw);
+#pragma unroll
+                for (int q = 0; q < U; q++) d.uo[q] = p.target[clampi(b0, q) * K + c];
+#pragma unroll
+                for (int g = 0; g < GT; g++) {
+                    const T *cf = p.coef + p.coef_off[g] * K + c;
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        d.gv[g][q] = cf[(int64_t)rr[g][q] * K];
+                        d.cv[g][q] = p.cache[g][clampi(b0, q)];
+                    }
+                }
+            };
+            // the step's loads have arrived: form the D*v sums (FEs in order)
+            auto settle = [&](const InFlight &d, const int32_t (&rr)[GA][U], Step &st) {
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    st.sum[q] = (T)0;
+                    st.uo[q] = d.uo[q];
+                    st.cs[q] = st.cd[q] = (T)0;
+                    st.rs[q] = st.rd[q] = 0;
+                }
+#pragma unroll
+                for (int g = 0; g < GT; g++)
+#pragma unroll
+                    for (int q = 0; q < U; q++) st.sum[q] += d.cv[g][q] * d.gv[g][q];
+#pragma unroll
+                for (int g = 0; g < GT; g++) {
+                    if (g == p.s) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) {
+                            st.cs[q] = d.cv[g][q];
+                            st.rs[q] = rr[g][q];
+                        }
+                    }
+                    if (p.ndense && g == p.dense_g[0]) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) {
+                            st.cd[q] = d.cv[g][q];
+                            st.rd[q] = rr[g][q];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NW; j++) st.hw[j] = d.hw[j];
+            };
+            int32_t r1[GA][U], r2[GA][U];
+            InFlight d;
+            Step st;
+            issue_refs(start, r1);
+            issue_data(start, r1, d);
+            issue_refs(start + UP, r2);
+            settle(d, r1, st);
+#pragma unroll
+            for (int g = 0; g < GT; g++)
+#pragma unroll
+                for (int q = 0; q < U; q++) r1[g][q] = r2[g][q];
+            for (int64_t base0 = start; base0 < end; base0 += UP) {
+                issue_refs(base0 + 2 * UP, r2);
+                issue_data(base0 + UP, r1, d);
+                compute(st, base0);
+                settle(d, r1, st);
+#pragma unroll
+                for (int g = 0; g < GT; g++)
+#pragma unroll
+                    for (int q = 0; q < U; q++) r1[g][q] = r2[g][q];
+            }
+        } else {
+            for (int64_t base0 = start; base0 < end; base0 += UP) {
+                Step st;
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    st.sum[q] = (T)0;
+                    st.uo[q] = p.target[clampi(base0, q) * K + c];
+                    st.cs[q] = st.cd[q] = (T)0;
+                    st.rs[q] = st.rd[q] = 0;
+                }
+                load_words(base0, st.hw);
+                for (int g = 0; g < p.G; g++) {
+                    int32_t r[U];
+                    T cv[U];
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        r[q] = p.refs[g][clampi(base0, q)];
+                        cv[q] = p.cache[g][clampi(base0, q)];
+                    }
+                    const T *cf = p.coef + p.coef_off[g] * K + c;
+#pragma unroll
+                    for (int q = 0; q < U; q++) st.sum[q] += cv[q] * cf[(int64_t)r[q] * K];
+                    if (g == p.s) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) {
+                            st.cs[q] = cv[q];
+                            st.rs[q] = r[q];
+                        }
+                    }
+                    if (p.ndense && g == p.dense_g[0]) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) {
+                            st.cd[q] = cv[q];
+                            st.rd[q] = r[q];
+                        }
+                    }
+                }
+                compute(st, base0);
             }
         }
         if (cur.open && on && psub == 0) {
