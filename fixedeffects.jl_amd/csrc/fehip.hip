@@ -120,6 +120,8 @@ struct fehip_solver {
     float *ws_pred = nullptr;    // p
     double *ws_sig = nullptr;    // 1
     double *ws_coef = nullptr;   // p (host-provided initial coefficients)
+    double *ws_x0 = nullptr;     // p x ws_x0_cols (the network's starts)
+    int ws_x0_cols = 0;
     int t_sec = -1;              // FE giving the secondary order key (-1: none)
 };
 
@@ -1075,14 +1077,21 @@ __global__ void k_x0_from_coef(const double *__restrict__ b0, WsParams prm, int6
 // alternating-projection start, in scaled coordinates)
 template <typename T>
 __global__ void k_x0_from_pred(const T *__restrict__ xap, const float *__restrict__ pred, const double *__restrict__ sig,
-                               WsParams prm, int64_t p, T *__restrict__ x, int K, int c) {
+                               WsParams prm, int64_t p, double *__restrict__ x0) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= p) return;
     int g = 0;
     while (g + 1 < prm.G && e >= prm.fe[g + 1].off_p) g++;
     const double sc = prm.fe[g].scale[e - prm.fe[g].off_p];
     const double sigma = sig[0] > 0 ? sig[0] : 1.0;
-    x[e * K + c] = sc > 0 ? (T)((double)xap[e] + (double)pred[e] * sigma / sc) : (T)0;
+    x0[e] = sc > 0 ? (double)xap[e] + (double)pred[e] * sigma / sc : 0.0;
+}
+
+// column c of the solve's x (p x K, level-major) <- x0
+template <typename T>
+__global__ void k_put_col(const double *__restrict__ x0, int64_t p, T *__restrict__ x, int K, int c) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < p) x[e * K + c] = (T)x0[e];
 }
 
 // sig[0] = sqrt(sum b^2 / n) over one column (single workgroup, fixed order)
@@ -1600,18 +1609,33 @@ struct Engine {
         Plan *P = nullptr;
         CHK((get_plan_for<T, Acc, K>(h, &P)));
         CHK(ensure_ws(h));
-        CHK(start(h, P, dY, Kreal, ldY, true));
         const unsigned gp = grid_for(h->p, kBlock);
-        for (int c = 0; c < Kreal; c++) {
-            if (net) {
+        // the network's starts first: the feature pass runs on the K = 1
+        // buffers, which are this solve's own when K = 1
+        if (net) {
+            if (h->ws_x0_cols < Kreal) {
+                dfree(h, h->ws_x0);
+                h->ws_x0 = nullptr;
+                CHK(dalloc(h, &h->ws_x0, (size_t)h->p * Kreal));
+                h->ws_x0_cols = Kreal;
+            }
+            for (int c = 0; c < Kreal; c++) {
                 CHK((Engine<T, Acc, Tr, 1>::features(h, dY + (int64_t)c * ldY, h->ws_feats, nullptr)));
                 HIPCHK(hipStreamSynchronize(h->stream));
                 CHK(mlp_predict_device(net, h->ws_feats, h->p, h->ws_pred));
                 Plan *P1 = nullptr;
                 CHK((get_plan_for<T, Acc, 1>(h, &P1)));
                 hipLaunchKernelGGL((k_x0_from_pred<T>), dim3(gp), dim3(kBlock), 0, h->stream, (const T *)P1->hh,
-                                   (const float *)h->ws_pred, (const double *)h->ws_sig, ws_params(h), h->p, (T *)P->x,
-                                   K, c);
+                                   (const float *)h->ws_pred, (const double *)h->ws_sig, ws_params(h), h->p,
+                                   h->ws_x0 + (size_t)c * h->p);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        CHK(start(h, P, dY, Kreal, ldY, true));
+        for (int c = 0; c < Kreal; c++) {
+            if (net) {
+                hipLaunchKernelGGL((k_put_col<T>), dim3(gp), dim3(kBlock), 0, h->stream,
+                                   (const double *)h->ws_x0 + (size_t)c * h->p, h->p, (T *)P->x, K, c);
             } else if (coef0 && coef0[c]) {
                 HIPCHK(hipMemcpyAsync(h->ws_coef, coef0[c], sizeof(double) * h->p, hipMemcpyHostToDevice, h->stream));
                 hipLaunchKernelGGL((k_x0_from_coef<T>), dim3(gp), dim3(kBlock), 0, h->stream,

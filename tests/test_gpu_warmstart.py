@@ -147,3 +147,23 @@ def test_start_from_coefficients(gpu):
     _say(f"  iterations: reference start {it_ref}, alternating projection {it_ap}, exact {it_ex}")
     assert it_ex[0] <= 3
     assert _rel(r_ap, r_ref) <= 1e-8 and _rel(r_ex, r_ref) <= 1e-8
+
+
+@pytest.mark.parametrize("seed", [900001, 900002])
+def test_learned_start_residuals_agree(gpu, seed):
+    # the network's start changes only where LSMR begins: the residuals
+    # agree with the reference start's, for one column (whose feature pass
+    # shares the solve's buffers) and for several
+    _say(f"case: learned start, panel {seed}")
+    p = synth.uneven_panel(seed, 1e4, 6e4)
+    fes = [api.FixedEffect.from_refs(r, k, z) for r, k, z in zip(p.refs, p.n, p.interactions)]
+    net = warmstart.WarmStartNet(seed=11)
+    with api.FixedEffectSolverROCm(fes, p.weights, "fp64") as s:
+        ref = [np.ascontiguousarray(p.Y[:, c]).copy() for c in range(p.K)]
+        it_ref, cv_ref = warmstart.solve_residuals_from(s, ref, None, tol=1e-11)
+        lrn = [np.ascontiguousarray(p.Y[:, c]).copy() for c in range(p.K)]
+        it_l, cv_l = warmstart.solve_residuals_learned(s, lrn, net, tol=1e-11)
+    errs = [_rel(a, b) for a, b in zip(lrn, ref)]
+    _say(f"  K={p.K} iterations reference {it_ref} learned {it_l} rel diff {max(errs):.2e}")
+    assert all(cv_ref) and all(cv_l)
+    assert max(errs) <= 1e-7
