@@ -103,6 +103,11 @@ struct fehip_solver {
     int32_t *perm_s = nullptr;  // internal position -> caller's observation
     double *w_orig = nullptr;   // caller's order (nullptr: UnitWeights)
     double *sqrtw_int = nullptr;
+    // no FE has an interaction: cache_g = sqrt(w) * scale_g[ref], and the
+    // fused pass reads sqrt(w) once instead of every cache (k_fused FACT)
+    bool factored = false;
+    void *sqrtw_T[2] = {nullptr, nullptr};    // N, internal order, [0] double (= sqrtw_int), [1] float
+    void *scale_all[2] = {nullptr, nullptr};  // p, every FE's scale, [0] double, [1] float
     int32_t *d_flag = nullptr;  // range-check flag
     int32_t *h_flag = nullptr;  // pinned
     std::vector<void *> allocs;
@@ -148,6 +153,14 @@ static void dfree(fehip_solver *h, void *q) {
     auto it = std::find(h->allocs.begin(), h->allocs.end(), q);
     if (it != h->allocs.end()) h->allocs.erase(it);
     (void)hipFree(q);
+}
+
+// integer tuning knob from the environment, clamped to [lo, hi]
+static int64_t env_int(const char *name, int64_t dflt, int64_t lo, int64_t hi) {
+    const char *e = getenv(name);
+    if (!e || !*e) return dflt;
+    const long long v = atoll(e);
+    return v < lo ? lo : (v > hi ? hi : (int64_t)v);
 }
 
 static inline unsigned grid_for(int64_t n, int per_block) {
@@ -362,6 +375,12 @@ __global__ void k_cache(const int32_t *__restrict__ perm_s, int64_t N, const int
     cache[j] = (T)((zi * sqrt(wi)) * (double)scale[ref_int[j]]);
 }
 
+template <typename T>
+__global__ void k_copy_range(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) b[e] = a[e];
+}
+
 __global__ void k_round_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) b[e] = (float)a[e];
@@ -375,6 +394,24 @@ __global__ void k_sqrtw(const double *__restrict__ w, const int32_t *__restrict_
 
 template <typename T>
 constexpr int pidx() { return sizeof(T) == 8 ? 0 : 1; }
+
+// sqrtw_T[q] follows sqrtw_int: an alias for double, a rounded copy for float
+static int set_sqrtw_T(fehip_solver *h, int q) {
+    if (q == 0) {
+        h->sqrtw_T[0] = h->sqrtw_int;
+        return 0;
+    }
+    if (!h->sqrtw_int) {
+        dfree(h, h->sqrtw_T[1]);
+        h->sqrtw_T[1] = nullptr;
+        return 0;
+    }
+    if (!h->sqrtw_T[1]) CHK(dalloc_bytes(h, &h->sqrtw_T[1], sizeof(float) * h->N));
+    hipLaunchKernelGGL(k_round_f32, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream,
+                       (const double *)h->sqrtw_int, (float *)h->sqrtw_T[1], h->N);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
 
 template <typename T>
 static int compute_scale_cache(fehip_solver *h) {
@@ -411,6 +448,14 @@ static int compute_scale_cache(fehip_solver *h) {
                            h->N, h->sqrtw_int);
         HIPCHK(hipGetLastError());
     }
+    // every FE's scale in one p-vector, and sqrt(w) in T (the fused pass's
+    // factored form)
+    if (!h->scale_all[q]) CHK(dalloc_bytes(h, &h->scale_all[q], sizeof(T) * h->p));
+    for (int g = 0; g < h->G; g++)
+        hipLaunchKernelGGL(k_copy_range<T>, dim3(grid_for(h->n[g], kBlock)), dim3(kBlock), 0, h->stream,
+                           (const T *)h->fe[g].scale_T[q], (T *)h->scale_all[q] + h->off[g], h->n[g]);
+    CHK(set_sqrtw_T(h, q));
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -428,6 +473,7 @@ static int do_update_weights(fehip_solver *h, const double *w) {
         dfree(h, h->sqrtw_int);
         h->w_orig = nullptr;
         h->sqrtw_int = nullptr;
+        h->sqrtw_T[0] = nullptr;
     }
     if (h->precision == FEHIP_FP32) return compute_scale_cache<float>(h);
     CHK(compute_scale_cache<double>(h));
@@ -444,6 +490,10 @@ static int do_update_weights(fehip_solver *h, const double *w) {
                                    (const double *)f.cache_csr[0], (float *)f.cache_csr[1], h->N);
             HIPCHK(hipGetLastError());
         }
+        if (!h->scale_all[1]) CHK(dalloc_bytes(h, &h->scale_all[1], sizeof(float) * h->p));
+        hipLaunchKernelGGL(k_round_f32, dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream,
+                           (const double *)h->scale_all[0], (float *)h->scale_all[1], h->p);
+        CHK(set_sqrtw_T(h, 1));
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     return 0;
@@ -481,6 +531,9 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
     h->precision = precision;
     h->has64 = precision != FEHIP_FP32;
     h->has32 = precision != FEHIP_FP64;
+    h->factored = env_int("FEHIP_FACTORED", 1, 0, 1) != 0;
+    for (int g = 0; g < G; g++)
+        if (interaction && interaction[g]) h->factored = false;
     if (const char *e = std::getenv("FEHIP_MIXED_TOL")) h->mixed_tol = std::atof(e);
     h->n.assign(nlevels, nlevels + G);
     h->off.resize(G + 1);
@@ -644,12 +697,6 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
 // it) uses ~4096 tiles, the CSR FEs ~8192 (their gathers want more waves in
 // flight).  Both are multiples of 256 positions.  FEHIP_TILES_S and
 // FEHIP_TILES_C override the counts.
-static int64_t env_int(const char *name, int64_t dflt, int64_t lo, int64_t hi) {
-    const char *e = getenv(name);
-    if (!e || !*e) return dflt;
-    const long long v = atoll(e);
-    return v < lo ? lo : (v > hi ? hi : (int64_t)v);
-}
 static int64_t tile_size(int64_t N, int64_t ntiles) {
     const int64_t t = (N + ntiles - 1) / ntiles;
     return std::max<int64_t>(256, (t + 255) / 256 * 256);
@@ -676,6 +723,7 @@ struct Plan {
     unsigned fused_blocks = 0;
     size_t fused_lds = 0;
     double *part_fused = nullptr, *wsum = nullptr;
+    void *vt = nullptr;  // p x K, scale .* v (factored fused pass)
     uint8_t *sdirect = nullptr;
     void *state = nullptr;
     ColOps *ops_db = nullptr, *ops_dtx = nullptr, *ops_once = nullptr;
@@ -723,6 +771,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             if (en <= st) {  // level without observations: coef <- b*coef only
                 FixItem fi{};
                 fi.kind = 2;
+                fi.fe = g;
                 fi.level = (int32_t)k;
                 fi.coef_off = h->off[g];
                 items.push_back(fi);
@@ -735,6 +784,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             }
             FixItem fi{};
             fi.kind = 0;
+            fi.fe = g;
             fi.level = (int32_t)k;
             fi.coef_off = h->off[g];
             fi.first = t0 + tfirst;
@@ -789,6 +839,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
         for (int64_t k = 0; k < f.n; k++) {
             FixItem fi{};
             fi.kind = 1;
+            fi.fe = g;
             fi.level = (int32_t)k;
             fi.coef_off = h->off[g];
             fi.first = row0 + k * nblk;
@@ -835,6 +886,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     CHK(dalloc_bytes(h, &P->v, ts * h->p * K));
     CHK(dalloc_bytes(h, &P->hh, ts * h->p * K));
     CHK(dalloc_bytes(h, &P->hbar, ts * h->p * K));
+    if (h->factored) CHK(dalloc_bytes(h, &P->vt, ts * h->p * K));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->plans[key] = pl.release();
     *out = P;
@@ -873,9 +925,12 @@ static int get_plan_for(fehip_solver *h, Plan **out) {
         *out = it->second;
         return 0;
     }
-    const void *kf = h->G == 3   ? (const void *)k_fused<T, Acc, K, 3>
-                     : h->G == 2 ? (const void *)k_fused<T, Acc, K, 2>
-                                 : (const void *)k_fused<T, Acc, K, 0>;
+    const void *kf = h->factored ? (h->G == 3   ? (const void *)k_fused<T, Acc, K, 3, true>
+                                    : h->G == 2 ? (const void *)k_fused<T, Acc, K, 2, true>
+                                                : (const void *)k_fused<T, Acc, K, 0, true>)
+                                 : (h->G == 3   ? (const void *)k_fused<T, Acc, K, 3, false>
+                                    : h->G == 2 ? (const void *)k_fused<T, Acc, K, 2, false>
+                                                : (const void *)k_fused<T, Acc, K, 0, false>);
     const int64_t wf = resident_waves(kf, fused_lds_bytes(h, sizeof(T)));
     const int64_t wc = resident_waves((const void *)k_dtx_seg<T, Acc, K>, 0);
     return get_plan(h, K, sizeof(T), out, wf, wc);
@@ -884,7 +939,7 @@ static int get_plan_for(fehip_solver *h, Plan **out) {
 static void destroy_plan(fehip_solver *h, Plan *P) {
     if (P->graph) (void)hipGraphExecDestroy(P->graph);
     void *ptrs[] = {P->items, P->tileA, P->tileB, P->dpart, P->part_db, P->part_seg, P->part_fix, P->part_upd,
-                    P->part_fused, P->wsum, P->sdirect, P->state, P->ops_db, P->ops_dtx, P->ops_once, P->all_done, P->u, P->r, P->x,
+                    P->part_fused, P->wsum, P->sdirect, P->vt, P->state, P->ops_db, P->ops_dtx, P->ops_once, P->all_done, P->u, P->r, P->x,
                     P->v, P->hh, P->hbar, P->dY};
     for (void *q : ptrs) dfree(h, q);
     delete P;
@@ -1114,11 +1169,6 @@ __global__ __launch_bounds__(kFinBlock) void k_col_rms(const T *__restrict__ b, 
     if (threadIdx.x == 0) sig[0] = n > 0 ? sqrt(red[0] / (double)n) : 0.0;
 }
 
-template <typename T>
-__global__ void k_copy_range(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) b[e] = a[e];
-}
 
 // ---------------------------------------------------------------------------
 // engine: launches of one precision / column count
@@ -1242,10 +1292,18 @@ struct Engine {
         p.ops = ops;
         p.partials = P->part_fused;
         p.all_done = P->all_done;
+        p.sqrtw = (const T *)h->sqrtw_T[pidx<T>()];
+        p.vt = (const T *)P->vt;
         const dim3 gr(P->fused_blocks), bl(kBlock);
-        if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3>), gr, bl, P->fused_lds, h->stream, p);
-        else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2>), gr, bl, P->fused_lds, h->stream, p);
-        else hipLaunchKernelGGL((k_fused<T, Acc, K, 0>), gr, bl, P->fused_lds, h->stream, p);
+        if (h->factored) {
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true>), gr, bl, P->fused_lds, h->stream, p);
+            else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true>), gr, bl, P->fused_lds, h->stream, p);
+            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, true>), gr, bl, P->fused_lds, h->stream, p);
+        } else {
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, false>), gr, bl, P->fused_lds, h->stream, p);
+            else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2, false>), gr, bl, P->fused_lds, h->stream, p);
+            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, false>), gr, bl, P->fused_lds, h->stream, p);
+        }
     }
 
     static void dtx_dense(fehip_solver *h, Plan *P, const ColOps *ops, const T *y, const int32_t *early_exit) {
@@ -1292,6 +1350,12 @@ struct Engine {
             f.sdirect = P->sdirect;
             f.ns = h->n[h->s];
             f.s_off = h->off[h->s];
+            f.s = h->s;
+            if (h->factored) {
+                f.fscale[h->s] = (const T *)h->fe[h->s].scale_T[pidx<T>()];
+                if (P->fused_dense)
+                    for (int g : P->dense_fe) f.fscale[g] = (const T *)h->fe[g].scale_T[pidx<T>()];
+            }
         }
         hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
     }
@@ -1310,9 +1374,11 @@ struct Engine {
         CHK((get_plan_for<T, Acc, K>(h, &P)));
         const int64_t NK = h->N * K, pK = h->p * K;
         hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->u, NK, (T)1);
-        void *pv[] = {P->x, P->v, P->hh, P->hbar};
+        void *pv[] = {P->x, P->v, P->hh, P->hbar, P->vt};
         for (void *q : pv)
-            hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(pK, kBlock)), dim3(kBlock), 0, h->stream, (T *)q, pK, (T)0.5);
+            if (q)
+                hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(pK, kBlock)), dim3(kBlock), 0, h->stream, (T *)q, pK,
+                                   (T)0.5);
         hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, K, 0.0,
                            0.0, 0.0, (int64_t)1 << 40, P->all_done);
         hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.5, K, K);
@@ -1374,6 +1440,8 @@ struct Engine {
         p.partials = P->part_upd;
         p.all_done = P->all_done;
         p.state = P->state;
+        p.scale = (const T *)h->scale_all[pidx<T>()];
+        p.vt = (T *)P->vt;
         hipLaunchKernelGGL((k_update<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, p);
     }
 
@@ -1424,7 +1492,8 @@ struct Engine {
         dtx(h, P, MODE_INIT, P->ops_dtx, (const T *)P->u, (T *)P->v, nullptr);
         fin_dtx(h, P, MODE_INIT, nullptr);
         hipLaunchKernelGGL((k_init_vectors<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, h->p * K,
-                           (T *)P->v, (T *)P->hh, (T *)P->hbar, (const void *)P->state);
+                           (T *)P->v, (T *)P->hh, (T *)P->hbar, (const void *)P->state,
+                           (const T *)h->scale_all[pidx<T>()], (T *)P->vt);
         HIPCHK(hipGetLastError());
         CHK(ensure_graph(h, P));
         HIPCHK(hipMemcpyAsync(h->h_flag, P->all_done, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));

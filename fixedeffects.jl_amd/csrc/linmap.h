@@ -726,6 +726,12 @@ struct FusedParams {
     const int32_t *seg_level;
     const T *coef;     // v
     T *target;         // u
+    // FACT: cache_g = sqrt(w) * scale_g[ref] for every FE (no interactions),
+    // so D*v = sqrt(w) * sum_g vt_g[ref_g] with vt = scale .* v (kept by the
+    // vector update) and the raw D'u sums are of sqrt(w)*u (scaled in the
+    // fix-up); the caches are not read
+    const T *sqrtw;    // internal order; nullptr = unit weights
+    const T *vt;       // p x K
     double *wsum;      // p_s x K
     double *tileA, *tileB;
     double *dpart;
@@ -742,7 +748,7 @@ struct FusedParams {
 // head words and the coefficient gathers, whose refs arrived one step
 // earlier) and the refs of step j+2 are in flight.  GT = 0: any FE count,
 // one FE at a time, no pipelining.
-template <typename T, typename Acc, int K, int GT>
+template <typename T, typename Acc, int K, int GT, bool FACT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
     extern __shared__ __align__(16) unsigned char fused_smem[];
@@ -833,7 +839,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     for (int q = 0; q < U; q++) {
                         const int64_t i = clampi(base0, q);
                         sl[q] = sd + rf[i];
-                        x[q] = mul_acc<Acc, T>(cf[i], un[q]);
+                        x[q] = mul_acc<Acc, T>(FACT ? st.cs[q] : cf[i], un[q]);
                     }
                     lds_accumulate<U, Acc>(acc, sl, x, lane);
                 }
@@ -849,20 +855,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             };
             // stage B: every other load of a step (gathers use its refs)
             struct InFlight {
-                T uo[U], cv[GA][U], gv[GA][U];
+                T uo[U], cv[FACT ? 1 : GA][U], gv[GA][U], sw[U];
                 uint64_t hw[NW];
             };
             auto issue_data = [&](int64_t b0, const int32_t (&rr)[GA][U], InFlight &d) {
                 load_words(b0, d.hw);
 #pragma unroll
-                for (int q = 0; q < U; q++) d.uo[q] = p.target[clampi(b0, q) * K + c];
+                for (int q = 0; q < U; q++) {
+                    d.uo[q] = p.target[clampi(b0, q) * K + c];
+                    d.sw[q] = (T)1;
+                    if constexpr (FACT) {
+                        if (p.sqrtw) d.sw[q] = p.sqrtw[clampi(b0, q)];  // wave-uniform test
+                    }
+                }
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
-                    const T *cf = p.coef + p.coef_off[g] * K + c;
+                    const T *cf = (FACT ? p.vt : p.coef) + p.coef_off[g] * K + c;
 #pragma unroll
                     for (int q = 0; q < U; q++) {
                         d.gv[g][q] = cf[(int64_t)rr[g][q] * K];
-                        d.cv[g][q] = p.cache[g][clampi(b0, q)];
+                        if constexpr (!FACT) d.cv[g][q] = p.cache[g][clampi(b0, q)];
                     }
                 }
             };
@@ -875,23 +887,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     st.cs[q] = st.cd[q] = (T)0;
                     st.rs[q] = st.rd[q] = 0;
                 }
+                if constexpr (FACT) {
 #pragma unroll
-                for (int g = 0; g < GT; g++)
+                    for (int g = 0; g < GT; g++)
 #pragma unroll
-                    for (int q = 0; q < U; q++) st.sum[q] += d.cv[g][q] * d.gv[g][q];
+                        for (int q = 0; q < U; q++) st.sum[q] += d.gv[g][q];
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        st.sum[q] = d.sw[q] * st.sum[q];
+                        st.cs[q] = st.cd[q] = d.sw[q];
+                    }
+                } else {
+#pragma unroll
+                    for (int g = 0; g < GT; g++)
+#pragma unroll
+                        for (int q = 0; q < U; q++) st.sum[q] += d.cv[g][q] * d.gv[g][q];
+                }
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
                     if (g == p.s) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
-                            st.cs[q] = d.cv[g][q];
+                            if constexpr (!FACT) st.cs[q] = d.cv[g][q];
                             st.rs[q] = rr[g][q];
                         }
                     }
                     if (p.ndense && g == p.dense_g[0]) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
-                            st.cd[q] = d.cv[g][q];
+                            if constexpr (!FACT) st.cd[q] = d.cv[g][q];
                             st.rd[q] = rr[g][q];
                         }
                     }
@@ -931,31 +955,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     st.rs[q] = st.rd[q] = 0;
                 }
                 load_words(base0, st.hw);
+                T sw[U];
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    sw[q] = (T)1;
+                    if constexpr (FACT) {
+                        if (p.sqrtw) sw[q] = p.sqrtw[clampi(base0, q)];  // wave-uniform test
+                    }
+                }
                 for (int g = 0; g < p.G; g++) {
                     int32_t r[U];
                     T cv[U];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
                         r[q] = p.refs[g][clampi(base0, q)];
-                        cv[q] = p.cache[g][clampi(base0, q)];
+                        cv[q] = FACT ? (T)1 : p.cache[g][clampi(base0, q)];
                     }
-                    const T *cf = p.coef + p.coef_off[g] * K + c;
+                    const T *cf = (FACT ? p.vt : p.coef) + p.coef_off[g] * K + c;
 #pragma unroll
-                    for (int q = 0; q < U; q++) st.sum[q] += cv[q] * cf[(int64_t)r[q] * K];
+                    for (int q = 0; q < U; q++) st.sum[q] += FACT ? cf[(int64_t)r[q] * K] : cv[q] * cf[(int64_t)r[q] * K];
                     if (g == p.s) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
-                            st.cs[q] = cv[q];
+                            st.cs[q] = FACT ? sw[q] : cv[q];
                             st.rs[q] = r[q];
                         }
                     }
                     if (p.ndense && g == p.dense_g[0]) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
-                            st.cd[q] = cv[q];
+                            st.cd[q] = FACT ? sw[q] : cv[q];
                             st.rd[q] = r[q];
                         }
                     }
+                }
+                if constexpr (FACT) {
+#pragma unroll
+                    for (int q = 0; q < U; q++) st.sum[q] = sw[q] * st.sum[q];
                 }
                 compute(st, base0);
             }
@@ -1001,7 +1037,7 @@ struct FixItem {
     int32_t kind, level;
     int64_t coef_off;
     int64_t first;
-    int32_t count, pad;
+    int32_t count, fe;  // fe: the FE the level belongs to
 };
 
 template <typename T>
@@ -1019,6 +1055,11 @@ struct FixParams {
     const double *wsum;
     const uint8_t *sdirect;
     int64_t ns, s_off;
+    int32_t s, pad2;
+    // after a FACT fused pass: the sums of FE g's levels left by it (FE s
+    // segments, few-level FEs) are of sqrt(w)*u and take scale_g here;
+    // nullptr for FEs whose sums already carry their caches
+    const T *fscale[kMaxG];
 };
 
 template <typename T, int K>
@@ -1039,7 +1080,8 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
         for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nsK; e += (int64_t)gridDim.x * kBlock) {
             if (!on || !p.sdirect[e / K]) continue;
             T *dst = p.v + p.s_off * K + e;
-            T out = a * (T)p.wsum[e];
+            const double tot = p.fscale[p.s] ? (double)p.fscale[p.s][e / K] * p.wsum[e] : p.wsum[e];
+            T out = a * (T)tot;
             if (keep) out = b * (*dst) + out;
             *dst = out;
             sq += (double)out * (double)out;
@@ -1067,6 +1109,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
         s = column_allreduce<K>(s);
         if (on && psub == 0) {
             if (it.kind == 0) s = p.tileB[it.first * K + c] + s;
+            if (it.kind != 2 && p.fscale[it.fe]) s = (double)p.fscale[it.fe][it.level] * s;
             T *dst = p.v + (it.coef_off + it.level) * K + c;
             T out = a * (T)s;
             if (keep) out = b * (*dst) + out;
@@ -1094,6 +1137,8 @@ struct UpdParams {
     double *partials;
     const int32_t *all_done;
     const void *state;
+    const T *scale;  // p: with vt, the fused pass's scale .* v (FACT)
+    T *vt;
 };
 
 template <typename T, typename Tr, int K>
@@ -1112,6 +1157,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdParams<T> p) {
                 vn = vn * ia;
                 p.v[e] = vn;
             }
+            if (p.vt) p.vt[e] = p.scale[e / K] * vn;
             const T hv = p.h[e];
             const T hb = p.hbar[e] * chb + hv;
             p.hbar[e] = hb;
@@ -1170,7 +1216,8 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_update(const double *partials
 
 // LSMR set-up of the p-length vectors (:68, :77-78): v /= alpha; h = v; hbar = 0.
 template <typename T, typename Tr, int K>
-__global__ __launch_bounds__(kBlock) void k_init_vectors(int64_t pK, T *v, T *h, T *hbar, const void *state) {
+__global__ __launch_bounds__(kBlock) void k_init_vectors(int64_t pK, T *v, T *h, T *hbar, const void *state,
+                                                        const T *scale, T *vt) {
     const LsmrState<Tr> *st = (const LsmrState<Tr> *)state;
     const int c = threadIdx.x % K;
     const T ia = (T)st[c].inv_alpha;
@@ -1179,6 +1226,7 @@ __global__ __launch_bounds__(kBlock) void k_init_vectors(int64_t pK, T *v, T *h,
         v[e] = vn;
         h[e] = vn;
         hbar[e] = (T)0;
+        if (vt) vt[e] = scale[e / K] * vn;
     }
 }
 
