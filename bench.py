@@ -44,23 +44,25 @@ def log(*a):
 # (scaled by a plain device copy of known size), by (workload, precision):
 # profiles/worker_firm_1e7_fp64_k1.md
 PMC_TRAFFIC = {
-    ("worker_firm_1e7", "fp64"): {"k_fused": 579.0e6, "k_dtx_csr": 352.6e6, "k_update": 70.5e6,
-                                  "k_dtx_fixup": 27.1e6},
+    ("worker_firm_1e7", "fp64"): {"k_fused": 408.3e6, "k_dtx_csr": 352.7e6, "k_update": 88.1e6,
+                                  "k_dtx_fixup": 35.2e6},
 }
 
 
-def kernel_bytes(N, G, n, kinds, K, s):
+def kernel_bytes(N, G, n, kinds, K, s, factored, weighted):
     """Algorithmic HBM bytes per launch of each kernel of one LSMR iteration
     (each array element counted once; gathers counted as their distinct
-    elements)."""
+    elements).  factored: no FE is interacted, so the fused pass reads one
+    sqrt(w) stream (none without weights) instead of every FE's cache."""
     p = sum(n)
     gs = kinds.index("sorted")
     ndense_lev = sum(k for k, kd in zip(n, kinds) if kd == "dense")
     fused_dense = 0 < ndense_lev <= 32
     out = {}
-    # fused pass: refs + cache of every FE, u read + write, each coefficient
-    # once, FE s head bits, raw FE s sums (double) out
-    out["k_fused"] = N * G * (4 + s) + K * N * 2 * s + K * p * s + N // 8 + 8 * K * n[gs]
+    # fused pass: refs of every FE, caches (or sqrt(w)), u read + write, each
+    # coefficient once, FE s head bits, raw FE s sums (double) out
+    per_obs = N * G * (4 + s) if not factored else N * G * 4 + (N * s if weighted else 0)
+    out["k_fused"] = per_obs + K * N * 2 * s + K * p * s + N // 8 + 8 * K * n[gs]
     csr = 0
     dense = 0
     for g in range(G):
@@ -70,9 +72,10 @@ def kernel_bytes(N, G, n, kinds, K, s):
             dense += N * (4 + s) + K * N * s
     out["k_dtx_csr"] = csr
     out["k_dtx_dense"] = dense
-    # FE s levels: raw sum (double) + v read + write, one flag byte
-    out["k_dtx_fixup"] = K * n[gs] * (8 + 2 * s) + n[gs]
-    out["k_update"] = 8 * K * p * s
+    # FE s levels: raw sum (double) + v read + write, one flag byte (+ scale)
+    out["k_dtx_fixup"] = K * n[gs] * (8 + 2 * s) + n[gs] + (n[gs] * s if factored else 0)
+    # v, h, hbar, x read + write (+ scale read, scale .* v written)
+    out["k_update"] = 8 * K * p * s + ((K + 1) * p * s if factored else 0)
     # the pair formula of the survey: D*v + D'u over all G FEs with perm+cache each
     out["pair"] = 2 * N * G * (4 + s) + K * (2 * N * s + N * G * s + 3 * p * s) + 4 * (p + G)
     return out
@@ -197,7 +200,9 @@ def main():
     s_max = int(np.argmax(panel.n))
     for gi, k in enumerate(panel.n):
         kinds.append("sorted" if gi == s_max else ("dense" if k <= 64 else "csr"))
-    kb = kernel_bytes(N, len(panel.n), panel.n, kinds, K, 8 if args.precision == "fp64" else 4)
+    factored = all(z is None for z in panel.interactions)
+    kb = kernel_bytes(N, len(panel.n), panel.n, kinds, K, 8 if args.precision == "fp64" else 4, factored,
+                      panel.weights is not None)
     kt = {nm: float(m) for nm, m in zip(names, ms)}
     dom = max(["k_fused", "k_dtx_csr", "k_dtx_dense", "k_dtx_fixup", "k_update"], key=lambda q: kt[q])
     achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
