@@ -128,6 +128,7 @@ def main():
     ap.add_argument("--maxiter", type=int, default=10000)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--pcie-steps", type=int, default=2, help="timed solves through the host-buffer entry point")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-sized default schedule")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the solve is not sharded across GPUs "
@@ -150,6 +151,12 @@ def main():
     feM = api.FixedEffectSolverROCm(fes, panel.weights, args.precision)
     setup_s = time.perf_counter() - t0
     log(f"[bench] solver set-up {setup_s:.2f} s")
+    sched = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        sched = feM.autotune(K=K, reps=5)
+        sched["seconds"] = time.perf_counter() - t0
+        log(f"[bench] autotune: {sched}")
 
     N = panel.N
     pristine = device.DeviceBuffer(N * K)   # column c at offset c*N
@@ -238,6 +245,7 @@ def main():
         "converged": bool(all(cvs)),
         "normal_eq_rel": neq,
         "setup_s": setup_s,
+        "schedule": sched,
         "kernel_ms": kt,
         "hbm_gbs_pair": pair_gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -117,6 +117,10 @@ struct fehip_solver {
     uint32_t *hist = nullptr, *sums = nullptr;
     int64_t sort_cap = 0;
     int poll_iters = 8;
+    // schedule overrides (fehip_set_schedule / fehip_autotune); 0 or -1 =
+    // the default (environment, else sized by occupancy)
+    int64_t sched_tiles_s = 0, sched_tiles_c = 0;
+    int sched_fuse_dense = -1;
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
     std::vector<double> last_info;  // LSMR scalars of the last solve, 9 per column
     // warm start: per-level degrees (lazy), feature / prediction scratch
@@ -697,6 +701,12 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
 // it) uses ~4096 tiles, the CSR FEs ~8192 (their gathers want more waves in
 // flight).  Both are multiples of 256 positions.  FEHIP_TILES_S and
 // FEHIP_TILES_C override the counts.
+// few-level FEs accumulated by the fused pass (when they fit its LDS)
+static bool fuse_dense_on(const fehip_solver *h) {
+    if (h->sched_fuse_dense >= 0) return h->sched_fuse_dense != 0;
+    return env_int("FEHIP_FUSE_DENSE", 1, 0, 1) != 0;
+}
+
 static int64_t tile_size(int64_t N, int64_t ntiles) {
     const int64_t t = (N + ntiles - 1) / ntiles;
     return std::max<int64_t>(256, (t + 255) / 256 * 256);
@@ -704,6 +714,7 @@ static int64_t tile_size(int64_t N, int64_t ntiles) {
 
 struct Plan {
     int K = 0;
+    int64_t waves_f = 0, waves_c = 0;  // resident waves of the fused pass / the CSR launch
     int32_t total_tiles = 0, nseg = 0, ndense = 0, dense_blocks = 0, dense_chunk = 0, nitems = 0;
     int32_t dense_wpb = 1, dense_lds_wave = 0;
     std::vector<int> seg_fe, dense_fe;  // seg_fe[0] is FE s
@@ -747,8 +758,11 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     Plan *P = pl.get();
     P->K = K;
     const int64_t N = h->N;
-    P->R = tile_size(N, env_int("FEHIP_TILES_S", waves_f, 1, 1 << 20));
-    const int64_t Rc = tile_size(N, env_int("FEHIP_TILES_C", waves_c, 1, 1 << 20));
+    P->waves_f = waves_f;
+    P->waves_c = waves_c;
+    P->R = tile_size(N, h->sched_tiles_s > 0 ? h->sched_tiles_s : env_int("FEHIP_TILES_S", waves_f, 1, 1 << 20));
+    const int64_t Rc =
+        tile_size(N, h->sched_tiles_c > 0 ? h->sched_tiles_c : env_int("FEHIP_TILES_C", waves_c, 1, 1 << 20));
     P->ntiles_s = (int32_t)((N + P->R - 1) / P->R);
     P->fused_blocks = grid_for(P->ntiles_s, kWavesPerBlock);
     std::vector<FixItem> items;
@@ -810,8 +824,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             P->slot0.push_back(P->nslot);
             P->nslot += (int32_t)h->fe[g].n;
         }
-    P->fused_dense = P->nslot > 0 && (size_t)P->nslot * kWave * accsz * kWavesPerBlock <= 65536 &&
-                     env_int("FEHIP_FUSE_DENSE", 1, 0, 1) != 0;
+    P->fused_dense = P->nslot > 0 && (size_t)P->nslot * kWave * accsz * kWavesPerBlock <= 65536 && fuse_dense_on(h);
     P->dense_lds_wave = (int32_t)(accsz * kWave * maxlev);
     int64_t chunk;
     if (P->fused_dense) {
@@ -901,7 +914,7 @@ static size_t fused_lds_bytes(const fehip_solver *h, size_t ts) {
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) nslot += h->fe[g].n;
     const size_t b = (size_t)nslot * kWave * accsz * kWavesPerBlock;
-    return (b <= 65536 && env_int("FEHIP_FUSE_DENSE", 1, 0, 1) != 0) ? b : 0;
+    return (b <= 65536 && fuse_dense_on(h)) ? b : 0;
 }
 
 // resident waves of a kernel on this device (all CUs, kBlock threads)
@@ -943,6 +956,12 @@ static void destroy_plan(fehip_solver *h, Plan *P) {
                     P->v, P->hh, P->hbar, P->dY};
     for (void *q : ptrs) dfree(h, q);
     delete P;
+}
+
+static void destroy_all_plans(fehip_solver *h) {
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto &kv : h->plans) destroy_plan(h, kv.second);
+    h->plans.clear();
 }
 
 // ---------------------------------------------------------------------------
@@ -1427,6 +1446,63 @@ struct Engine {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         HIPCHK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+
+    // Time the launches of one LSMR iteration under candidate schedules and
+    // keep the fastest: tiles of the fused pass and of the CSR launch around
+    // the occupancy-sized defaults, and the few-level FEs in or out of the
+    // fused pass.  out: tiles_s, tiles_c, fuse_dense, poll_iters, best ms,
+    // default ms.
+    static int autotune(fehip_solver *h, int reps, double *out) {
+        h->sched_tiles_s = h->sched_tiles_c = 0;
+        h->sched_fuse_dense = -1;
+        destroy_all_plans(h);
+        Plan *P = nullptr;
+        CHK((get_plan_for<T, Acc, K>(h, &P)));
+        const int64_t wf = P->waves_f, wc = P->waves_c;
+        const bool dense_choice = P->nslot > 0 && fused_lds_bytes(h, sizeof(T)) > 0;
+        auto measure = [&](int64_t ts, int64_t tc, int fd, double &ms) -> int {
+            h->sched_tiles_s = ts;
+            h->sched_tiles_c = tc;
+            h->sched_fuse_dense = fd;
+            destroy_all_plans(h);
+            double k[8];
+            CHK(time_kernels(h, reps, k));
+            ms = 0.0;
+            for (double x : k) ms += x;
+            return 0;
+        };
+        double def_ms = 0.0;
+        CHK(measure(wf, wc, 1, def_ms));
+        double best = def_ms;
+        int64_t bs = wf, bc = wc;
+        int bd = 1;
+        const int64_t cs[] = {wf / 2, wf, 2 * wf};
+        const int64_t cc[] = {wc / 2, wc, 2 * wc, 4 * wc};
+        for (int fd = 1; fd >= (dense_choice ? 0 : 1); fd--)
+            for (int64_t ts : cs)
+                for (int64_t tc : cc) {
+                    if (ts < 1 || tc < 1 || (ts == wf && tc == wc && fd == 1)) continue;
+                    double ms = 0.0;
+                    CHK(measure(ts, tc, fd, ms));
+                    if (ms < best) {
+                        best = ms;
+                        bs = ts;
+                        bc = tc;
+                        bd = fd;
+                    }
+                }
+        h->sched_tiles_s = bs;
+        h->sched_tiles_c = bc;
+        h->sched_fuse_dense = bd;
+        destroy_all_plans(h);
+        out[0] = (double)bs;
+        out[1] = (double)bc;
+        out[2] = (double)bd;
+        out[3] = (double)h->poll_iters;
+        out[4] = best;
+        out[5] = def_ms;
         return 0;
     }
 
@@ -2220,6 +2296,28 @@ int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms) {
     if (!h || !out_ms) return fail(FEHIP_EARG, "bad arguments");
     if (K < 1 || K > kMaxK || reps < 1) return fail(FEHIP_EARG, "K must be in 1..32 and reps >= 1");
     return dispatch(h->precision, padded_K(K), [&](auto e) { return decltype(e)::time_kernels(h, reps, out_ms); });
+}
+
+int fehip_set_schedule(fehip_t *h, int64_t tiles_s, int64_t tiles_c, int32_t fuse_dense, int32_t poll_iters) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (tiles_s < 0 || tiles_c < 0 || tiles_s > (1 << 20) || tiles_c > (1 << 20))
+        return fail(FEHIP_EARG, "tile counts must be in 0..2^20 (0 = default)");
+    if (fuse_dense < -1 || fuse_dense > 1) return fail(FEHIP_EARG, "fuse_dense must be -1, 0 or 1");
+    if (poll_iters < 0 || poll_iters > 1024) return fail(FEHIP_EARG, "poll_iters must be in 0..1024 (0 = keep)");
+    h->sched_tiles_s = tiles_s;
+    h->sched_tiles_c = tiles_c;
+    h->sched_fuse_dense = fuse_dense;
+    if (poll_iters > 0) h->poll_iters = poll_iters;
+    destroy_all_plans(h);
+    return 0;
+}
+
+int fehip_autotune(fehip_t *h, int32_t K, int32_t reps, double *out) {
+    g_err.clear();
+    if (!h || !out) return fail(FEHIP_EARG, "bad arguments");
+    if (K < 1 || K > kMaxK || reps < 1) return fail(FEHIP_EARG, "K must be in 1..32 and reps >= 1");
+    return dispatch(h->precision, padded_K(K), [&](auto e) { return decltype(e)::autotune(h, reps, out); });
 }
 
 int fehip_dims(const fehip_t *h, int64_t *N, int32_t *G, int64_t *p) {

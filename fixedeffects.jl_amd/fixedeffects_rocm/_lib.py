@@ -49,6 +49,8 @@ EXPORTS = (
     "fehip_mul_At",
     "fehip_lsmr_info",
     "fehip_time_kernels",
+    "fehip_set_schedule",
+    "fehip_autotune",
     "fehip_dims",
     "fehip_last_error",
     "fehip_destroy",
@@ -90,6 +92,8 @@ def _load() -> C.CDLL:
     L.fehip_mul_At.argtypes = [_vp, _dp, _dp, C.c_double, C.c_double]
     L.fehip_lsmr_info.argtypes = [_vp, C.c_int32, _dp]
     L.fehip_time_kernels.argtypes = [_vp, C.c_int32, C.c_int32, _dp]
+    L.fehip_set_schedule.argtypes = [_vp, C.c_int64, C.c_int64, C.c_int32, C.c_int32]
+    L.fehip_autotune.argtypes = [_vp, C.c_int32, C.c_int32, _dp]
     L.fehip_dims.argtypes = [_vp, _i64p, _i32p, _i64p]
     L.fehip_last_error.restype = C.c_char_p
     L.fehip_last_error.argtypes = []

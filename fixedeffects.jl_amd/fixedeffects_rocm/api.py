@@ -261,6 +261,17 @@ class FixedEffectSolverROCm:
         check(lib.fehip_get_cache(self._h, g, _ptr(out, C.c_double)))
         return out
 
+    def set_schedule(self, tiles_s: int = 0, tiles_c: int = 0, fuse_dense: int = -1, poll_iters: int = 0):
+        """Schedule of the LSMR launches (0 / -1 = defaults); see fehip_set_schedule."""
+        check(lib.fehip_set_schedule(self._h, int(tiles_s), int(tiles_c), int(fuse_dense), int(poll_iters)))
+
+    def autotune(self, K: int = 1, reps: int = 10) -> dict:
+        """Time candidate schedules on this handle's data and keep the fastest."""
+        out = np.zeros(6, np.float64)
+        check(lib.fehip_autotune(self._h, int(K), int(reps), _ptr(out, C.c_double)))
+        return {"tiles_s": int(out[0]), "tiles_c": int(out[1]), "fuse_dense": int(out[2]),
+                "poll_iters": int(out[3]), "ms_per_iter": float(out[4]), "default_ms_per_iter": float(out[5])}
+
     def lsmr_info(self, col: int = 0) -> dict:
         """LSMR scalars of column col of the last solve."""
         out = np.zeros(9, np.float64)

@@ -212,6 +212,23 @@ int fehip_lsmr_info(fehip_t *h, int32_t col, double *out);
  *   out_ms[7] ||x|| and stopping-rule finalize (one launch) */
 int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms);
 
+/* Schedule of the LSMR launches (no reference counterpart; the reference's
+ * CPU loops have no schedule): tiles of the fused pass over the FE with most
+ * levels and of the CSR D'u launch (one wave per tile; 0 = sized by
+ * occupancy, or FEHIP_TILES_S / FEHIP_TILES_C), few-level FEs accumulated in
+ * the fused pass (1), in their own launch (0) or by default (-1), and LSMR
+ * iterations per captured graph (0 = keep).  Drops the handle's plans; the
+ * next solve rebuilds them.  Results do not depend on the schedule beyond
+ * the order of floating-point sums. */
+int fehip_set_schedule(fehip_t *h, int64_t tiles_s, int64_t tiles_c, int32_t fuse_dense, int32_t poll_iters);
+
+/* Autotuner (no reference counterpart): times the launches of one LSMR
+ * iteration on K columns of this handle's data (`reps` launches each) under
+ * candidate schedules around the occupancy-sized default and keeps the
+ * fastest on the handle.  out[6] = {tiles_s, tiles_c, fuse_dense,
+ * poll_iters, best ms per iteration, default ms per iteration}. */
+int fehip_autotune(fehip_t *h, int32_t K, int32_t reps, double *out);
+
 /* Dimensions of a handle. */
 int fehip_dims(const fehip_t *h, int64_t *N, int32_t *G, int64_t *p);
 

@@ -396,3 +396,25 @@ def test_matvec_skewed(gpu):
         for a, b in ((1.0, 0.0), (0.3, -1.7)):
             np.testing.assert_allclose(s.mul_At(y, x, a, b), o.mul_At(y, x, a, b), rtol=1e-11, atol=1e-11)
             np.testing.assert_allclose(s.mul_A(x, y, a, b), o.mul_A(x, y, a, b), rtol=1e-12, atol=1e-12)
+
+
+def test_schedule_and_autotune(gpu):
+    # any schedule gives the same residuals up to the order of sums; the
+    # autotuner returns a schedule no slower than the default it measured
+    _say("case: schedules / autotune")
+    p = synth.config("worker_firm_1e7", scale=0.02)
+    fes = [api.FixedEffect.from_refs(r, k, z) for r, k, z in zip(p.refs, p.n, p.interactions)]
+    y = np.ascontiguousarray(p.Y[:, 0])
+    o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights)
+    ro = o.solve_residuals(y.copy(), tol=1e-12, maxiter=10000)[0]
+    with api.FixedEffectSolverROCm(fes, p.weights, "fp64") as s:
+        for sched in ((0, 0, -1, 0), (37, 1000, 0, 3), (300, 50, 1, 16)):
+            s.set_schedule(*sched)
+            r = s.solve_residuals(y.copy(), tol=1e-12, maxiter=10000)[0]
+            _say(f"  schedule {sched}: rel err {_rel(r, ro):.2e}")
+            assert _rel(r, ro) <= RES_RTOL
+        best = s.autotune(K=1, reps=3)
+        _say(f"  autotune: {best}")
+        assert best["ms_per_iter"] <= best["default_ms_per_iter"]
+        r = s.solve_residuals(y.copy(), tol=1e-12, maxiter=10000)[0]
+        assert _rel(r, ro) <= RES_RTOL
