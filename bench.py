@@ -33,6 +33,7 @@ WORKLOADS = {
     "worker_firm_1e7": ("worker_firm_1e7", 1),
     "worker_firm_1e7_k32": ("worker_firm_1e7_k32", 32),
     "worker_firm_1e8_k16": ("worker_firm_1e8_k16", 16),
+    "uneven_panels": ("uneven_panels", None),
 }
 
 
@@ -116,6 +117,80 @@ def cpu_baseline(panel, iters_gpu, budget_s, nthreads):
     }
 
 
+def run_panels(args, precision):
+    """Many uneven panels (N log-uniform in [1e4, 2e6], 2-3 FEs, 1-8 columns
+    each) residualized in one fehip_solve_residuals_panels call; the same
+    panels solved one after another for comparison."""
+    import numpy as np
+
+    from fixedeffects_rocm import api, device, synth
+
+    t0 = time.perf_counter()
+    panels = [synth.uneven_panel(500_000 + i, 1e4, 2e6) for i in range(args.panels)]
+    log(f"[bench] {len(panels)} panels, {sum(p.N for p in panels)} obs, {sum(p.K for p in panels)} columns "
+        f"({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    solvers = [api.FixedEffectSolverROCm([api.FixedEffect.from_refs(r, k, z) for r, k, z in
+                                          zip(p.refs, p.n, p.interactions)], p.weights, precision) for p in panels]
+    setup_s = time.perf_counter() - t0
+    pristine, work = [], []
+    for p in panels:
+        b = device.DeviceBuffer(p.N * p.K)
+        b.upload(np.asfortranarray(p.Y).ravel(order="F"))
+        pristine.append(b)
+        work.append(device.DeviceBuffer(p.N * p.K))
+    cols = sum(p.K for p in panels)
+    Ks, lds = [p.K for p in panels], [p.N for p in panels]
+
+    def reset():
+        for a, b in zip(pristine, work):
+            b.copy_from(a)
+        device.synchronize()
+
+    def together():
+        return api.solve_residuals_panels(solvers, [b.address for b in work], Ks, lds, tol=args.tol,
+                                          maxiter=args.maxiter)
+
+    def one_by_one():
+        for s, b, p in zip(solvers, work, panels):
+            s.solve_residuals_device(b.address, p.K, p.N, tol=args.tol, maxiter=args.maxiter)
+
+    for _ in range(args.warmup):
+        reset()
+        together()
+        reset()
+        one_by_one()
+    tt = ts = 0.0
+    for _ in range(args.steps):
+        reset()
+        t0 = time.perf_counter()
+        its, cvs = together()
+        tt += time.perf_counter() - t0
+        reset()
+        t0 = time.perf_counter()
+        one_by_one()
+        ts += time.perf_counter() - t0
+    ms = tt / args.steps * 1e3
+    ms_seq = ts / args.steps * 1e3
+    log(f"[bench] panels together {ms:.1f} ms/step, one by one {ms_seq:.1f} ms/step")
+    flat = [i - 1 for a in its for i in a]
+    out = {
+        "metric": "residualized columns/s @1e-8, uneven panels (N log-uniform 1e4..2e6, 2-3 FEs, 1-8 columns)",
+        "value": cols / (ms / 1e3), "unit": "columns/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded numpy PCG64 uneven worker/firm panels)",
+        "config": {"workload": "uneven_panels", "panels": len(panels), "N_total": int(sum(p.N for p in panels)),
+                   "columns": cols, "tol": args.tol, "precision": precision},
+        "one_by_one_value": cols / (ms_seq / 1e3), "lsmr_iters_mean": float(np.mean(flat)),
+        "converged": bool(all(all(c) for c in cvs)), "setup_s": setup_s,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    for s in solvers:
+        s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +204,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--pcie-steps", type=int, default=2, help="timed solves through the host-buffer entry point")
     ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-sized default schedule")
+    ap.add_argument("--panels", type=int, default=64, help="panels of the uneven_panels workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the solve is not sharded across GPUs "
@@ -143,6 +219,9 @@ def main():
     from fixedeffects_rocm._lib import check, lib
 
     name, K = WORKLOADS[args.workload]
+    if name == "uneven_panels":
+        run_panels(args, args.precision)
+        return
     t0 = time.perf_counter()
     panel = synth.config(name)
     log(f"[bench] panel {name}: N={panel.N} n={panel.n} K={panel.K} ({time.perf_counter() - t0:.1f} s)")

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -1557,7 +1558,23 @@ struct Engine {
 
     // LSMR on the columns held in P->u (b) with x as the initial iterate
     // (src/utils/lsmr.jl:46-222); per-column results in the state array.
-    static int lsmr(fehip_solver *h, Plan *P, int Kreal, double tol, int64_t maxiter) {
+    // lsmr_start queues the set-up and the first two replays of the captured
+    // iterations; lsmr_poll advances one replay when its stop flag has come
+    // back (blocking or not).  Launches after the stop exit at entry.
+    struct Run {
+        int64_t launched = 0, maxiter = 0;
+        int cur = 0;
+    };
+
+    static int launch_replay(fehip_solver *h, Plan *P, Run &run, int slot) {
+        HIPCHK(hipGraphLaunch(P->graph, h->stream));
+        HIPCHK(hipMemcpyAsync(h->h_flag + 1 + slot, P->all_done, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipEventRecord(h->poll_ev[slot], h->stream));
+        run.launched += P->graph_iters;
+        return 0;
+    }
+
+    static int lsmr_start(fehip_solver *h, Plan *P, int Kreal, double tol, int64_t maxiter, Run &run) {
         hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, Kreal,
                            tol, tol, 1e8, maxiter, P->all_done);
         // u = b - A x ; beta = ||u||  (:63-65)
@@ -1572,31 +1589,84 @@ struct Engine {
                            (const T *)h->scale_all[pidx<T>()], (T *)P->vt);
         HIPCHK(hipGetLastError());
         CHK(ensure_graph(h, P));
-        HIPCHK(hipMemcpyAsync(h->h_flag, P->all_done, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        if (h->h_flag[0]) return 0;  // stopped at set-up (b = 0, A'b = 0 or maxiter = 0)
-        // Two replays in flight: while the host waits on replay i's copy of the
-        // stop flag, replay i+1 is already queued, so the GPU never drains.
-        // Launches after the stop exit at entry.
-        int64_t launched = 0;
-        auto launch = [&](int slot) -> int {
-            HIPCHK(hipGraphLaunch(P->graph, h->stream));
-            HIPCHK(hipMemcpyAsync(h->h_flag + 1 + slot, P->all_done, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                  h->stream));
-            HIPCHK(hipEventRecord(h->poll_ev[slot], h->stream));
-            launched += P->graph_iters;
-            return 0;
-        };
-        CHK(launch(0));
-        CHK(launch(1));
-        for (int cur = 0;; cur ^= 1) {
-            HIPCHK(hipEventSynchronize(h->poll_ev[cur]));
-            if (h->h_flag[1 + cur]) break;
-            if (launched > maxiter + 3 * P->graph_iters)
-                return fail(FEHIP_EHIP, "LSMR did not reach its stopping rule within maxiter");
-            CHK(launch(cur));
+        // Two replays in flight: while the host waits on replay i's copy of
+        // the stop flag, replay i+1 is already queued, so the GPU never
+        // drains.  A stop at set-up (b = 0, A'b = 0 or maxiter = 0) makes
+        // both exit at entry.
+        run = Run{};
+        run.maxiter = maxiter;
+        CHK(launch_replay(h, P, run, 0));
+        CHK(launch_replay(h, P, run, 1));
+        return 0;
+    }
+
+    // finished = true once the stop flag is seen (the stream is then idle)
+    static int lsmr_poll(fehip_solver *h, Plan *P, Run &run, bool block, bool &finished) {
+        finished = false;
+        if (block) {
+            HIPCHK(hipEventSynchronize(h->poll_ev[run.cur]));
+        } else {
+            const hipError_t q = hipEventQuery(h->poll_ev[run.cur]);
+            if (q == hipErrorNotReady) return 0;
+            if (q != hipSuccess) return fail(FEHIP_EHIP, std::string("event query: ") + hipGetErrorString(q));
         }
-        HIPCHK(hipStreamSynchronize(h->stream));
+        if (h->h_flag[1 + run.cur]) {
+            HIPCHK(hipStreamSynchronize(h->stream));
+            finished = true;
+            return 0;
+        }
+        if (run.launched > run.maxiter + 3 * P->graph_iters)
+            return fail(FEHIP_EHIP, "LSMR did not reach its stopping rule within maxiter");
+        CHK(launch_replay(h, P, run, run.cur));
+        run.cur ^= 1;
+        return 0;
+    }
+
+    static int lsmr(fehip_solver *h, Plan *P, int Kreal, double tol, int64_t maxiter) {
+        Run run;
+        CHK(lsmr_start(h, P, Kreal, tol, maxiter, run));
+        for (bool fin = false; !fin;) CHK(lsmr_poll(h, P, run, true, fin));
+        return 0;
+    }
+
+    // solve_residuals! of one panel as a resumable job (for the concurrent
+    // driver of many panels): begin queues the load and the LSMR, step
+    // advances it without blocking, end stores the residuals
+    struct Job {
+        fehip_solver *h = nullptr;
+        Plan *P = nullptr;
+        Run run;
+        double *dY = nullptr;
+        int Kreal = 0;
+        int64_t ldY = 0;
+        int64_t *iters = nullptr;
+        uint8_t *conv = nullptr;
+    };
+    static int job_begin(Job &j, double tol, int64_t maxiter) {
+        CHK((get_plan_for<T, Acc, K>(j.h, &j.P)));
+        CHK(start(j.h, j.P, j.dY, j.Kreal, j.ldY, true));
+        if (j.h->G > 1) CHK(lsmr_start(j.h, j.P, j.Kreal, tol, maxiter - 1, j.run));
+        return 0;
+    }
+    static int job_step(Job &j, bool &finished) {
+        if (j.h->G == 1) {
+            finished = true;
+            return 0;
+        }
+        return lsmr_poll(j.h, j.P, j.run, false, finished);
+    }
+    static int job_end(Job &j) {
+        std::vector<int64_t> it(j.Kreal, 1);
+        std::vector<int> cv(j.Kreal, 1);
+        if (j.h->G > 1) {
+            CHK(read_state(j.h, j.P, j.Kreal, it, cv));
+            for (int c = 0; c < j.Kreal; c++) it[c] += 1;
+        }
+        CHK(finish(j.h, j.P, j.dY, j.Kreal, j.ldY));
+        for (int c = 0; c < j.Kreal; c++) {
+            if (j.iters) j.iters[c] = it[c];
+            if (j.conv) j.conv[c] = (uint8_t)cv[c];
+        }
         return 0;
     }
 
@@ -2028,6 +2098,77 @@ int fehip_solve_residuals_device(fehip_t *h, double *dY, int32_t K, int64_t ldY,
                                 converged ? converged + c0 : nullptr));
     }
     return 0;
+}
+
+int fehip_solve_residuals_panels(fehip_t *const *hs, int32_t npanels, double *const *dY, const int32_t *K,
+                                 const int64_t *ldY, double tol, int64_t maxiter, int64_t *const *iters,
+                                 uint8_t *const *converged) {
+    g_err.clear();
+    if (!hs || !dY || !K || !ldY || npanels < 0) return fail(FEHIP_EARG, "bad arguments");
+    if (maxiter < 1) return fail(FEHIP_EARG, "maxiter must be >= 1");
+    for (int32_t i = 0; i < npanels; i++) {
+        if (!hs[i] || !dY[i]) return fail(FEHIP_EARG, "panel " + std::to_string(i) + ": NULL handle or column block");
+        if (K[i] < 1 || K[i] > kMaxK) return fail(FEHIP_EARG, "panel " + std::to_string(i) + ": K must be in 1..32");
+        if (ldY[i] < hs[i]->N) return fail(FEHIP_EDIM, "panel " + std::to_string(i) + ": ldY < N");
+        for (int32_t j = 0; j < i; j++)
+            if (hs[j] == hs[i]) return fail(FEHIP_EARG, "a handle appears twice (one stream per panel)");
+    }
+    // one resumable job per panel, each on its handle's own stream; mixed
+    // precision panels (two LSMR phases) run as one blocking step
+    struct AnyJob {
+        std::function<int()> begin, end;
+        std::function<int(bool &)> step;
+        bool done = false;
+    };
+    std::vector<AnyJob> jobs(npanels);
+    for (int32_t i = 0; i < npanels; i++) {
+        fehip_solver *h = hs[i];
+        int64_t *it = iters ? iters[i] : nullptr;
+        uint8_t *cv = converged ? converged[i] : nullptr;
+        if (h->precision == FEHIP_MIXED) {
+            double *d = dY[i];
+            const int32_t k = K[i];
+            const int64_t ld = ldY[i];
+            jobs[i].begin = [=] { return solve_residuals_dev(h, d, k, ld, tol, maxiter, it, cv); };
+            jobs[i].step = [](bool &f) {
+                f = true;
+                return 0;
+            };
+            jobs[i].end = [] { return 0; };
+            continue;
+        }
+        CHK(dispatch(h->precision, padded_K(K[i]), [&](auto e) {
+            using E = decltype(e);
+            auto j = std::make_shared<typename E::Job>();
+            j->h = h;
+            j->dY = dY[i];
+            j->Kreal = K[i];
+            j->ldY = ldY[i];
+            j->iters = it;
+            j->conv = cv;
+            jobs[i].begin = [j, tol, maxiter] { return E::job_begin(*j, tol, maxiter); };
+            jobs[i].step = [j](bool &f) { return E::job_step(*j, f); };
+            jobs[i].end = [j] { return E::job_end(*j); };
+            return 0;
+        }));
+    }
+    int rc = 0;
+    for (int32_t i = 0; i < npanels && rc == 0; i++) rc = jobs[i].begin();
+    for (int32_t left = npanels; rc == 0 && left > 0;) {
+        for (int32_t i = 0; i < npanels && rc == 0; i++) {
+            if (jobs[i].done) continue;
+            bool fin = false;
+            rc = jobs[i].step(fin);
+            if (rc == 0 && fin) {
+                rc = jobs[i].end();
+                jobs[i].done = true;
+                left--;
+            }
+        }
+    }
+    if (rc != 0)
+        for (int32_t i = 0; i < npanels; i++) (void)hipStreamSynchronize(hs[i]->stream);
+    return rc;
 }
 
 int fehip_solve_residuals(fehip_t *h, double *const *cols, int32_t K, double tol, int64_t maxiter, int64_t *iters,

@@ -41,6 +41,7 @@ EXPORTS = (
     "fehip_update_weights",
     "fehip_solve_residuals",
     "fehip_solve_residuals_device",
+    "fehip_solve_residuals_panels",
     "fehip_solve_coefficients",
     "fehip_get_csr",
     "fehip_get_scale",
@@ -84,6 +85,9 @@ def _load() -> C.CDLL:
     L.fehip_update_weights.argtypes = [_vp, _dp]
     L.fehip_solve_residuals.argtypes = [_vp, C.POINTER(_dp), C.c_int32, C.c_double, C.c_int64, _i64p, _u8p]
     L.fehip_solve_residuals_device.argtypes = [_vp, _vp, C.c_int32, C.c_int64, C.c_double, C.c_int64, _i64p, _u8p]
+    L.fehip_solve_residuals_panels.argtypes = [C.POINTER(_vp), C.c_int32, C.POINTER(_vp), C.POINTER(C.c_int32),
+                                               C.POINTER(C.c_int64), C.c_double, C.c_int64, C.POINTER(_i64p),
+                                               C.POINTER(_u8p)]
     L.fehip_solve_coefficients.argtypes = [_vp, _dp, C.c_double, C.c_int64, C.POINTER(_dp), _i64p, _u8p]
     L.fehip_get_csr.argtypes = [_vp, C.c_int32, _i32p, _i32p]
     L.fehip_get_scale.argtypes = [_vp, C.c_int32, _dp]

@@ -343,6 +343,24 @@ def _outer(method, double_precision, tol):
     return double_precision, tol
 
 
+def solve_residuals_panels(solvers, dptrs, Ks, lds, tol: float = 1e-8, maxiter: int = 10_000):
+    """Residualize many panels in one call (fehip_solve_residuals_panels):
+    panel i has K[i] device-resident columns at dptrs[i] (column c at
+    dptrs[i] + 8*c*lds[i]) and its own solver.  All panels' LSMR iterations
+    run concurrently; returns (iterations, converged) per panel."""
+    P = len(solvers)
+    hs = (C.c_void_p * P)(*[s._h for s in solvers])
+    ds = (C.c_void_p * P)(*[C.c_void_p(int(d)) for d in dptrs])
+    ks = (C.c_int32 * P)(*[int(k) for k in Ks])
+    ls = (C.c_int64 * P)(*[int(x) for x in lds])
+    its = [np.zeros(int(k), np.int64) for k in Ks]
+    cvs = [np.zeros(int(k), np.uint8) for k in Ks]
+    ip = (C.POINTER(C.c_int64) * P)(*[_ptr(a, C.c_int64) for a in its])
+    cp = (C.POINTER(C.c_uint8) * P)(*[_ptr(a, C.c_uint8) for a in cvs])
+    check(lib.fehip_solve_residuals_panels(hs, P, ds, ks, ls, float(tol), int(maxiter), ip, cp))
+    return [[int(v) for v in a] for a in its], [[bool(v) for v in a] for a in cvs]
+
+
 def solve_residuals(y, fes, w=None, *, method: str = "rocm", double_precision: Optional[bool] = None,
                     tol: Optional[float] = None, maxiter: int = 10000, precision: Optional[str] = None,
                     progress_bar: bool = True):

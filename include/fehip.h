@@ -96,6 +96,19 @@ int fehip_solve_residuals_device(fehip_t *h, double *dY, int32_t K,
                                  int64_t ldY, double tol, int64_t maxiter,
                                  int64_t *iters, uint8_t *converged);
 
+/* Many panels in one call (multi-panel form of the multi-column
+ * solve_residuals!, src/AbstractFixedEffectSolver.jl:74-97, over several
+ * fixed-effect structures): panel i residualizes K[i] <= 32 device-resident
+ * columns dY[i] (column c at dY[i] + c*ldY[i]) with its own handle hs[i].
+ * All panels' LSMR iterations are in flight together, each on its handle's
+ * stream; the host advances whichever panel's stop flag has come back.
+ * Results are those of fehip_solve_residuals_device on each handle alone.
+ * iters[i] / converged[i] (K[i] entries each) may be NULL.  Handles must be
+ * distinct. */
+int fehip_solve_residuals_panels(fehip_t *const *hs, int32_t npanels, double *const *dY, const int32_t *K,
+                                 const int64_t *ldY, double tol, int64_t maxiter, int64_t *const *iters,
+                                 uint8_t *const *converged);
+
 /* Replaces the inner solve_coefficients!(r, feM; tol, maxiter)
  * (src/AbstractFixedEffectSolver.jl:147-160) up to the collect at :158:
  * LSMR from x0 = 0 with maxiter, then x .*= scale.  coef[g] receives the

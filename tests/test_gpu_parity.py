@@ -418,3 +418,33 @@ def test_schedule_and_autotune(gpu):
         assert best["ms_per_iter"] <= best["default_ms_per_iter"]
         r = s.solve_residuals(y.copy(), tol=1e-12, maxiter=10000)[0]
         assert _rel(r, ro) <= RES_RTOL
+
+
+def test_panels_match_single_solves(gpu):
+    # uneven panels solved together give exactly the residuals and
+    # iteration counts of solving each on its own handle
+    _say("case: panels")
+    from fixedeffects_rocm import device
+    panels = [synth.uneven_panel(700 + i, 1e4, 8e4) for i in range(6)]
+    solvers = [api.FixedEffectSolverROCm([api.FixedEffect.from_refs(r, k, z) for r, k, z in
+                                          zip(p.refs, p.n, p.interactions)], p.weights, "fp64") for p in panels]
+    try:
+        single, bufs = [], []
+        for p, s in zip(panels, solvers):
+            b = device.DeviceBuffer(p.N * p.K)
+            b.upload(np.asfortranarray(p.Y).ravel(order="F"))
+            its, cvs = s.solve_residuals_device(b.address, p.K, p.N, tol=1e-8, maxiter=10000)
+            single.append((b.download(p.N * p.K), its))
+            b.upload(np.asfortranarray(p.Y).ravel(order="F"))
+            bufs.append(b)
+        its, cvs = api.solve_residuals_panels(solvers, [b.address for b in bufs], [p.K for p in panels],
+                                              [p.N for p in panels], tol=1e-8, maxiter=10000)
+        for i, (p, b) in enumerate(zip(panels, bufs)):
+            r = b.download(p.N * p.K)
+            _say(f"  panel {i}: N={p.N} K={p.K} iters {its[i]}")
+            assert its[i] == single[i][1]
+            assert all(cvs[i])
+            np.testing.assert_array_equal(r, single[i][0])
+    finally:
+        for s in solvers:
+            s.close()
