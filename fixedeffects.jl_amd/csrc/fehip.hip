@@ -761,7 +761,13 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     const int64_t N = h->N;
     P->waves_f = waves_f;
     P->waves_c = waves_c;
+    // the fused pass addresses a tile of u and the coefficient vectors with
+    // 32-bit byte offsets: R*K*8 and p*K*8 stay below 2^32
+    if ((double)h->p * K * sizeof(double) >= 4294967296.0)
+        return fail(FEHIP_EDIM, "p * K * 8 bytes exceeds 2^32: use fewer columns per batch");
     P->R = tile_size(N, h->sched_tiles_s > 0 ? h->sched_tiles_s : env_int("FEHIP_TILES_S", waves_f, 1, 1 << 20));
+    const int64_t rmax = ((int64_t)1 << 31) / ((int64_t)K * (int64_t)sizeof(double)) / 256 * 256;
+    P->R = std::min<int64_t>(P->R, std::max<int64_t>(rmax, 256));
     const int64_t Rc =
         tile_size(N, h->sched_tiles_c > 0 ? h->sched_tiles_c : env_int("FEHIP_TILES_C", waves_c, 1, 1 << 20));
     P->ntiles_s = (int32_t)((N + P->R - 1) / P->R);
@@ -2044,6 +2050,14 @@ static int for_K_mixed(int K, F &&f) {
     }
 }
 
+// widest batch (a power of two <= 32) whose coefficient vectors p*K*8 stay
+// below 2^32 bytes (the fused pass's 32-bit offsets); 0 if not even one
+static int max_batch(const fehip_solver *h) {
+    int k = kMaxK;
+    while (k > 0 && (double)h->p * k * sizeof(double) >= 4294967296.0) k >>= 1;
+    return k;
+}
+
 static int padded_K(int K) {
     int k = 1;
     while (k < K) k <<= 1;
@@ -2092,8 +2106,10 @@ int fehip_solve_residuals_device(fehip_t *h, double *dY, int32_t K, int64_t ldY,
     if (!dY) return fail(FEHIP_EARG, "dY is NULL");
     if (ldY < h->N) return fail(FEHIP_EDIM, "ldY < N");
     if (maxiter < 1) return fail(FEHIP_EARG, "maxiter must be >= 1");
-    for (int32_t c0 = 0; c0 < K; c0 += kMaxK) {
-        const int kb = std::min<int32_t>(kMaxK, K - c0);
+    const int bw = max_batch(h);
+    if (bw < 1) return fail(FEHIP_EDIM, "p * 8 bytes exceeds 2^32");
+    for (int32_t c0 = 0; c0 < K; c0 += bw) {
+        const int kb = std::min<int32_t>(bw, K - c0);
         CHK(solve_residuals_dev(h, dY + (int64_t)c0 * ldY, kb, ldY, tol, maxiter, iters ? iters + c0 : nullptr,
                                 converged ? converged + c0 : nullptr));
     }
@@ -2177,12 +2193,14 @@ int fehip_solve_residuals(fehip_t *h, double *const *cols, int32_t K, double tol
     if (!h) return fail(FEHIP_EARG, "handle is NULL");
     if (K < 1 || !cols) return fail(FEHIP_EARG, "need K >= 1 columns");
     if (maxiter < 1) return fail(FEHIP_EARG, "maxiter must be >= 1");
-    const int32_t kb = std::min<int32_t>(kMaxK, K);
+    const int bw = max_batch(h);
+    if (bw < 1) return fail(FEHIP_EDIM, "p * 8 bytes exceeds 2^32");
+    const int32_t kb = std::min<int32_t>(bw, K);
     double *dY = nullptr;
     CHK(dalloc(h, &dY, (size_t)h->N * kb));
     int rc = 0;
-    for (int32_t c0 = 0; c0 < K && rc == 0; c0 += kMaxK) {
-        const int k = std::min<int32_t>(kMaxK, K - c0);
+    for (int32_t c0 = 0; c0 < K && rc == 0; c0 += bw) {
+        const int k = std::min<int32_t>(bw, K - c0);
         for (int c = 0; c < k && rc == 0; c++) {
             hipError_t e = hipMemcpyAsync(dY + (int64_t)c * h->N, cols[c0 + c], sizeof(double) * h->N,
                                           hipMemcpyHostToDevice, h->stream);

@@ -442,19 +442,17 @@ struct SegCursor {
     bool open;  // a run is still open after the last step
 };
 
-// One wave step at `base` (P = 64/K positions, base a multiple of P): the
+// One wave step at position `base` (P = 64/K positions, base a multiple of
+// P; sh = base mod 64, valid = this lane's position lies in the tile): the
 // run carried in is added and the step is scanned in place (x).  Returns true
 // on the lane where a segment that started in this tile ends (the caller
 // writes it); a segment that began in an earlier tile ends in *tileA_tc.
 // w = heads[base >> 6], wnext = heads[(base >> 6) + 1], loaded by the caller
 // together with the step's data, so no load sits on the scan's chain.
 template <int K, typename Acc>
-__device__ __forceinline__ bool seg_step(uint64_t w, uint64_t wnext, int64_t base, int64_t end, int psub, int c,
-                                         bool on, Acc &x, SegCursor<Acc> &cur, double *tileA_tc) {
+__device__ __forceinline__ bool seg_step(uint64_t w, uint64_t wnext, int sh, bool valid, int psub, int c, bool on,
+                                         Acc &x, SegCursor<Acc> &cur, double *tileA_tc) {
     constexpr int P = kWave / K;
-    const int64_t pos = base + psub;
-    const bool valid = pos < end;
-    const int sh = (int)(base & 63);
     const uint64_t hb = (P == 64) ? w : ((w >> sh) & ((1ull << (P & 63)) - 1ull));
     const uint64_t nb = (sh + P < 64) ? ((w >> (sh + P)) & 1ull) : (wnext & 1ull);
     const uint64_t hle = hb & lanes_le(psub);
@@ -591,7 +589,8 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
             for (int q = 0; q < U; q++) {
                 const int64_t base = base0 + q * P;
                 if (base >= end) break;  // wave-uniform
-                em[q] = seg_step<K>(hw[q], hn[q], base, end, psub, c, on, val[q], cur, &p.tileA[(int64_t)t * K + c]);
+                em[q] = seg_step<K>(hw[q], hn[q], (int)(base & 63), base + psub < end, psub, c, on, val[q], cur,
+                                    &p.tileA[(int64_t)t * K + c]);
             }
 #pragma unroll
             for (int q = 0; q < U; q++) {
@@ -748,6 +747,18 @@ struct FusedParams {
 // head words and the coefficient gathers, whose refs arrived one step
 // earlier) and the refs of step j+2 are in flight.  GT = 0: any FE count,
 // one FE at a time, no pipelining.
+// Loads and stores at a 32-bit byte offset from a wave-uniform base (the
+// scalar-base + 32-bit-offset addressing form): the host keeps every offset
+// below 2^32 (tile of R positions: R*K*8; coefficient vectors: p*K*8).
+template <typename V>
+__device__ __forceinline__ V ldo(const V *base, uint32_t i) {
+    return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(V));
+}
+template <typename V>
+__device__ __forceinline__ void sto(V *base, uint32_t i, V v) {
+    *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(V)) = v;
+}
+
 template <typename T, typename Acc, int K, int GT, bool FACT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
@@ -756,7 +767,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // wave steps per loop step; FP64 and batched columns take fewer to stay
     // within 128 VGPRs (4 waves per SIMD) without spills
     constexpr int U = (sizeof(T) == 8 || K > 1) ? FUSED_U64 : 4;
-    constexpr int UP = U * P;
+    constexpr uint32_t UP = U * P;
     // head words one loop step touches: steps q use word (q*P)/64 and the next
     constexpr int NW = UP / kWave + 1 > 2 ? UP / kWave + 1 : 2;
     constexpr int GA = GT > 0 ? GT : 1;
@@ -773,6 +784,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (t < p.ntiles) {  // wave-uniform
         const int64_t start = (int64_t)t * p.R;
         const int64_t end = start + p.R < p.N ? start + p.R : p.N;
+        const uint32_t len = (uint32_t)(end - start);
+        // tile-relative bases; positions below are offsets from start
+        T *ub = p.target + start * K;
+        const T *swb = (FACT && p.sqrtw) ? p.sqrtw + start : nullptr;
         SegCursor<Acc> cur{Acc(0.f), true, false};
         // one loop step, ready to compute: D*v sums, old u, FE s and first
         // dense FE cache/ref, head words
@@ -781,44 +796,45 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             int32_t rs[U], rd[U];
             uint64_t hw[NW];
         };
-        // position of lane, step q, clamped into the tile (loads are
-        // unconditional, so no load waits inside a branch)
-        auto clampi = [&](int64_t b0, int q) -> int64_t {
-            const int64_t i = b0 + q * P + psub;
-            return i < end ? i : start;
+        // offset of lane, step q of the loop step at lb, clamped into the
+        // tile (loads are unconditional, so no load waits inside a branch)
+        auto loc = [&](uint32_t lb, int q) -> uint32_t {
+            const uint32_t o = lb + (uint32_t)(q * P + psub);
+            return o < len ? o : 0u;
         };
-        // base0 is a multiple of U*P and of 64 when U*P >= 64; the words up
-        // to (end-1)/64 + 1 exist (the head bits run one word past N)
-        auto load_words = [&](int64_t b0, uint64_t (&hw)[NW]) {
+        // start is a multiple of 256 and lb of U*P; the words up to
+        // (end-1)/64 + 1 exist (the head bits run one word past N)
+        auto load_words = [&](uint32_t lb, uint64_t (&hw)[NW]) {
+            const int64_t b0 = start + lb;
 #pragma unroll
             for (int j = 0; j < NW; j++) {
                 const int64_t wi = (b0 >> 6) + j;
-                hw[j] = (b0 < end && wi <= ((end - 1) >> 6) + 1) ? p.heads[wi] : 0ull;
+                hw[j] = (lb < len && wi <= ((end - 1) >> 6) + 1) ? p.heads[wi] : 0ull;
             }
         };
-        auto compute = [&](const Step &st, int64_t base0) {
+        auto compute = [&](const Step &st, uint32_t lb) {
             T un[U];
 #pragma unroll
             for (int q = 0; q < U; q++) {
-                const int64_t idx = base0 + q * P + psub;
-                const bool ok = on && idx < end;
+                const uint32_t o = lb + (uint32_t)(q * P + psub);
+                const bool ok = on && o < len;
                 T out = a * st.sum[q];
                 out = keep ? b * st.uo[q] + out : out;
                 un[q] = ok ? out : (T)0;
-                if (ok) p.target[idx * K + c] = out;
+                if (ok) sto(ub, o * K + c, out);
                 sq += (double)un[q] * (double)un[q];
             }
             // FE s: segmented sums of cache_s * u; a segment ending at a
             // position belongs to that position's level
 #pragma unroll
             for (int q = 0; q < U; q++) {
-                const int64_t base = base0 + q * P;
-                if (base >= end) break;  // wave-uniform
+                const uint32_t ob = lb + (uint32_t)(q * P);
+                if (ob >= len) break;  // wave-uniform
                 Acc x = mul_acc<Acc, T>(st.cs[q], un[q]);
                 const int jw = (q * P) / kWave;  // constant after unrolling
-                if (seg_step<K>(st.hw[jw], st.hw[jw + 1], base, end, psub, c, on, x, cur,
-                                &p.tileA[(int64_t)t * K + c]))
-                    p.wsum[(int64_t)st.rs[q] * K + c] = acc_to_double(x);
+                if (seg_step<K>(st.hw[jw], st.hw[jw + 1], (int)(ob & 63u), ob + (uint32_t)psub < len, psub, c, on, x,
+                                cur, &p.tileA[(int64_t)t * K + c]))
+                    sto(p.wsum, (uint32_t)st.rs[q] * K + c, acc_to_double(x));
             }
             // dense FEs: lane-private LDS accumulators
             if (p.ndense) {
@@ -832,14 +848,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
                 lds_accumulate<U, Acc>(acc, sl, x, lane);
                 for (int d = 1; d < p.ndense; d++) {
-                    const int32_t *rf = p.refs[p.dense_g[d]];
-                    const T *cf = p.cache[p.dense_g[d]];
+                    const int32_t *rf = p.refs[p.dense_g[d]] + start;
+                    const T *cf = p.cache[p.dense_g[d]] + start;
                     const int sd = p.dense_slot0[d];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        const int64_t i = clampi(base0, q);
-                        sl[q] = sd + rf[i];
-                        x[q] = mul_acc<Acc, T>(FACT ? st.cs[q] : cf[i], un[q]);
+                        const uint32_t i = loc(lb, q);
+                        sl[q] = sd + ldo(rf, i);
+                        x[q] = mul_acc<Acc, T>(FACT ? st.cs[q] : ldo(cf, i), un[q]);
                     }
                     lds_accumulate<U, Acc>(acc, sl, x, lane);
                 }
@@ -847,34 +863,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         };
         if constexpr (GT > 0) {
             // stage A: refs of a step
-            auto issue_refs = [&](int64_t b0, int32_t (&rr)[GA][U]) {
+            auto issue_refs = [&](uint32_t lb, int32_t (&rr)[GA][U]) {
 #pragma unroll
-                for (int g = 0; g < GT; g++)
+                for (int g = 0; g < GT; g++) {
+                    const int32_t *rb = p.refs[g] + start;
 #pragma unroll
-                    for (int q = 0; q < U; q++) rr[g][q] = p.refs[g][clampi(b0, q)];
+                    for (int q = 0; q < U; q++) rr[g][q] = ldo(rb, loc(lb, q));
+                }
             };
             // stage B: every other load of a step (gathers use its refs)
             struct InFlight {
                 T uo[U], cv[FACT ? 1 : GA][U], gv[GA][U], sw[U];
                 uint64_t hw[NW];
             };
-            auto issue_data = [&](int64_t b0, const int32_t (&rr)[GA][U], InFlight &d) {
-                load_words(b0, d.hw);
+            auto issue_data = [&](uint32_t lb, const int32_t (&rr)[GA][U], InFlight &d) {
+                load_words(lb, d.hw);
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    d.uo[q] = p.target[clampi(b0, q) * K + c];
+                    const uint32_t i = loc(lb, q);
+                    d.uo[q] = ldo(ub, i * K + c);
                     d.sw[q] = (T)1;
                     if constexpr (FACT) {
-                        if (p.sqrtw) d.sw[q] = p.sqrtw[clampi(b0, q)];  // wave-uniform test
+                        if (swb) d.sw[q] = ldo(swb, i);  // wave-uniform test
                     }
                 }
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
-                    const T *cf = (FACT ? p.vt : p.coef) + p.coef_off[g] * K + c;
+                    const T *cf = (FACT ? p.vt : p.coef) + p.coef_off[g] * K;
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        d.gv[g][q] = cf[(int64_t)rr[g][q] * K];
-                        if constexpr (!FACT) d.cv[g][q] = p.cache[g][clampi(b0, q)];
+                        d.gv[g][q] = ldo(cf, (uint32_t)rr[g][q] * K + c);
+                        if constexpr (!FACT) d.cv[g][q] = ldo(p.cache[g] + start, loc(lb, q));
                     }
                 }
             };
@@ -926,18 +945,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             int32_t r1[GA][U], r2[GA][U];
             InFlight d;
             Step st;
-            issue_refs(start, r1);
-            issue_data(start, r1, d);
-            issue_refs(start + UP, r2);
+            issue_refs(0u, r1);
+            issue_data(0u, r1, d);
+            issue_refs(UP, r2);
             settle(d, r1, st);
 #pragma unroll
             for (int g = 0; g < GT; g++)
 #pragma unroll
                 for (int q = 0; q < U; q++) r1[g][q] = r2[g][q];
-            for (int64_t base0 = start; base0 < end; base0 += UP) {
-                issue_refs(base0 + 2 * UP, r2);
-                issue_data(base0 + UP, r1, d);
-                compute(st, base0);
+            for (uint32_t lb = 0; lb < len; lb += UP) {
+                issue_refs(lb + 2 * UP, r2);
+                issue_data(lb + UP, r1, d);
+                compute(st, lb);
                 settle(d, r1, st);
 #pragma unroll
                 for (int g = 0; g < GT; g++)
@@ -945,35 +964,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     for (int q = 0; q < U; q++) r1[g][q] = r2[g][q];
             }
         } else {
-            for (int64_t base0 = start; base0 < end; base0 += UP) {
+            for (uint32_t lb = 0; lb < len; lb += UP) {
                 Step st;
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     st.sum[q] = (T)0;
-                    st.uo[q] = p.target[clampi(base0, q) * K + c];
+                    st.uo[q] = ldo(ub, loc(lb, q) * K + c);
                     st.cs[q] = st.cd[q] = (T)0;
                     st.rs[q] = st.rd[q] = 0;
                 }
-                load_words(base0, st.hw);
+                load_words(lb, st.hw);
                 T sw[U];
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     sw[q] = (T)1;
                     if constexpr (FACT) {
-                        if (p.sqrtw) sw[q] = p.sqrtw[clampi(base0, q)];  // wave-uniform test
+                        if (swb) sw[q] = ldo(swb, loc(lb, q));  // wave-uniform test
                     }
                 }
                 for (int g = 0; g < p.G; g++) {
                     int32_t r[U];
                     T cv[U];
+                    const int32_t *rb = p.refs[g] + start;
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        r[q] = p.refs[g][clampi(base0, q)];
-                        cv[q] = FACT ? (T)1 : p.cache[g][clampi(base0, q)];
+                        r[q] = ldo(rb, loc(lb, q));
+                        cv[q] = FACT ? (T)1 : ldo(p.cache[g] + start, loc(lb, q));
                     }
-                    const T *cf = (FACT ? p.vt : p.coef) + p.coef_off[g] * K + c;
+                    const T *cf = (FACT ? p.vt : p.coef) + p.coef_off[g] * K;
 #pragma unroll
-                    for (int q = 0; q < U; q++) st.sum[q] += FACT ? cf[(int64_t)r[q] * K] : cv[q] * cf[(int64_t)r[q] * K];
+                    for (int q = 0; q < U; q++) {
+                        const T gv = ldo(cf, (uint32_t)r[q] * K + c);
+                        st.sum[q] += FACT ? gv : cv[q] * gv;
+                    }
                     if (g == p.s) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
@@ -993,7 +1016,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                     for (int q = 0; q < U; q++) st.sum[q] = sw[q] * st.sum[q];
                 }
-                compute(st, base0);
+                compute(st, lb);
             }
         }
         if (cur.open && on && psub == 0) {
