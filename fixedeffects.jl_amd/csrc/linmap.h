@@ -764,8 +764,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (*p.all_done) return;
     extern __shared__ __align__(16) unsigned char fused_smem[];
     constexpr int P = kWave / K;
-    // wave steps per loop step; FP64 and batched columns take fewer to stay
-    // within 128 VGPRs (4 waves per SIMD) without spills
+    // wave steps per loop step (their loads are in flight together); FP64
+    // and batched columns take fewer to stay within 128 VGPRs (4 waves per
+    // SIMD) without spills (measured: U = 4 is slower for FP64 even without
+    // spills)
     constexpr int U = (sizeof(T) == 8 || K > 1) ? FUSED_U64 : 4;
     constexpr uint32_t UP = U * P;
     // head words one loop step touches: steps q use word (q*P)/64 and the next
