@@ -99,6 +99,23 @@ template <int CTRL, int RM>
 __device__ __forceinline__ F32x2 dpp_move(F32x2 v) {
     return F32x2(dpp_move<CTRL, RM>(v.hi), dpp_move<CTRL, RM>(v.lo));
 }
+// Swap of the wave's 32-lane halves (gfx950 v_permlane32_swap, a VALU op):
+// every lane gets the value of lane (l + 32) mod 64.
+__device__ __forceinline__ int half_swap_i32(int x) {
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (threadIdx.x & 32) ? (int)r[0] : (int)r[1];
+}
+__device__ __forceinline__ double half_swap(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)half_swap_i32((int)(unsigned)u);
+    const unsigned hi = (unsigned)half_swap_i32((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float half_swap(float v) {
+    return __builtin_bit_cast(float, half_swap_i32(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ F32x2 half_swap(F32x2 v) { return F32x2(half_swap(v.hi), half_swap(v.lo)); }
+
 // value of lane l, uniform across the wave
 __device__ __forceinline__ double lane_value(double v, int l) {
     const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
@@ -133,6 +150,12 @@ __device__ __forceinline__ A seg_scan(A v, int psub, int rs) {
         if ((psub & 16) && rs < (psub & ~15)) v += up;
         up = dpp_move<0x143, 0xC>(v);  // rows 2 and 3 read lane 31
         if (psub >= 32 && rs < 32) v += up;
+        return v;
+    } else if constexpr (K == 32) {
+        // two positions per step: the upper half adds the lower half's value
+        // unless its own position starts a segment
+        const A up = half_swap(v);
+        if (psub == 1 && rs == 0) v += up;
         return v;
     } else {
         constexpr int P = kWave / K;
@@ -467,8 +490,14 @@ __device__ __forceinline__ bool seg_step(uint64_t w, uint64_t wnext, int sh, boo
         else em = true;
     }
     Acc last;
-    if constexpr (K == 1) last = lane_value(x, kWave - 1);
-    else last = __shfl(x, (P - 1) * K + c, kWave);
+    if constexpr (K == 1) {
+        last = lane_value(x, kWave - 1);
+    } else if constexpr (K == 32) {
+        const Acc other = half_swap(x);
+        last = psub == 0 ? other : x;  // lane 32 + c for every lane of column c
+    } else {
+        last = __shfl(x, (P - 1) * K + c, kWave);
+    }
     cur.open = (nb == 0);
     cur.carry = cur.open ? last : Acc(0.f);
     cur.cont = cur.cont && (hb == 0);
