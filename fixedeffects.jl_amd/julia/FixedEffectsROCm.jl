@@ -143,4 +143,66 @@ function FixedEffects.solve_coefficients!(r::AbstractVector, feM::FixedEffectSol
     return FixedEffects.full(FixedEffects.normalize!(x, feM.fes), feM.fes), Int(iters[]), conv[] != 0
 end
 
+# Schedule autotuner (no reference counterpart): times candidate tile
+# schedules on this solver's data for K columns and keeps the fastest.
+function autotune!(feM::FixedEffectSolverROCm, K::Integer = 1; reps::Integer = 10)
+    out = zeros(Float64, 6)
+    check(ccall((:fehip_autotune, libfehip), Cint, (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}),
+                feM.handle, Int32(K), Int32(reps), out))
+    return (tiles_s = Int(out[1]), tiles_c = Int(out[2]), fuse_dense = Int(out[3]),
+            ms_per_iter = out[5], default_ms_per_iter = out[6])
+end
+
+# Many panels at once (multi-panel form of the multi-column solve_residuals!):
+# xss[i] holds the columns of panel i (host vectors), residualized in place;
+# the columns are staged in device memory (HIP runtime, same library libfehip
+# links) and all panels' LSMR iterations run concurrently on the GPU.
+const libhip = "libamdhip64.so.7"
+
+function solve_residuals_panels!(xss::Vector, feMs::Vector{<:FixedEffectSolverROCm};
+                                 tol::Real = 1e-8, maxiter::Integer = 10_000)
+    length(xss) == length(feMs) || error("one column set per solver")
+    P = length(feMs)
+    Ks = Int32[length(xs) for xs in xss]
+    lds = Int64[feM.N for feM in feMs]
+    dbufs = [Ptr{Cvoid}(C_NULL) for _ in 1:P]
+    iters = [zeros(Int64, k) for k in Ks]
+    convs = [zeros(UInt8, k) for k in Ks]
+    try
+        for i in 1:P
+            bytes = sizeof(Float64) * lds[i] * Ks[i]
+            ref = Ref{Ptr{Cvoid}}(C_NULL)
+            ccall((:hipMalloc, libhip), Cint, (Ref{Ptr{Cvoid}}, Csize_t), ref, bytes) == 0 || error("hipMalloc failed")
+            dbufs[i] = ref[]
+            for (c, x) in enumerate(xss[i])
+                col = convert(Vector{Float64}, x)
+                ccall((:hipMemcpy, libhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Csize_t, Cint),
+                      dbufs[i] + sizeof(Float64) * lds[i] * (c - 1), col, sizeof(Float64) * lds[i], 1) == 0 ||
+                    error("hipMemcpy failed")
+            end
+        end
+        handles = [feM.handle for feM in feMs]
+        GC.@preserve iters convs begin
+            ip = [pointer(a) for a in iters]
+            cp = [pointer(a) for a in convs]
+            check(ccall((:fehip_solve_residuals_panels, libfehip), Cint,
+                        (Ptr{Ptr{Cvoid}}, Int32, Ptr{Ptr{Cvoid}}, Ptr{Int32}, Ptr{Int64}, Float64, Int64,
+                         Ptr{Ptr{Int64}}, Ptr{Ptr{UInt8}}),
+                        handles, Int32(P), dbufs, Ks, lds, tol, maxiter, ip, cp))
+        end
+        for i in 1:P, (c, x) in enumerate(xss[i])
+            col = Vector{Float64}(undef, lds[i])
+            ccall((:hipMemcpy, libhip), Cint, (Ptr{Float64}, Ptr{Cvoid}, Csize_t, Cint),
+                  col, dbufs[i] + sizeof(Float64) * lds[i] * (c - 1), sizeof(Float64) * lds[i], 2) == 0 ||
+                error("hipMemcpy failed")
+            copyto!(x, col)
+        end
+    finally
+        for d in dbufs
+            d == C_NULL || ccall((:hipFree, libhip), Cint, (Ptr{Cvoid},), d)
+        end
+    end
+    return xss, iters, [Bool.(c) for c in convs]
+end
+
 end # module
