@@ -448,3 +448,44 @@ def test_panels_match_single_solves(gpu):
     finally:
         for s in solvers:
             s.close()
+
+
+def test_solve_more_than_32_columns(gpu):
+    # 40 columns run as a batch of 32 and a batch of 8; every column matches
+    # the oracle at a tight tolerance
+    _say("case: 40 columns")
+    p = synth.config("worker_firm_1e7_k32", scale=0.005)
+    rng = np.random.default_rng(3)
+    Y = np.concatenate([p.Y, p.Y[:, :8] + rng.standard_normal((p.N, 8))], axis=1)
+    worst, ig, io = _check_solve(p.refs, p.n, p.interactions, p.weights, Y, tol=1e-12, iter_slack=10)
+    _say(f"  worst rel err {worst:.2e}")
+    assert worst <= RES_RTOL
+
+
+def test_panels_of_mixed_precisions(gpu):
+    # panels whose handles run FP64, FP32 and mixed precision in one call give
+    # the residuals of solving each alone
+    _say("case: panels of several precisions")
+    from fixedeffects_rocm import device
+    panels = [synth.uneven_panel(900 + i, 1e4, 4e4) for i in range(3)]
+    precs = ["fp64", "fp32", "mixed"]
+    solvers = [api.FixedEffectSolverROCm([api.FixedEffect.from_refs(r, k, z) for r, k, z in
+                                          zip(p.refs, p.n, p.interactions)], p.weights, pr)
+               for p, pr in zip(panels, precs)]
+    try:
+        single, bufs = [], []
+        for p, s in zip(panels, solvers):
+            b = device.DeviceBuffer(p.N * p.K)
+            b.upload(np.asfortranarray(p.Y).ravel(order="F"))
+            s.solve_residuals_device(b.address, p.K, p.N, tol=1e-8, maxiter=10000)
+            single.append(b.download(p.N * p.K))
+            b.upload(np.asfortranarray(p.Y).ravel(order="F"))
+            bufs.append(b)
+        its, cvs = api.solve_residuals_panels(solvers, [b.address for b in bufs], [p.K for p in panels],
+                                              [p.N for p in panels], tol=1e-8, maxiter=10000)
+        for i, (p, b) in enumerate(zip(panels, bufs)):
+            _say(f"  panel {i} ({precs[i]}): K={p.K} iters {its[i]}")
+            np.testing.assert_array_equal(b.download(p.N * p.K), single[i])
+    finally:
+        for s in solvers:
+            s.close()
