@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/fehip.h"
+#include "components.h"
 #include "csr_build.h"
 #include "fehip_common.h"
 #include "linmap.h"
@@ -133,6 +134,13 @@ struct fehip_solver {
     double *ws_x0 = nullptr;     // p x ws_x0_cols (the network's starts)
     int ws_x0_cols = 0;
     int t_sec = -1;              // FE giving the secondary order key (-1: none)
+    // connected components for normalize! (lazy, see ensure_components)
+    bool cc_ready = false;
+    int cc_nu = 0;                // FEs without interaction
+    std::vector<int> cc_role;     // per FE: -1 not rescaled, else position among them
+    int32_t *cc_nodes = nullptr;  // p
+    int32_t *cc_start = nullptr;  // p + 2
+    uint32_t *cc_ncomp = nullptr; // 1
 };
 
 template <typename P>
@@ -2075,6 +2083,111 @@ static int solve_residuals_dev(fehip_solver *h, double *dY, int Kreal, int64_t l
     });
 }
 
+// LSMR from zero, coefficients unscaled into dc (p doubles, device).
+static int solve_coefficients_dev(fehip_solver *h, double *dy, double tol, int64_t maxiter, double *dc,
+                                  int64_t *iters, uint8_t *converged) {
+    if (h->precision == FEHIP_MIXED)
+        return for_K_mixed(1, [&](auto e) {
+            return decltype(e)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged);
+        });
+    return dispatch(h->precision, 1, [&](auto e) {
+        return decltype(e)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// normalize! / full on the device (src/FixedEffect.jl:106-140)
+// ---------------------------------------------------------------------------
+
+// Connected components of the FEs without interaction (normalize! only
+// rescales those, :107-109), built once per handle: they depend on the refs
+// alone.  Leaves cc_nodes (levels sorted by component, unobserved last),
+// cc_start (component offsets into cc_nodes) and cc_ncomp (count, device).
+static int ensure_components(fehip_solver *h) {
+    if (h->cc_ready) return 0;
+    std::vector<int> idx;
+    for (int g = 0; g < h->G; g++)
+        if (!h->fe[g].z_orig) idx.push_back(g);
+    if (idx.size() < 2) {
+        h->cc_nu = (int)idx.size();
+        h->cc_ready = true;
+        return 0;
+    }
+    const int64_t p = h->p;
+    if (p >= (int64_t)INT32_MAX - 2) return fail(FEHIP_EARG, "too many levels for the component labels");
+    int32_t *parent = nullptr, *key = nullptr, *skeys = nullptr;
+    uint8_t *seen = nullptr;
+    uint32_t *flag = nullptr, *rid = nullptr, *sums = nullptr;
+    CHK(dalloc(h, &parent, p));
+    CHK(dalloc(h, &seen, p));
+    CHK(dalloc(h, &flag, p));
+    CHK(dalloc(h, &rid, p + 1));
+    CHK(dalloc(h, &sums, (p + kSortTile - 1) / kSortTile + 2));
+    CHK(dalloc(h, &key, p));
+    CHK(dalloc(h, &skeys, p));
+    CHK(dalloc(h, &h->cc_nodes, p));
+    CHK(dalloc(h, &h->cc_start, p + 2));
+    CHK(dalloc(h, &h->cc_ncomp, 1));
+    hipLaunchKernelGGL(k_cc_init, dim3(grid_for(p, kBlock)), dim3(kBlock), 0, h->stream, parent, seen, p);
+    CcParams prm{};
+    for (size_t j = 0; j < idx.size(); j++) {
+        prm.refs[j] = h->fe[idx[j]].ref_int;
+        prm.node0[j] = h->off[idx[j]];
+    }
+    prm.nu = (int32_t)idx.size();
+    hipLaunchKernelGGL(k_cc_hook, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, prm, h->N, parent,
+                       seen);
+    hipLaunchKernelGGL(k_cc_roots, dim3(grid_for(p, kBlock)), dim3(kBlock), 0, h->stream, parent,
+                       (const uint8_t *)seen, p, flag);
+    HIPCHK(hipGetLastError());
+    CHK(exclusive_scan_u32(h, flag, p, rid, true, sums));
+    hipLaunchKernelGGL(k_cc_label, dim3(grid_for(p, kBlock)), dim3(kBlock), 0, h->stream, parent,
+                       (const uint8_t *)seen, (const uint32_t *)rid, p, key);
+    HIPCHK(hipGetLastError());
+    CHK(radix_sort(h, key, p, p + 1, skeys, h->cc_nodes));
+    hipLaunchKernelGGL(k_cc_starts, dim3(grid_for(p + 1, kBlock)), dim3(kBlock), 0, h->stream,
+                       (const int32_t *)skeys, p, h->cc_start);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h->cc_ncomp, rid + p, sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, parent);
+    dfree(h, seen);
+    dfree(h, flag);
+    dfree(h, rid);
+    dfree(h, sums);
+    dfree(h, key);
+    dfree(h, skeys);
+    h->cc_nu = (int)idx.size();
+    h->cc_role.assign(h->G, -1);
+    for (size_t j = 0; j < idx.size(); j++) h->cc_role[idx[j]] = (int)j;
+    h->cc_ready = true;
+    return 0;
+}
+
+// normalize! (:106-111) of the concatenated coefficients dc (p doubles, device).
+static int normalize_dev(fehip_solver *h, double *dc) {
+    CHK(ensure_components(h));
+    if (h->cc_nu < 2) return 0;
+    RescaleParams prm{};
+    for (int g = 0; g <= h->G; g++) prm.off[g] = h->off[g];
+    for (int g = 0; g < h->G; g++) prm.role[g] = h->cc_role[g];
+    prm.G = h->G;
+    prm.nu = h->cc_nu;
+    const unsigned grid = (unsigned)std::min<int64_t>(h->p, 1024);
+    hipLaunchKernelGGL(k_rescale, dim3(grid), dim3(kBlock), 0, h->stream, prm, (const int32_t *)h->cc_nodes,
+                       (const int32_t *)h->cc_start, (const uint32_t *)h->cc_ncomp, dc);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// full (:137-140) of FE g into out (N doubles, device, caller's order).
+static int full_dev(fehip_solver *h, const double *dc, int g, double *out) {
+    hipLaunchKernelGGL(k_full, dim3(grid_for(h->N, kBlock)), dim3(kBlock), 0, h->stream, dc + h->off[g],
+                       (const int32_t *)h->fe[g].ref_int, (const int32_t *)h->perm_s, h->N, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
@@ -2237,19 +2350,79 @@ int fehip_solve_coefficients(fehip_t *h, const double *y, double tol, int64_t ma
         hipError_t e = hipMemcpyAsync(dy, y, sizeof(double) * h->N, hipMemcpyHostToDevice, h->stream);
         if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
     }
-    if (rc == 0 && h->precision == FEHIP_MIXED)
-        rc = for_K_mixed(1, [&](auto e) {
-            return decltype(e)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged);
-        });
-    else if (rc == 0)
-        rc = dispatch(h->precision, 1, [&](auto e) {
-            return decltype(e)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged);
-        });
+    if (rc == 0) rc = solve_coefficients_dev(h, dy, tol, maxiter, dc, iters, converged);
     for (int g = 0; g < h->G && rc == 0; g++) {
         hipError_t e = hipMemcpy(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
     }
     dfree(h, dy);
+    dfree(h, dc);
+    return rc;
+}
+
+int fehip_solve_coefficients_full(fehip_t *h, const double *y, double tol, int64_t maxiter, double *const *coef,
+                                  double *const *obs, int64_t *iters, uint8_t *converged) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (!y) return fail(FEHIP_EARG, "y is required");
+    if (maxiter < 0) return fail(FEHIP_EARG, "maxiter must be >= 0");
+    double *dy = nullptr, *dc = nullptr;
+    CHK(dalloc(h, &dy, h->N));
+    CHK(dalloc(h, &dc, h->p));
+    int rc = 0;
+    {
+        hipError_t e = hipMemcpyAsync(dy, y, sizeof(double) * h->N, hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    if (rc == 0) rc = solve_coefficients_dev(h, dy, tol, maxiter, dc, iters, converged);
+    if (rc == 0) rc = normalize_dev(h, dc);
+    for (int g = 0; coef && g < h->G && rc == 0; g++) {
+        if (!coef[g]) continue;
+        hipError_t e = hipMemcpyAsync(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost,
+                                      h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    // dy is free once the solve has read it: it holds one FE's expansion at a time
+    for (int g = 0; obs && g < h->G && rc == 0; g++) {
+        if (!obs[g]) continue;
+        rc = full_dev(h, dc, g, dy);
+        if (rc) break;
+        hipError_t e = hipMemcpyAsync(obs[g], dy, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    if (rc == 0) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    dfree(h, dy);
+    dfree(h, dc);
+    return rc;
+}
+
+int fehip_normalize(fehip_t *h, double *const *coef) {
+    g_err.clear();
+    if (!h || !coef) return fail(FEHIP_EARG, "bad arguments");
+    for (int g = 0; g < h->G; g++)
+        if (!coef[g]) return fail(FEHIP_EARG, "coef[g] is required for every fixed effect");
+    double *dc = nullptr;
+    CHK(dalloc(h, &dc, h->p));
+    int rc = 0;
+    for (int g = 0; g < h->G && rc == 0; g++) {
+        hipError_t e = hipMemcpyAsync(dc + h->off[g], coef[g], sizeof(double) * h->n[g], hipMemcpyHostToDevice,
+                                      h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    if (rc == 0) rc = normalize_dev(h, dc);
+    for (int g = 0; g < h->G && rc == 0; g++) {
+        hipError_t e = hipMemcpyAsync(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost,
+                                      h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    if (rc == 0) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
     dfree(h, dc);
     return rc;
 }

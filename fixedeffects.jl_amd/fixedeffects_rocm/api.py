@@ -225,10 +225,38 @@ class FixedEffectSolverROCm:
 
     def solve_coefficients(self, y, tol: Optional[float] = None, maxiter: int = 100_000):
         """Inner solve_coefficients!(r, feM; tol, maxiter) (:147-160): returns
-        (G per-observation coefficient vectors, div(iterations, 2), converged)."""
-        blocks, it, cv = self.solve_coefficients_raw(y, tol, maxiter)
-        blocks = normalize(blocks, self.fes)
-        return full(blocks, self.fes), it, cv
+        (G per-observation coefficient vectors, div(iterations, 2), converged).
+        normalize! and full run on the device (fehip_solve_coefficients_full)."""
+        obs, _, it, cv = self.solve_coefficients_full(y, tol, maxiter, levels=False)
+        return obs, it, cv
+
+    def solve_coefficients_full(self, y, tol: Optional[float] = None, maxiter: int = 100_000, levels: bool = True,
+                                per_obs: bool = True):
+        """(per-observation vectors or None, normalized level blocks or None,
+        div(iterations, 2), converged)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if len(y) != self.N:
+            raise ValueError("FixedEffects must have the same length as y")
+        blocks = [np.empty(int(k), np.float64) for k in self.n] if levels else None
+        obs = [np.empty(self.N, np.float64) for _ in range(self.G)] if per_obs else None
+        barr = (_dp * self.G)(*[_ptr(b, C.c_double) for b in blocks]) if levels else None
+        oarr = (_dp * self.G)(*[_ptr(o, C.c_double) for o in obs]) if per_obs else None
+        it = C.c_int64()
+        cv = C.c_uint8()
+        check(lib.fehip_solve_coefficients_full(self._h, _ptr(y, C.c_double), self._default_tol(tol), int(maxiter),
+                                                barr, oarr, C.byref(it), C.byref(cv)))
+        return obs, blocks, int(it.value), bool(cv.value)
+
+    def normalize(self, blocks: List[np.ndarray]) -> List[np.ndarray]:
+        """normalize! of G level blocks in place, on the device."""
+        if len(blocks) != self.G or any(len(b) != k for b, k in zip(blocks, self.n)):
+            raise ValueError("one block of nlevels doubles per fixed effect is required")
+        for b in blocks:
+            if b.dtype != np.float64 or not b.flags.c_contiguous:
+                raise TypeError("blocks must be contiguous float64 arrays")
+        arr = (_dp * self.G)(*[_ptr(b, C.c_double) for b in blocks])
+        check(lib.fehip_normalize(self._h, arr))
+        return blocks
 
     def solve_coefficients_raw(self, y, tol: Optional[float] = None, maxiter: int = 100_000):
         """LSMR from zero, x .*= scale (:149-158), before normalize!/full."""

@@ -127,12 +127,27 @@ function _solve_many!(cols::Vector, feM::FixedEffectSolverROCm, tol, maxiter)
 end
 
 # inner solve_coefficients!(r, feM; tol, maxiter)  (src/AbstractFixedEffectSolver.jl:147-160)
+# The coefficients are collected into eltype(r) before normalize! (:158-159):
+# for Float64 callers the solve, normalize! and full all run on the device
+# (fehip_solve_coefficients_full); other element types take the raw
+# coefficients and the reference's own normalize!/full in that type.
 function FixedEffects.solve_coefficients!(r::AbstractVector, feM::FixedEffectSolverROCm{T};
                                           tol::Real = sqrt(eps(T)), maxiter::Integer = 100_000) where {T}
     y = convert(Vector{Float64}, r)
-    coefs = [zeros(Float64, fe.n) for fe in feM.fes]
     iters = Ref{Int64}(0)
     conv = Ref{UInt8}(0)
+    if eltype(r) === Float64
+        obs = [zeros(Float64, length(y)) for _ in feM.fes]
+        GC.@preserve obs begin
+            optrs = Ptr{Float64}[pointer(o) for o in obs]
+            check(ccall((:fehip_solve_coefficients_full, libfehip), Cint,
+                        (Ptr{Cvoid}, Ptr{Float64}, Float64, Int64, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Int64},
+                         Ptr{UInt8}),
+                        feM.handle, y, tol, maxiter, C_NULL, optrs, iters, conv))
+        end
+        return obs, Int(iters[]), conv[] != 0
+    end
+    coefs = [zeros(Float64, fe.n) for fe in feM.fes]
     GC.@preserve coefs begin
         ptrs = Ptr{Float64}[pointer(c) for c in coefs]
         check(ccall((:fehip_solve_coefficients, libfehip), Cint,

@@ -120,6 +120,23 @@ int fehip_solve_coefficients(fehip_t *h, const double *y, double tol,
                              int64_t maxiter, double *const *coef,
                              int64_t *iters, uint8_t *converged);
 
+/* The whole inner solve_coefficients!(r, feM) (src/AbstractFixedEffectSolver.jl
+ * :147-160) on the device: the solve above, then normalize! (src/FixedEffect.jl
+ * :106-135: per connected component of the FEs without interaction, every FE
+ * but the first demeaned, the means added to the first) and full (:137-140).
+ *   coef : optional, G host arrays of nlevels[g] doubles (normalized)
+ *   obs  : optional, G host arrays of N doubles, obs[g][i] = coef[g][refs[g][i]]
+ *          (the reference's return value); NULL entries are skipped
+ * Components are computed once per handle (they depend on the refs only). */
+int fehip_solve_coefficients_full(fehip_t *h, const double *y, double tol,
+                                  int64_t maxiter, double *const *coef,
+                                  double *const *obs, int64_t *iters,
+                                  uint8_t *converged);
+
+/* normalize!(fecoefs, fes) (src/FixedEffect.jl:106-111) of G host arrays in
+ * place, for the handle's fixed effects. */
+int fehip_normalize(fehip_t *h, double *const *coef);
+
 /* Initial iterate (no reference counterpart as an argument; the reference's
  * lsmr! takes it through its x argument, src/utils/lsmr.jl:42-43, and
  * solve_residuals! always passes x0 = A'b, src/AbstractFixedEffectSolver.jl:56).
