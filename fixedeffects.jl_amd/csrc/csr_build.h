@@ -18,7 +18,9 @@ constexpr int kSortItems = 16;                    // items per thread per tile
 constexpr int kSortTile = kBlock * kSortItems;    // 4096 keys per workgroup
 
 // Per-tile digit histogram, stored digit-major: hist[d * ntiles + tile].
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const int32_t *__restrict__ keys, int64_t n,
+// KT: int32_t (non-negative level keys) or uint64_t (grouping keys).
+template <typename KT>
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const KT *__restrict__ keys, int64_t n,
                                                        int shift, uint32_t *__restrict__ hist,
                                                        int64_t ntiles) {
     __shared__ uint32_t h[256];
@@ -27,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const int32_t *__restrict
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
     for (int r = 0; r < kSortItems; r++) {
         int64_t i = base + (int64_t)r * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+        if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255)], 1u);
     }
     __syncthreads();
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -39,9 +41,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const int32_t *__restrict
 // the digit count of earlier rounds + the count in lower waves of this round
 // + its rank among same-digit lanes of its wave (ballot peer mask).
 // vals_in == nullptr means the identity (element index).
+template <typename KT>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
-    const int32_t *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
-    int32_t *__restrict__ keys_out, int32_t *__restrict__ vals_out, int64_t n, int shift,
+    const KT *__restrict__ keys_in, const int32_t *__restrict__ vals_in,
+    KT *__restrict__ keys_out, int32_t *__restrict__ vals_out, int64_t n, int shift,
     const uint32_t *__restrict__ scanned, int64_t ntiles) {
     __shared__ uint32_t running[256];
     __shared__ uint32_t wcnt[kWavesPerBlock][256];
@@ -55,9 +58,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     for (int r = 0; r < kSortItems; r++) {
         const int64_t i = base + (int64_t)r * kBlock + tid;
         const bool valid = i < n;
-        const int32_t key = valid ? keys_in[i] : 0;
+        const KT key = valid ? keys_in[i] : (KT)0;
         const int32_t val = valid ? (vals_in ? vals_in[i] : (int32_t)i) : 0;
-        const int d = (key >> shift) & 255;
+        const int d = (int)((key >> shift) & 255);
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {

@@ -28,6 +28,7 @@
 #include "../../include/fehip.h"
 #include "components.h"
 #include "csr_build.h"
+#include "grouping.h"
 #include "fehip_common.h"
 #include "linmap.h"
 
@@ -238,10 +239,10 @@ static int radix_sort(fehip_solver *h, const int32_t *keys, int64_t N, int64_t n
         const bool lastp = ps == passes - 1;
         int32_t *kout = lastp ? keys_out : h->sk[ps & 1];
         int32_t *vout = lastp ? vals_out : h->sv[ps & 1];
-        hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, kin, N, 8 * ps,
+        hipLaunchKernelGGL(k_radix_hist<int32_t>, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, kin, N, 8 * ps,
                            h->hist, ntiles);
         CHK(exclusive_scan_u32(h, h->hist, 256 * ntiles, h->hist, false, h->sums));
-        hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, kin, vin, kout,
+        hipLaunchKernelGGL(k_radix_scatter<int32_t>, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, kin, vin, kout,
                            vout, N, 8 * ps, h->hist, ntiles);
         HIPCHK(hipGetLastError());
         kin = kout;
@@ -2189,6 +2190,110 @@ static int full_dev(fehip_solver *h, const double *dc, int g, double *out) {
 }
 
 // ---------------------------------------------------------------------------
+// grouping of raw keys (src/FixedEffect.jl:18-21)
+// ---------------------------------------------------------------------------
+
+// Number the distinct values of u (n unsigned keys < 2^bits) densely in
+// ascending order into dense (n int64); returns the count in *ngroups.
+// kb/vb: ping-pong key/value buffers of n entries; hist/sums/flag/ex: scratch.
+struct GroupScratch {
+    uint64_t *kb[2] = {nullptr, nullptr};
+    int32_t *vb[2] = {nullptr, nullptr};
+    uint32_t *hist = nullptr, *sums = nullptr, *flag = nullptr, *ex = nullptr, *sums2 = nullptr;
+    int64_t ntiles = 0;
+};
+
+static int number_keys(fehip_solver *h, GroupScratch &gs, const uint64_t *u, int64_t n, int bits, int64_t *dense,
+                       int64_t *ngroups) {
+    const int passes = std::max(1, (bits + 7) / 8);
+    const uint64_t *kin = u;
+    const int32_t *vin = nullptr;
+    int buf = 0;
+    for (int ps = 0; ps < passes; ps++) {
+        uint64_t *kout = gs.kb[buf];
+        int32_t *vout = gs.vb[buf];
+        hipLaunchKernelGGL(k_radix_hist<uint64_t>, dim3((unsigned)gs.ntiles), dim3(kBlock), 0, h->stream, kin, n,
+                           8 * ps, gs.hist, gs.ntiles);
+        CHK(exclusive_scan_u32(h, gs.hist, 256 * gs.ntiles, gs.hist, false, gs.sums));
+        hipLaunchKernelGGL(k_radix_scatter<uint64_t>, dim3((unsigned)gs.ntiles), dim3(kBlock), 0, h->stream, kin,
+                           vin, kout, vout, n, 8 * ps, gs.hist, gs.ntiles);
+        HIPCHK(hipGetLastError());
+        kin = kout;
+        vin = vout;
+        buf ^= 1;
+    }
+    hipLaunchKernelGGL(k_group_heads, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, h->stream, kin, n, gs.flag);
+    CHK(exclusive_scan_u32(h, gs.flag, n, gs.ex, true, gs.sums2));
+    hipLaunchKernelGGL(k_group_assign, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, h->stream,
+                       (const uint32_t *)gs.ex, (const uint32_t *)gs.flag, vin, n, dense);
+    HIPCHK(hipGetLastError());
+    uint32_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, gs.ex + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *ngroups = total;
+    return 0;
+}
+
+static int bits_of(uint64_t range) {
+    int b = 0;
+    while (b < 64 && (range >> b) != 0) b++;
+    return b;
+}
+
+static int group_keys(fehip_solver *h, int64_t N, int32_t ncols, const int64_t *const *keys, int64_t *refs,
+                      int64_t *ngroups) {
+    GroupScratch gs;
+    gs.ntiles = (N + kSortTile - 1) / kSortTile;
+    for (int b = 0; b < 2; b++) {
+        CHK(dalloc(h, &gs.kb[b], N));
+        CHK(dalloc(h, &gs.vb[b], N));
+    }
+    CHK(dalloc(h, &gs.hist, 256 * gs.ntiles + 1));
+    CHK(dalloc(h, &gs.sums, (256 * gs.ntiles + kSortTile - 1) / kSortTile + 2));
+    CHK(dalloc(h, &gs.flag, N));
+    CHK(dalloc(h, &gs.ex, N + 1));
+    CHK(dalloc(h, &gs.sums2, (N + kSortTile - 1) / kSortTile + 2));
+    int64_t *dkey = nullptr, *dense = nullptr, *dcol = nullptr;
+    uint64_t *u = nullptr;
+    long long *mm = nullptr;
+    CHK(dalloc(h, &dkey, N));
+    CHK(dalloc(h, &dense, N));
+    CHK(dalloc(h, &dcol, N));
+    CHK(dalloc(h, &u, N));
+    CHK(dalloc(h, &mm, 2));
+    int64_t n_prev = 0;
+    for (int c = 0; c < ncols; c++) {
+        HIPCHK(hipMemcpyAsync(dkey, keys[c], sizeof(int64_t) * N, hipMemcpyHostToDevice, h->stream));
+        const long long init[2] = {INT64_MAX, INT64_MIN};
+        HIPCHK(hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_minmax_i64, dim3((unsigned)std::min<int64_t>(grid_for(N, kBlock), 4096)), dim3(kBlock),
+                           0, h->stream, (const int64_t *)dkey, N, mm);
+        long long hm[2];
+        HIPCHK(hipMemcpyAsync(hm, mm, sizeof(hm), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        hipLaunchKernelGGL(k_key_offset, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
+                           (const int64_t *)dkey, N, (int64_t)hm[0], u);
+        int64_t nc = 0;
+        CHK(number_keys(h, gs, u, N, bits_of((uint64_t)hm[1] - (uint64_t)hm[0]), c == 0 ? dense : dcol, &nc));
+        if (c == 0) {
+            n_prev = nc;
+            continue;
+        }
+        // combinations of distinct values: n_prev * nc <= N * N < 2^63
+        hipLaunchKernelGGL(k_group_combine, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
+                           (const int64_t *)dense, (const int64_t *)dcol, N, nc, u);
+        CHK(number_keys(h, gs, u, N, bits_of((uint64_t)(n_prev * nc - 1)), dense, &n_prev));
+    }
+    hipLaunchKernelGGL(k_plus_one, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, (const int64_t *)dense, N,
+                       dkey);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(refs, dkey, sizeof(int64_t) * N, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *ngroups = n_prev;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -2397,6 +2502,23 @@ int fehip_solve_coefficients_full(fehip_t *h, const double *y, double tol, int64
     }
     dfree(h, dy);
     dfree(h, dc);
+    return rc;
+}
+
+int fehip_group(int64_t N, int32_t ncols, const int64_t *const *keys, int64_t *refs, int64_t *ngroups) {
+    g_err.clear();
+    if (N < 0 || ncols < 1 || !keys || !refs || !ngroups) return fail(FEHIP_EARG, "bad arguments");
+    for (int c = 0; c < ncols; c++)
+        if (!keys[c]) return fail(FEHIP_EARG, "keys[c] is required for every column");
+    if (N == 0) {
+        *ngroups = 0;
+        return 0;
+    }
+    if (N >= (int64_t)INT32_MAX) return fail(FEHIP_EARG, "too many observations for one grouping call");
+    fehip_solver *h = new fehip_solver();
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    int rc = e == hipSuccess ? group_keys(h, N, ncols, keys, refs, ngroups) : fail(FEHIP_EHIP, hipGetErrorString(e));
+    fehip_destroy(h);
     return rc;
 }
 

@@ -45,6 +45,7 @@ EXPORTS = (
     "fehip_solve_coefficients",
     "fehip_solve_coefficients_full",
     "fehip_normalize",
+    "fehip_group",
     "fehip_get_csr",
     "fehip_get_scale",
     "fehip_get_cache",
@@ -94,6 +95,7 @@ def _load() -> C.CDLL:
     L.fehip_solve_coefficients_full.argtypes = [_vp, _dp, C.c_double, C.c_int64, C.POINTER(_dp), C.POINTER(_dp),
                                                 _i64p, _u8p]
     L.fehip_normalize.argtypes = [_vp, C.POINTER(_dp)]
+    L.fehip_group.argtypes = [C.c_int64, C.c_int32, C.POINTER(_i64p), _i64p, _i64p]
     L.fehip_get_csr.argtypes = [_vp, C.c_int32, _i32p, _i32p]
     L.fehip_get_scale.argtypes = [_vp, C.c_int32, _dp]
     L.fehip_get_cache.argtypes = [_vp, C.c_int32, _dp]

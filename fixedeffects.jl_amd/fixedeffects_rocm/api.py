@@ -47,19 +47,25 @@ class FixedEffect:
     into dense 1-based refs (the reference uses GroupedArray with
     ``sort = nothing``, src/FixedEffect.jl:18-21; here levels are numbered in
     sorted key order -- level numbering does not change residuals, which is
-    what test/solve.jl:18-24 checks).  ``FixedEffect.from_refs(refs, n,
-    interaction)`` is the inner constructor (src/FixedEffect.jl:11-15).
+    what test/solve.jl:18-24 checks).  ``device=True`` numbers integer keys
+    on the GPU instead (``group``), with the same numbering.
+    ``FixedEffect.from_refs(refs, n, interaction)`` is the inner constructor
+    (src/FixedEffect.jl:11-15).
     """
 
     __slots__ = ("refs", "interaction", "n")
 
-    def __init__(self, *keys, interaction=None):
+    def __init__(self, *keys, interaction=None, device: bool = False):
         if len(keys) == 0:
             raise ValueError("FixedEffect needs at least one key vector")
         arrs = [np.asarray(k) for k in keys]
         N = len(arrs[0])
         if any(len(a) != N for a in arrs):
             raise DimensionMismatch("all key vectors must have the same length")
+        if device:
+            refs, n = group(*arrs)
+            self._set(refs, interaction, n)
+            return
         if len(arrs) == 1:
             _, inv = np.unique(arrs[0], return_inverse=True)
         else:
@@ -100,6 +106,30 @@ class FixedEffect:
 
     def __repr__(self) -> str:
         return "Fixed Effects"
+
+
+def group(*keys):
+    """Dense 1-based refs of the distinct combinations of integer key columns,
+    computed on the GPU (fehip_group; GroupedArray in src/FixedEffect.jl:18-21),
+    numbered in ascending lexicographic key order.  Returns (refs, n)."""
+    if len(keys) == 0:
+        raise ValueError("group needs at least one key vector")
+    cols = []
+    for k in keys:
+        a = np.asarray(k)
+        if a.dtype.kind not in "iub":
+            raise TypeError("device grouping takes integer keys")
+        if a.dtype.kind == "u" and a.dtype.itemsize == 8 and len(a) and a.max() > np.iinfo(np.int64).max:
+            raise TypeError("uint64 keys beyond the int64 range are not supported")
+        cols.append(np.ascontiguousarray(a, dtype=np.int64))
+    N = len(cols[0])
+    if any(len(c) != N for c in cols):
+        raise DimensionMismatch("all key vectors must have the same length")
+    refs = np.empty(N, np.int64)
+    ng = C.c_int64()
+    arr = (_i64p * len(cols))(*[_ptr(c, C.c_int64) for c in cols])
+    check(lib.fehip_group(N, len(cols), arr, _ptr(refs, C.c_int64), C.byref(ng)))
+    return refs, int(ng.value)
 
 
 # ---------------------------------------------------------------------------

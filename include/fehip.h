@@ -10,9 +10,11 @@
  *   - the scale / cache construction (src/SolverCPU.jl:67-95),
  *   - the LSMR loop (src/utils/lsmr.jl:46-222),
  *   - the inner solve_residuals! / solve_coefficients! methods
- *     (src/AbstractFixedEffectSolver.jl:49-68, :74-97, :147-160).
- * FixedEffect construction, normalize!/full and the outer keyword methods stay
- * with the caller.
+ *     (src/AbstractFixedEffectSolver.jl:49-68, :74-97, :147-160),
+ *   - normalize! / components / full (src/FixedEffect.jl:57-140) and the
+ *     grouping of raw integer keys (src/FixedEffect.jl:18-21), as optional
+ *     device steps.
+ * The outer keyword methods stay with the caller.
  *
  * Conventions
  *   - refs are 1-based, as FixedEffect.refs (src/FixedEffect.jl:7-21); the
@@ -113,7 +115,8 @@ int fehip_solve_residuals_panels(fehip_t *const *hs, int32_t npanels, double *co
  * (src/AbstractFixedEffectSolver.jl:147-160) up to the collect at :158:
  * LSMR from x0 = 0 with maxiter, then x .*= scale.  coef[g] receives the
  * nlevels[g] unnormalized coefficients of FE g; the caller applies
- * normalize! and full (src/FixedEffect.jl:106-140).
+ * normalize! and full (src/FixedEffect.jl:106-140), or calls
+ * fehip_solve_coefficients_full, which does both on the device.
  *   iters     : div(lsmr iterations, 2) (:159)
  *   converged : istop not in (3, 6, 7) */
 int fehip_solve_coefficients(fehip_t *h, const double *y, double tol,
@@ -132,6 +135,17 @@ int fehip_solve_coefficients_full(fehip_t *h, const double *y, double tol,
                                   int64_t maxiter, double *const *coef,
                                   double *const *obs, int64_t *iters,
                                   uint8_t *converged);
+
+/* FixedEffect(keys...) grouping (src/FixedEffect.jl:18-21, which calls
+ * GroupedArray(keys..., sort = nothing)): dense 1-based refs for the distinct
+ * combinations of ncols int64 key columns (keys[c]: N host values), numbered
+ * in ascending lexicographic key order (GroupedArray's numbering for
+ * sort = nothing is unspecified; residuals do not depend on it).
+ *   refs    : N host int64, out
+ *   ngroups : number of levels (the FixedEffect's n), out
+ * Needs no handle; N < 2^31. */
+int fehip_group(int64_t N, int32_t ncols, const int64_t *const *keys,
+                int64_t *refs, int64_t *ngroups);
 
 /* normalize!(fecoefs, fes) (src/FixedEffect.jl:106-111) of G host arrays in
  * place, for the handle's fixed effects. */
