@@ -19,7 +19,7 @@ module FixedEffectsROCm
 
 using FixedEffects
 using FixedEffects: AbstractFixedEffectSolver, FixedEffect
-using StatsBase: AbstractWeights, UnitWeights
+using StatsBase: AbstractWeights, UnitWeights, uweights
 
 const libfehip = get(ENV, "FEHIP_LIB", joinpath(@__DIR__, "..", "lib", "libfehip.so"))
 
@@ -218,6 +218,22 @@ function solve_residuals_panels!(xss::Vector, feMs::Vector{<:FixedEffectSolverRO
         end
     end
     return xss, iters, [Bool.(c) for c in convs]
+end
+
+# FixedEffect(keys...; interaction) with the grouping on the device
+# (src/FixedEffect.jl:18-21): integer key columns only; levels are numbered in
+# ascending lexicographic key order.
+function fixedeffect(keys::AbstractVector{<:Integer}...; interaction::AbstractVector = uweights(length(keys[1])))
+    cols = [convert(Vector{Int64}, k) for k in keys]
+    N = length(cols[1])
+    all(length(c) == N for c in cols) || throw(DimensionMismatch("all key vectors must have the same length"))
+    refs = Vector{Int64}(undef, N)
+    ng = Ref{Int64}(0)
+    GC.@preserve cols begin
+        check(ccall((:fehip_group, libfehip), Cint, (Int64, Int32, Ptr{Ptr{Int64}}, Ptr{Int64}, Ptr{Int64}),
+                    N, Int32(length(cols)), Ptr{Int64}[pointer(c) for c in cols], refs, ng))
+    end
+    return FixedEffect{typeof(refs), typeof(interaction)}(refs, interaction, Int(ng[]))
 end
 
 end # module
