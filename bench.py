@@ -131,7 +131,8 @@ def run_panels(args, precision):
         f"({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
     solvers = [api.FixedEffectSolverROCm([api.FixedEffect.from_refs(r, k, z) for r, k, z in
-                                          zip(p.refs, p.n, p.interactions)], p.weights, precision) for p in panels]
+                                          zip(p.refs, p.n, p.interactions)], p.weights, precision,
+                                         algorithm=args.algorithm) for p in panels]
     setup_s = time.perf_counter() - t0
     pristine, work = [], []
     for p in panels:
@@ -205,6 +206,8 @@ def main():
     ap.add_argument("--pcie-steps", type=int, default=2, help="timed solves through the host-buffer entry point")
     ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-sized default schedule")
     ap.add_argument("--panels", type=int, default=64, help="panels of the uneven_panels workload")
+    ap.add_argument("--algorithm", default="lsmr", choices=["lsmr", "projected"],
+                    help="lsmr: the reference's LSMR; projected: LSMR with the FE of most levels eliminated")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the solve is not sharded across GPUs "
@@ -227,7 +230,7 @@ def main():
     log(f"[bench] panel {name}: N={panel.N} n={panel.n} K={panel.K} ({time.perf_counter() - t0:.1f} s)")
     fes = [api.FixedEffect.from_refs(r, k, z) for r, k, z in zip(panel.refs, panel.n, panel.interactions)]
     t0 = time.perf_counter()
-    feM = api.FixedEffectSolverROCm(fes, panel.weights, args.precision)
+    feM = api.FixedEffectSolverROCm(fes, panel.weights, args.precision, algorithm=args.algorithm)
     setup_s = time.perf_counter() - t0
     log(f"[bench] solver set-up {setup_s:.2f} s")
     sched = None
@@ -317,8 +320,10 @@ def main():
         "data": "synthetic (seeded numpy PCG64 worker/firm/year panel)",
         "config": {"workload": name, "N": N, "levels": [int(k) for k in panel.n], "K": K, "tol": args.tol,
                    "weights": "LogNormal(0,0.5)", "precision": args.precision,
-                   "mode": "plain LSMR, reference start" if args.precision == "fp64"
-                   else "FP32-stream LSMR phase, then FP64 LSMR iterations from its solution"},
+                   "mode": ("plain LSMR, reference start" if args.precision == "fp64"
+                            else "FP32-stream LSMR phase, then FP64 LSMR iterations from its solution")
+                   + ("" if args.algorithm == "lsmr" else "; projected LSMR (FE of most levels eliminated)"),
+                   "algorithm": args.algorithm},
         "lsmr_iters": [int(i) - 1 for i in its],
         "normal_eq_note": "max_level |A'(sqrt(w) r)| / ||sqrt(w) y|| of column 0",
         "converged": bool(all(cvs)),

@@ -135,6 +135,10 @@ struct fehip_solver {
     double *ws_x0 = nullptr;     // p x ws_x0_cols (the network's starts)
     int ws_x0_cols = 0;
     int t_sec = -1;              // FE giving the secondary order key (-1: none)
+    // projected LSMR (FE s eliminated): requested by fehip_set_algorithm,
+    // in effect for the residual solves (fwl_now is set per call)
+    bool fwl_req = false, fwl_now = false;
+    void *scale2_T[2] = {nullptr, nullptr};  // FE s: scale_s^2 (factored form), per storage precision
     // connected components for normalize! (lazy, see ensure_components)
     bool cc_ready = false;
     int cc_nu = 0;                // FEs without interaction
@@ -142,6 +146,13 @@ struct fehip_solver {
     int32_t *cc_nodes = nullptr;  // p
     int32_t *cc_start = nullptr;  // p + 2
     uint32_t *cc_ncomp = nullptr; // 1
+};
+
+// the projected LSMR applies to the residual solves of a call (G >= 2)
+struct FwlScope {
+    fehip_solver *h;
+    explicit FwlScope(fehip_solver *hh) : h(hh) { h->fwl_now = h->fwl_req && h->G >= 2; }
+    ~FwlScope() { h->fwl_now = false; }
 };
 
 template <typename P>
@@ -753,6 +764,13 @@ struct Plan {
     double *dY = nullptr;  // staging for host columns (N x K doubles)
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
+    bool graph_fwl = false;
+    // projected iteration: FE s items (crossing segments, into zs from level
+    // 0), the other FEs' items, zs = the FE s correction (n_s x K)
+    FixItem *items_s = nullptr, *items_o = nullptr;
+    int32_t nitems_s = 0, nitems_o = 0;
+    void *zs = nullptr;
+    ColOps *ops_one = nullptr, *ops_neg = nullptr, *ops_sub = nullptr;
 };
 
 // waves_f / waves_c: how many waves of the fused pass / of the CSR D'u launch
@@ -893,6 +911,32 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     CHK(dalloc(h, &P->items, std::max<int32_t>(P->nitems, 1)));
     if (P->nitems)
         HIPCHK(hipMemcpyAsync(P->items, items.data(), sizeof(FixItem) * P->nitems, hipMemcpyHostToDevice, h->stream));
+    std::vector<FixItem> its, ito;  // alive until the synchronize below
+    {
+        for (const FixItem &fi : items) {
+            if (fi.fe != h->s) {
+                ito.push_back(fi);
+            } else if (fi.kind == 0) {
+                FixItem z = fi;
+                z.coef_off = 0;
+                its.push_back(z);
+            }
+        }
+        P->nitems_s = (int32_t)its.size();
+        P->nitems_o = (int32_t)ito.size();
+        CHK(dalloc(h, &P->items_s, std::max<int32_t>(P->nitems_s, 1)));
+        CHK(dalloc(h, &P->items_o, std::max<int32_t>(P->nitems_o, 1)));
+        if (P->nitems_s)
+            HIPCHK(hipMemcpyAsync(P->items_s, its.data(), sizeof(FixItem) * P->nitems_s, hipMemcpyHostToDevice,
+                                  h->stream));
+        if (P->nitems_o)
+            HIPCHK(hipMemcpyAsync(P->items_o, ito.data(), sizeof(FixItem) * P->nitems_o, hipMemcpyHostToDevice,
+                                  h->stream));
+        CHK(dalloc_bytes(h, &P->zs, ts * h->n[h->s] * K));
+        CHK(dalloc(h, &P->ops_one, 1));
+        CHK(dalloc(h, &P->ops_neg, 1));
+        CHK(dalloc(h, &P->ops_sub, 1));
+    }
     CHK(dalloc(h, &P->sdirect, sdirect.size()));
     HIPCHK(hipMemcpyAsync(P->sdirect, sdirect.data(), sdirect.size(), hipMemcpyHostToDevice, h->stream));
     CHK(dalloc(h, &P->tileA, (size_t)std::max<int32_t>(P->total_tiles, 1) * K));
@@ -1049,6 +1093,12 @@ template <typename T>
 __global__ void k_copy(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) b[e] = a[e];
+}
+
+template <typename T>
+__global__ void k_square(const T *__restrict__ a, int64_t n, T *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i] * a[i];
 }
 
 template <typename T>
@@ -1292,8 +1342,27 @@ struct Engine {
                            h->stream, p);
     }
 
-    // u <- b*u + a*A v with the raw D'u sums of FE s and of the dense FEs
-    static void fused(fehip_solver *h, Plan *P, const ColOps *ops, const T *coef, T *target) {
+    template <int PM>
+    static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
+        const dim3 gr(P->fused_blocks), bl(kBlock);
+        if (h->factored) {
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, PM>), gr, bl, P->fused_lds, h->stream, p);
+            else if (h->G == 2)
+                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, PM>), gr, bl, P->fused_lds, h->stream, p);
+            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, true, PM>), gr, bl, P->fused_lds, h->stream, p);
+        } else {
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, false, PM>), gr, bl, P->fused_lds, h->stream, p);
+            else if (h->G == 2)
+                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, false, PM>), gr, bl, P->fused_lds, h->stream, p);
+            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, false, PM>), gr, bl, P->fused_lds, h->stream, p);
+        }
+    }
+
+    // u <- b*u + a*A v with the raw D'u sums of FE s and of the dense FEs.
+    // pm (projected iteration): 1 = only the FE s sums of D*v over the
+    // other FEs; 2 = u <- b*u + a*(A_o v - A_s zs), zs the FE s correction,
+    // without FE s sums
+    static void fused(fehip_solver *h, Plan *P, const ColOps *ops, const T *coef, T *target, int pm = 0) {
         FusedParams<T> p{};
         p.N = h->N;
         p.R = P->R;
@@ -1329,16 +1398,102 @@ struct Engine {
         p.all_done = P->all_done;
         p.sqrtw = (const T *)h->sqrtw_T[pidx<T>()];
         p.vt = (const T *)P->vt;
-        const dim3 gr(P->fused_blocks), bl(kBlock);
-        if (h->factored) {
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true>), gr, bl, P->fused_lds, h->stream, p);
-            else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true>), gr, bl, P->fused_lds, h->stream, p);
-            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, true>), gr, bl, P->fused_lds, h->stream, p);
+        for (int g = 0; g < h->G; g++) p.gsrc[g] = (h->factored ? (const T *)P->vt : coef) + h->off[g] * K;
+        if (pm == 1) {
+            p.ndense = 0;
+            p.nslot = 0;
+            launch_fused<1>(h, P, p);
+        } else if (pm == 2) {
+            p.gsrc[h->s] = (const T *)P->zs;
+            launch_fused<2>(h, P, p);
         } else {
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, false>), gr, bl, P->fused_lds, h->stream, p);
-            else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2, false>), gr, bl, P->fused_lds, h->stream, p);
-            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, false>), gr, bl, P->fused_lds, h->stream, p);
+            launch_fused<0>(h, P, p);
         }
+    }
+
+    // y <- y - A_s A_s' y: the projection off FE s (the columns of A_s are
+    // orthonormal: disjoint supports, unit norms by scale!), through zs.
+    // mode MODE_INIT leaves the ||y||^2 partials of k_db for fin_db.  Uses
+    // ops_one = (1, 0) and ops_sub = (-1, 1), set by start().
+    static void proj_s(fehip_solver *h, Plan *P, T *y, int mode) {
+        const FeInfo &f = h->fe[h->s];
+        DtxSegParams<T> q{};
+        q.N = h->N;
+        q.nseg = 1;
+        q.first_tile = 0;
+        q.total_tiles = P->ntiles_s;
+        SegFE<T> &sf = q.fe[0];
+        sf.perm = f.kind == KIND_SORTED ? nullptr : f.iperm;
+        sf.cache = (const T *)f.cache_csr[pidx<T>()];
+        sf.heads = f.heads;
+        sf.wprefix = f.wprefix;
+        sf.nsegs = std::max<int32_t>(P->seg_nsegs[0], 1);
+        sf.seg_level = P->seg_nsegs[0] == f.n ? nullptr : f.seg_level;
+        sf.coef_off = 0;
+        sf.tp = P->seg_tp[0];
+        sf.tile0 = 0;
+        sf.ntiles = P->ntiles_s;
+        q.u = y;
+        q.v = (T *)P->zs;
+        q.tileA = P->tileA;
+        q.tileB = P->tileB;
+        q.ops = P->ops_one;
+        q.partials = P->part_seg;
+        q.all_done = nullptr;
+        q.mode = MODE_NONE;
+        hipLaunchKernelGGL((k_dtx_seg<T, Acc, K>),
+                           dim3((unsigned)std::min<int64_t>(grid_for(P->ntiles_s, kWavesPerBlock), 2048)),
+                           dim3(kBlock), 0, h->stream, q);
+        FixParams<T> fx{};
+        fx.nitems = P->nitems_s;
+        fx.items = P->items_s;
+        fx.tileA = P->tileA;
+        fx.tileB = P->tileB;
+        fx.dpart = P->dpart;
+        fx.v = (T *)P->zs;
+        fx.ops = P->ops_one;
+        fx.partials = P->part_fix;
+        fx.all_done = nullptr;
+        fx.mode = MODE_NONE;
+        hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, fx);
+        DbParams<T> d{};
+        d.N = h->N;
+        d.G = 1;
+        d.refs[0] = f.ref_int;
+        d.cache[0] = (const T *)f.cache_int[pidx<T>()];
+        d.coef_off[0] = 0;
+        d.coef = (const T *)P->zs;
+        d.target = y;
+        d.ops = P->ops_sub;
+        d.partials = P->part_db;
+        d.all_done = nullptr;
+        d.mode = mode;
+        hipLaunchKernelGGL((k_db<T, K>), dim3(P->db_blocks), dim3(kBlock), 0, h->stream, d);
+    }
+
+    // projected iteration, after the sums-only fused pass: zs = -A_s' t for
+    // t = A_o v (FE s sums completed from the tiles; under the factored form
+    // the raw sums of sqrt(w)*t take scale_s^2, the FE s scale once for A_s'
+    // and once for the A_s that the second fused pass applies to zs)
+    static void fix_s(fehip_solver *h, Plan *P) {
+        FixParams<T> f{};
+        f.nitems = P->nitems_s;
+        f.items = P->items_s;
+        f.tileA = P->tileA;
+        f.tileB = P->tileB;
+        f.dpart = P->dpart;
+        f.v = (T *)P->zs;
+        f.ops = P->ops_neg;
+        f.partials = P->part_fix;
+        f.all_done = P->all_done;
+        f.mode = MODE_NONE;
+        f.wsum = P->wsum;
+        f.sdirect = P->sdirect;
+        f.ns = h->n[h->s];
+        f.s_off = 0;
+        f.s = h->s;
+        if (h->factored) f.fscale[h->s] = (const T *)h->scale2_T[pidx<T>()];
+        hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
     }
 
     static void dtx_dense(fehip_solver *h, Plan *P, const ColOps *ops, const T *y, const int32_t *early_exit) {
@@ -1367,11 +1522,13 @@ struct Engine {
                            h->stream, p);
     }
 
+    // proj: the projected iteration's form -- the items of the FEs other
+    // than s only, FE s untouched (its coefficients stay zero)
     static void dtx_fixup(fehip_solver *h, Plan *P, int mode, const ColOps *ops, T *coef,
-                          const int32_t *early_exit, bool fused = false) {
+                          const int32_t *early_exit, bool fused = false, bool proj = false) {
         FixParams<T> f{};
-        f.nitems = P->nitems;
-        f.items = P->items;
+        f.nitems = proj ? P->nitems_o : P->nitems;
+        f.items = proj ? P->items_o : P->items;
         f.tileA = P->tileA;
         f.tileB = P->tileB;
         f.dpart = P->dpart;
@@ -1381,13 +1538,15 @@ struct Engine {
         f.all_done = early_exit;
         f.mode = mode;
         if (fused) {
-            f.wsum = P->wsum;
-            f.sdirect = P->sdirect;
-            f.ns = h->n[h->s];
-            f.s_off = h->off[h->s];
-            f.s = h->s;
+            if (!proj) {
+                f.wsum = P->wsum;
+                f.sdirect = P->sdirect;
+                f.ns = h->n[h->s];
+                f.s_off = h->off[h->s];
+                f.s = h->s;
+            }
             if (h->factored) {
-                f.fscale[h->s] = (const T *)h->fe[h->s].scale_T[pidx<T>()];
+                if (!proj) f.fscale[h->s] = (const T *)h->fe[h->s].scale_T[pidx<T>()];
                 if (P->fused_dense)
                     for (int g : P->dense_fe) f.fscale[g] = (const T *)h->fe[g].scale_T[pidx<T>()];
             }
@@ -1542,6 +1701,20 @@ struct Engine {
     // beta, D'u of the CSR FEs, the fix-up, alpha and the rotations, the
     // vector update, the stopping rules
     static void iteration(fehip_solver *h, Plan *P) {
+        if (h->fwl_now) {
+            // projected: A = (I - A_s A_s') A_o, coefficients of FE s zero
+            fused(h, P, P->ops_one, (const T *)P->v, (T *)P->u, 1);
+            fix_s(h, P);
+            fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u, 2);
+            fin_db(h, P, MODE_ITER, P->all_done, true);
+            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done);
+            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true);
+            dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, true);
+            fin_dtx(h, P, MODE_ITER, P->all_done, true);
+            update(h, P);
+            fin_update(h, P);
+            return;
+        }
         fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
         fin_db(h, P, MODE_ITER, P->all_done, true);
         if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done);
@@ -1554,7 +1727,7 @@ struct Engine {
 
     static int ensure_graph(fehip_solver *h, Plan *P) {
         const int M = h->poll_iters;
-        if (P->graph && P->graph_iters == M) return 0;
+        if (P->graph && P->graph_iters == M && P->graph_fwl == h->fwl_now) return 0;
         if (P->graph) {
             (void)hipGraphExecDestroy(P->graph);
             P->graph = nullptr;
@@ -1568,6 +1741,7 @@ struct Engine {
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail(FEHIP_EHIP, std::string("graph instantiate failed: ") + hipGetErrorString(e));
         P->graph_iters = M;
+        P->graph_fwl = h->fwl_now;
         return 0;
     }
 
@@ -1592,13 +1766,31 @@ struct Engine {
     static int lsmr_start(fehip_solver *h, Plan *P, int Kreal, double tol, int64_t maxiter, Run &run) {
         hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, Kreal,
                            tol, tol, 1e8, maxiter, P->all_done);
-        // u = b - A x ; beta = ||u||  (:63-65)
-        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
-        db(h, P, MODE_INIT, P->ops_once, (const T *)P->x, (T *)P->u, nullptr);
-        fin_db(h, P, MODE_INIT, nullptr);
-        // v = A'u ; alpha = ||v|| and the scalar set-up  (:66-98)
-        dtx(h, P, MODE_INIT, P->ops_dtx, (const T *)P->u, (T *)P->v, nullptr);
-        fin_dtx(h, P, MODE_INIT, nullptr);
+        if (h->fwl_now) {
+            // projected: x_s = 0 ; u = (I - A_s A_s')(b - A_o x) with b
+            // already projected ; v = A_o'u with v_s = 0
+            const int64_t so = h->off[h->s] * K, sn = h->n[h->s] * K;
+            hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(sn, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->x + so, sn,
+                               (T)0);
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
+            db(h, P, MODE_NONE, P->ops_once, (const T *)P->x, (T *)P->u, nullptr);
+            proj_s(h, P, (T *)P->u, MODE_INIT);
+            fin_db(h, P, MODE_INIT, nullptr);
+            hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(sn, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->v + so, sn,
+                               (T)0);
+            dtx_seg(h, P, MODE_INIT, P->ops_dtx, (const T *)P->u, (T *)P->v, nullptr, true);
+            dtx_dense(h, P, P->ops_dtx, (const T *)P->u, nullptr);
+            dtx_fixup(h, P, MODE_INIT, P->ops_dtx, (T *)P->v, nullptr, false, true);
+            fin_dtx(h, P, MODE_INIT, nullptr, true);
+        } else {
+            // u = b - A x ; beta = ||u||  (:63-65)
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
+            db(h, P, MODE_INIT, P->ops_once, (const T *)P->x, (T *)P->u, nullptr);
+            fin_db(h, P, MODE_INIT, nullptr);
+            // v = A'u ; alpha = ||v|| and the scalar set-up  (:66-98)
+            dtx(h, P, MODE_INIT, P->ops_dtx, (const T *)P->u, (T *)P->v, nullptr);
+            fin_dtx(h, P, MODE_INIT, nullptr);
+        }
         hipLaunchKernelGGL((k_init_vectors<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, h->p * K,
                            (T *)P->v, (T *)P->hh, (T *)P->hbar, (const void *)P->state,
                            (const T *)h->scale_all[pidx<T>()], (T *)P->vt);
@@ -1658,6 +1850,7 @@ struct Engine {
         uint8_t *conv = nullptr;
     };
     static int job_begin(Job &j, double tol, int64_t maxiter) {
+        FwlScope fwl(j.h);
         CHK((get_plan_for<T, Acc, K>(j.h, &j.P)));
         CHK(start(j.h, j.P, j.dY, j.Kreal, j.ldY, true));
         if (j.h->G > 1) CHK(lsmr_start(j.h, j.P, j.Kreal, tol, maxiter - 1, j.run));
@@ -1671,6 +1864,7 @@ struct Engine {
         return lsmr_poll(j.h, j.P, j.run, false, finished);
     }
     static int job_end(Job &j) {
+        FwlScope fwl(j.h);
         std::vector<int64_t> it(j.Kreal, 1);
         std::vector<int> cv(j.Kreal, 1);
         if (j.h->G > 1) {
@@ -1718,6 +1912,22 @@ struct Engine {
         const int64_t NK = h->N * K;
         hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
                            h->perm_s, h->sqrtw_int, h->N, (T *)P->r, (T *)P->u);
+        if (h->fwl_now) {
+            // projected problem: b <- (I - A_s A_s') b
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_one, 1.0, 0.0, K, Kreal);
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_neg, -1.0, 0.0, K, Kreal);
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_sub, -1.0, 1.0, K, Kreal);
+            if (h->factored) {
+                constexpr int q = pidx<T>();
+                const int64_t ns = h->n[h->s];
+                if (!h->scale2_T[q]) CHK(dalloc_bytes(h, &h->scale2_T[q], sizeof(T) * ns));
+                hipLaunchKernelGGL(k_square<T>, dim3(grid_for(ns, kBlock)), dim3(kBlock), 0, h->stream,
+                                   (const T *)h->fe[h->s].scale_T[q], ns, (T *)h->scale2_T[q]);
+            }
+            proj_s(h, P, (T *)P->r, MODE_NONE);
+            hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (const T *)P->r,
+                               (T *)P->u, NK);
+        }
         if (x_from_At) {
             hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.0, K, Kreal);
             dtx(h, P, MODE_NONE, P->ops_once, (const T *)P->r, (T *)P->x, nullptr);
@@ -1731,6 +1941,7 @@ struct Engine {
         const int64_t NK = h->N * K;
         hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
         db(h, P, MODE_NONE, P->ops_once, (const T *)P->x, (T *)P->r, nullptr);
+        if (h->fwl_now) proj_s(h, P, (T *)P->r, MODE_NONE);  // r = b - A x with A projected
         hipLaunchKernelGGL(k_store_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
                            h->perm_s, h->sqrtw_int, h->N, (const T *)P->r);
         HIPCHK(hipGetLastError());
@@ -1836,6 +2047,7 @@ struct Engine {
     // network's start when net is given
     static int solve_residuals_from(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
                                     const double *const *coef0, fehip_mlp *net, int64_t *iters, uint8_t *converged) {
+        FwlScope fwl(h);
         Plan *P = nullptr;
         CHK((get_plan_for<T, Acc, K>(h, &P)));
         CHK(ensure_ws(h));
@@ -2075,6 +2287,7 @@ static int padded_K(int K) {
 
 static int solve_residuals_dev(fehip_solver *h, double *dY, int Kreal, int64_t ldY, double tol, int64_t maxiter,
                                int64_t *iters, uint8_t *converged) {
+    FwlScope fwl(h);
     if (h->precision == FEHIP_MIXED)
         return for_K_mixed(padded_K(Kreal), [&](auto e) {
             return decltype(e)::solve_residuals(h, dY, Kreal, ldY, tol, maxiter, iters, converged);
@@ -2750,6 +2963,14 @@ int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms) {
     if (!h || !out_ms) return fail(FEHIP_EARG, "bad arguments");
     if (K < 1 || K > kMaxK || reps < 1) return fail(FEHIP_EARG, "K must be in 1..32 and reps >= 1");
     return dispatch(h->precision, padded_K(K), [&](auto e) { return decltype(e)::time_kernels(h, reps, out_ms); });
+}
+
+int fehip_set_algorithm(fehip_t *h, int32_t algorithm) {
+    g_err.clear();
+    if (!h) return fail(FEHIP_EARG, "handle is NULL");
+    if (algorithm != FEHIP_LSMR && algorithm != FEHIP_LSMR_PROJECTED) return fail(FEHIP_EARG, "unknown algorithm");
+    h->fwl_req = algorithm == FEHIP_LSMR_PROJECTED;
+    return 0;
 }
 
 int fehip_set_schedule(fehip_t *h, int64_t tiles_s, int64_t tiles_c, int32_t fuse_dense, int32_t poll_iters) {

@@ -754,6 +754,10 @@ struct FusedParams {
     const int32_t *seg_level;
     const T *coef;     // v
     T *target;         // u
+    // per FE: the vector gathered at ref_g (scale .* v under FACT, else v,
+    // from the FE's first level); the projected iteration points FE s at
+    // its correction vector instead
+    const T *gsrc[kMaxG];
     // FACT: cache_g = sqrt(w) * scale_g[ref] for every FE (no interactions),
     // so D*v = sqrt(w) * sum_g vt_g[ref_g] with vt = scale .* v (kept by the
     // vector update) and the raw D'u sums are of sqrt(w)*u (scaled in the
@@ -788,7 +792,10 @@ __device__ __forceinline__ void sto(V *base, uint32_t i, V v) {
     *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(V)) = v;
 }
 
-template <typename T, typename Acc, int K, int GT, bool FACT>
+// PM (projected iteration, FE s eliminated): 0 = plain pass; 1 = sums only:
+// the FE s segment sums of cache_s * (D*v over the other FEs), no u load,
+// store or norm, no dense sums; 2 = the u pass without FE s segment sums.
+template <typename T, typename Acc, int K, int GT, bool FACT, int PM = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
     extern __shared__ __align__(16) unsigned char fused_smem[];
@@ -850,15 +857,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const uint32_t o = lb + (uint32_t)(q * P + psub);
                 const bool ok = on && o < len;
                 T out = a * st.sum[q];
-                out = keep ? b * st.uo[q] + out : out;
+                if constexpr (PM != 1) out = keep ? b * st.uo[q] + out : out;
                 un[q] = ok ? out : (T)0;
-                if (ok) sto(ub, o * K + c, out);
-                sq += (double)un[q] * (double)un[q];
+                if constexpr (PM != 1) {
+                    if (ok) sto(ub, o * K + c, out);
+                    sq += (double)un[q] * (double)un[q];
+                }
             }
             // FE s: segmented sums of cache_s * u; a segment ending at a
             // position belongs to that position's level
 #pragma unroll
-            for (int q = 0; q < U; q++) {
+            for (int q = 0; q < (PM == 2 ? 0 : U); q++) {
                 const uint32_t ob = lb + (uint32_t)(q * P);
                 if (ob >= len) break;  // wave-uniform
                 Acc x = mul_acc<Acc, T>(st.cs[q], un[q]);
@@ -868,7 +877,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     sto(p.wsum, (uint32_t)st.rs[q] * K + c, acc_to_double(x));
             }
             // dense FEs: lane-private LDS accumulators
-            if (p.ndense) {
+            if (PM != 1 && p.ndense) {
                 const int s0 = p.dense_slot0[0];
                 int32_t sl[U];
                 Acc x[U];
@@ -912,7 +921,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     const uint32_t i = loc(lb, q);
-                    d.uo[q] = ldo(ub, i * K + c);
+                    d.uo[q] = PM == 1 ? (T)0 : ldo(ub, i * K + c);
                     d.sw[q] = (T)1;
                     if constexpr (FACT) {
                         if (swb) d.sw[q] = ldo(swb, i);  // wave-uniform test
@@ -920,7 +929,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
-                    const T *cf = (FACT ? p.vt : p.coef) + p.coef_off[g] * K;
+                    const T *cf = p.gsrc[g];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
                         d.gv[g][q] = ldo(cf, (uint32_t)rr[g][q] * K + c);
@@ -941,7 +950,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                     for (int g = 0; g < GT; g++)
 #pragma unroll
-                        for (int q = 0; q < U; q++) st.sum[q] += d.gv[g][q];
+                        for (int q = 0; q < U; q++) st.sum[q] += (PM == 1 && g == p.s) ? (T)0 : d.gv[g][q];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
                         st.sum[q] = d.sw[q] * st.sum[q];
@@ -951,7 +960,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                     for (int g = 0; g < GT; g++)
 #pragma unroll
-                        for (int q = 0; q < U; q++) st.sum[q] += d.cv[g][q] * d.gv[g][q];
+                        for (int q = 0; q < U; q++)
+                            st.sum[q] += (PM == 1 && g == p.s) ? (T)0 : d.cv[g][q] * d.gv[g][q];
                 }
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
@@ -1000,7 +1010,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     st.sum[q] = (T)0;
-                    st.uo[q] = ldo(ub, loc(lb, q) * K + c);
+                    st.uo[q] = PM == 1 ? (T)0 : ldo(ub, loc(lb, q) * K + c);
                     st.cs[q] = st.cd[q] = (T)0;
                     st.rs[q] = st.rd[q] = 0;
                 }
@@ -1022,11 +1032,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         r[q] = ldo(rb, loc(lb, q));
                         cv[q] = FACT ? (T)1 : ldo(p.cache[g] + start, loc(lb, q));
                     }
-                    const T *cf = (FACT ? p.vt : p.coef) + p.coef_off[g] * K;
+                    const T *cf = p.gsrc[g];
+                    if (PM != 1 || g != p.s) {
 #pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        const T gv = ldo(cf, (uint32_t)r[q] * K + c);
-                        st.sum[q] += FACT ? gv : cv[q] * gv;
+                        for (int q = 0; q < U; q++) {
+                            const T gv = ldo(cf, (uint32_t)r[q] * K + c);
+                            st.sum[q] += FACT ? gv : cv[q] * gv;
+                        }
                     }
                     if (g == p.s) {
 #pragma unroll
@@ -1050,12 +1062,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 compute(st, lb);
             }
         }
-        if (cur.open && on && psub == 0) {
+        if (PM != 2 && cur.open && on && psub == 0) {
             if (cur.cont) p.tileA[(int64_t)t * K + c] = acc_to_double(cur.carry);
             else p.tileB[(int64_t)t * K + c] = acc_to_double(cur.carry);
         }
     }
-    if (p.ndense) {
+    if (PM != 1 && p.ndense) {
         // per-wave slot sums (written in place, see k_dtx_dense), then the
         // waves of the workgroup in order -> one dpart row per workgroup
         double *ws = reinterpret_cast<double *>(acc);

@@ -136,6 +136,7 @@ def group(*keys):
 # solver (the :rocm backend object)
 # ---------------------------------------------------------------------------
 _PRECISIONS = {"fp64": FEHIP_FP64, "fp32": FEHIP_FP32, "mixed": FEHIP_MIXED}
+_ALGORITHMS = {"lsmr": 0, "projected": 1}
 
 
 class FixedEffectSolverROCm:
@@ -145,7 +146,7 @@ class FixedEffectSolverROCm:
     device memory for the life of the object.
     """
 
-    def __init__(self, fes: Sequence[FixedEffect], weights=None, precision: str = "fp64"):
+    def __init__(self, fes: Sequence[FixedEffect], weights=None, precision: str = "fp64", algorithm: str = "lsmr"):
         fes = list(fes)
         if not fes:
             raise ValueError("at least one FixedEffect is required")
@@ -172,6 +173,9 @@ class FixedEffectSolverROCm:
                                C.cast(None, _dp) if w is None else _ptr(w, C.c_double), _PRECISIONS[precision]))
         self._h = h
         self.weights = w
+        self.algorithm = "lsmr"
+        if algorithm != "lsmr":
+            self.set_algorithm(algorithm)
 
     def _weights_array(self, weights):
         if weights is None:
@@ -318,6 +322,16 @@ class FixedEffectSolverROCm:
         out = np.empty(self.N, np.float64)
         check(lib.fehip_get_cache(self._h, g, _ptr(out, C.c_double)))
         return out
+
+    def set_algorithm(self, algorithm: str) -> "FixedEffectSolverROCm":
+        """"lsmr" (the reference's LSMR on all FEs) or "projected" (LSMR with
+        the FE of most levels eliminated; same least-squares residual, fewer
+        iterations; residual solves only)."""
+        if algorithm not in _ALGORITHMS:
+            raise ValueError(f"algorithm must be one of {sorted(_ALGORITHMS)}")
+        check(lib.fehip_set_algorithm(self._h, _ALGORITHMS[algorithm]))
+        self.algorithm = algorithm
+        return self
 
     def set_schedule(self, tiles_s: int = 0, tiles_c: int = 0, fuse_dense: int = -1, poll_iters: int = 0):
         """Schedule of the LSMR launches (0 / -1 = defaults); see fehip_set_schedule."""

@@ -136,6 +136,21 @@ int fehip_solve_coefficients_full(fehip_t *h, const double *y, double tol,
                                   double *const *obs, int64_t *iters,
                                   uint8_t *converged);
 
+/* LSMR algorithm of the residual solves (no reference counterpart).
+ * FEHIP_LSMR (default): the reference's LSMR on the whole map A = [A_1 ... A_G]
+ * (src/utils/lsmr.jl), iteration counts comparable with the reference.
+ * FEHIP_LSMR_PROJECTED: LSMR on the map with the FE of most levels (s)
+ * eliminated (Frisch-Waugh-Lovell): (I - A_s A_s') A_o over the other FEs'
+ * coefficients, right-hand side (I - A_s A_s') b, where A_s' and A_s are the
+ * exact within-level demeaning (the columns of A_s are orthonormal).  The
+ * least-squares residual is the same; LSMR needs fewer iterations (a smaller
+ * condition number) and the same stopping rule applies to the projected
+ * problem.  Applies to fehip_solve_residuals*, panels and the _x0/_learned
+ * forms; the coefficient solves always use FEHIP_LSMR. */
+#define FEHIP_LSMR 0
+#define FEHIP_LSMR_PROJECTED 1
+int fehip_set_algorithm(fehip_t *h, int32_t algorithm);
+
 /* FixedEffect(keys...) grouping (src/FixedEffect.jl:18-21, which calls
  * GroupedArray(keys..., sort = nothing)): dense 1-based refs for the distinct
  * combinations of ncols int64 key columns (keys[c]: N host values), numbered
