@@ -1386,7 +1386,8 @@ struct Engine {
         const FeInfo &fs = h->fe[h->s];
         p.heads = fs.heads;
         p.wprefix = fs.wprefix;
-        p.seg_level = fs.seg_level;
+        p.seg_level = P->seg_nsegs[0] == fs.n ? nullptr : fs.seg_level;
+        p.nsegs_s = std::max<int32_t>(P->seg_nsegs[0], 1);
         p.coef = coef;
         p.target = target;
         p.wsum = P->wsum;
@@ -1693,6 +1694,10 @@ struct Engine {
         p.state = P->state;
         p.scale = (const T *)h->scale_all[pidx<T>()];
         p.vt = (T *)P->vt;
+        if (h->fwl_now) {
+            p.skip0 = h->off[h->s] * K;
+            p.skipn = h->n[h->s] * K;
+        }
         hipLaunchKernelGGL((k_update<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, p);
     }
 

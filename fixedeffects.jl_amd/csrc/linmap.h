@@ -751,7 +751,8 @@ struct FusedParams {
     int64_t dense_row0[kMaxG];   // its first dpart row
     const uint64_t *heads;       // FE s
     const uint32_t *wprefix;
-    const int32_t *seg_level;
+    const int32_t *seg_level;    // nullptr = identity (every FE s level observed)
+    int32_t nsegs_s;             // observed levels of FE s
     const T *coef;     // v
     T *target;         // u
     // per FE: the vector gathered at ref_g (scale .* v under FACT, else v,
@@ -850,6 +851,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 hw[j] = (lb < len && wi <= ((end - 1) >> 6) + 1) ? p.heads[wi] : 0ull;
             }
         };
+        // the same words with their head-count prefixes (heads before the word)
+        auto load_words_wp = [&](uint32_t lb, uint64_t (&hw)[NW], uint32_t (&wp)[NW]) {
+            const int64_t b0 = start + lb;
+#pragma unroll
+            for (int j = 0; j < NW; j++) {
+                const int64_t wi = (b0 >> 6) + j;
+                const bool ok = lb < len && wi <= ((end - 1) >> 6) + 1;
+                hw[j] = ok ? p.heads[wi] : 0ull;
+                wp[j] = ok ? p.wprefix[wi] : 0u;
+            }
+        };
+        // FE s level of lane position lb + q*P + psub: the ordinal of its
+        // segment (heads up to it, minus one), clamped for positions past N,
+        // through seg_level when some level has no observation.  Replaces a
+        // load of ref_s: positions are sorted by FE s.
+        auto level_s = [&](uint32_t lb, int q, const uint64_t (&hw)[NW], const uint32_t (&wp)[NW]) -> int32_t {
+            const uint32_t ob = lb + (uint32_t)(q * P);
+            const int jw = (q * P) / kWave;  // constant after unrolling
+            const int bit = (int)((ob + (uint32_t)psub) & 63u);
+            uint32_t ord = wp[jw] + (uint32_t)__popcll(hw[jw] & lanes_le(bit)) - 1u;
+            ord = ord < (uint32_t)p.nsegs_s ? ord : (uint32_t)p.nsegs_s - 1u;
+            return p.seg_level ? p.seg_level[ord] : (int32_t)ord;
+        };
         auto compute = [&](const Step &st, uint32_t lb) {
             T un[U];
 #pragma unroll
@@ -902,22 +926,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
         };
         if constexpr (GT > 0) {
-            // stage A: refs of a step
-            auto issue_refs = [&](uint32_t lb, int32_t (&rr)[GA][U]) {
+            // stage A: refs of a step (FE s: none, its level comes from the
+            // head words, loaded here with their prefixes)
+            struct Refs {
+                int32_t r[GA][U];
+                uint64_t hw[NW];
+                uint32_t wp[NW];
+            };
+            auto issue_refs = [&](uint32_t lb, Refs &rr) {
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
                     const int32_t *rb = p.refs[g] + start;
 #pragma unroll
-                    for (int q = 0; q < U; q++) rr[g][q] = ldo(rb, loc(lb, q));
+                    for (int q = 0; q < U; q++) rr.r[g][q] = g == p.s ? 0 : ldo(rb, loc(lb, q));
                 }
+                load_words_wp(lb, rr.hw, rr.wp);
             };
             // stage B: every other load of a step (gathers use its refs)
             struct InFlight {
                 T uo[U], cv[FACT ? 1 : GA][U], gv[GA][U], sw[U];
-                uint64_t hw[NW];
             };
-            auto issue_data = [&](uint32_t lb, const int32_t (&rr)[GA][U], InFlight &d) {
-                load_words(lb, d.hw);
+            auto issue_data = [&](uint32_t lb, Refs &rr, InFlight &d) {
+#pragma unroll
+                for (int g = 0; g < GT; g++) {
+                    if (g == p.s) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) rr.r[g][q] = level_s(lb, q, rr.hw, rr.wp);
+                    }
+                }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     const uint32_t i = loc(lb, q);
@@ -932,13 +968,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     const T *cf = p.gsrc[g];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        d.gv[g][q] = ldo(cf, (uint32_t)rr[g][q] * K + c);
+                        d.gv[g][q] = ldo(cf, (uint32_t)rr.r[g][q] * K + c);
                         if constexpr (!FACT) d.cv[g][q] = ldo(p.cache[g] + start, loc(lb, q));
                     }
                 }
             };
             // the step's loads have arrived: form the D*v sums (FEs in order)
-            auto settle = [&](const InFlight &d, const int32_t (&rr)[GA][U], Step &st) {
+            auto settle = [&](const InFlight &d, const Refs &rr, Step &st) {
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     st.sum[q] = (T)0;
@@ -969,40 +1005,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                         for (int q = 0; q < U; q++) {
                             if constexpr (!FACT) st.cs[q] = d.cv[g][q];
-                            st.rs[q] = rr[g][q];
+                            st.rs[q] = rr.r[g][q];
                         }
                     }
                     if (p.ndense && g == p.dense_g[0]) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
                             if constexpr (!FACT) st.cd[q] = d.cv[g][q];
-                            st.rd[q] = rr[g][q];
+                            st.rd[q] = rr.r[g][q];
                         }
                     }
                 }
 #pragma unroll
-                for (int j = 0; j < NW; j++) st.hw[j] = d.hw[j];
+                for (int j = 0; j < NW; j++) st.hw[j] = rr.hw[j];
             };
-            int32_t r1[GA][U], r2[GA][U];
+            Refs r1, r2;
             InFlight d;
             Step st;
             issue_refs(0u, r1);
             issue_data(0u, r1, d);
             issue_refs(UP, r2);
             settle(d, r1, st);
-#pragma unroll
-            for (int g = 0; g < GT; g++)
-#pragma unroll
-                for (int q = 0; q < U; q++) r1[g][q] = r2[g][q];
+            r1 = r2;
             for (uint32_t lb = 0; lb < len; lb += UP) {
                 issue_refs(lb + 2 * UP, r2);
                 issue_data(lb + UP, r1, d);
                 compute(st, lb);
                 settle(d, r1, st);
-#pragma unroll
-                for (int g = 0; g < GT; g++)
-#pragma unroll
-                    for (int q = 0; q < U; q++) r1[g][q] = r2[g][q];
+                r1 = r2;
             }
         } else {
             for (uint32_t lb = 0; lb < len; lb += UP) {
@@ -1205,6 +1235,10 @@ struct UpdParams {
     const void *state;
     const T *scale;  // p: with vt, the fused pass's scale .* v (FACT)
     T *vt;
+    // elements [skip0, skip0 + skipn) are left alone (the coefficients of
+    // the eliminated FE in the projected iteration, identically zero);
+    // both multiples of K
+    int64_t skip0, skipn;
 };
 
 template <typename T, typename Tr, int K>
@@ -1217,7 +1251,9 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdParams<T> p) {
     const T ia = (T)st[c].inv_alpha, chb = (T)st[c].c_hbar, cx = (T)st[c].c_x, ch = (T)st[c].c_h;
     double sq = 0.0;
     if (act) {
-        for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < p.pK; e += (int64_t)gridDim.x * kBlock) {
+        const int64_t n = p.pK - p.skipn;
+        for (int64_t e2 = (int64_t)blockIdx.x * kBlock + threadIdx.x; e2 < n; e2 += (int64_t)gridDim.x * kBlock) {
+            const int64_t e = e2 < p.skip0 ? e2 : e2 + p.skipn;
             T vn = p.v[e];
             if (!vs) {
                 vn = vn * ia;

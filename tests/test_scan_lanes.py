@@ -88,3 +88,70 @@ def test_half_swap_scan_k32_matches_segmented_scan():
     last = np.where(np.arange(64) < 32, half_swap(x), x)
     np.testing.assert_array_equal(last[:32], x[32:])
     np.testing.assert_array_equal(last[32:], x[32:])
+
+
+# -- FE s level from the head words (k_fused level_s) ------------------------------
+
+def _heads_and_prefix(keys, n):
+    """Head bits (bit j of word j//64: j == 0, a key change, or j >= n) and the
+    per-word count of heads before the word (positions < n), as csr_build.h
+    builds them."""
+    nwords = n // 64 + 2
+    pos = np.arange(nwords * 64)
+    h = np.ones(nwords * 64, bool)
+    h[1:n] = keys[1:n] != keys[:n - 1]
+    real = h.copy()
+    real[n:] = False
+    words = np.zeros(nwords, np.uint64)
+    for b in range(64):
+        words |= (h[pos % 64 == b].astype(np.uint64) << np.uint64(b))
+    cnt = real.reshape(nwords, 64).sum(1)
+    wprefix = np.concatenate([[0], np.cumsum(cnt)])[:nwords].astype(np.uint32)
+    return words, wprefix
+
+
+def _level_s(words, wprefix, seg_level, nsegs, start, end, lb, q, psub, P, NW):
+    """k_fused level_s / load_words_wp, lane by lane (uint32 wrap-around and
+    clamping as in the kernel)."""
+    length = end - start
+    b0 = start + lb
+    jw = (q * P) // 64
+    wi = (b0 >> 6) + jw
+    ok = lb < length and wi <= ((end - 1) >> 6) + 1
+    hw = int(words[wi]) if ok else 0
+    wp = int(wprefix[wi]) if ok else 0
+    bit = (lb + q * P + psub) & 63
+    le = (1 << (bit + 1)) - 1
+    ordv = (wp + bin(hw & le).count("1") - 1) & 0xFFFFFFFF
+    ordv = ordv if ordv < nsegs else nsegs - 1
+    return int(seg_level[ordv]) if seg_level is not None else ordv
+
+
+def test_level_from_head_words():
+    rng = np.random.default_rng(3)
+    for trial in range(6):
+        n = int(rng.integers(300, 3000))
+        nlev = int(rng.integers(5, n // 2))
+        keys = np.sort(rng.integers(0, nlev, n))
+        if trial % 2:  # drop some levels entirely (seg_level not the identity)
+            keys = np.unique(keys, return_inverse=True)[1] * 2
+        words, wprefix = _heads_and_prefix(keys, n)
+        present = np.unique(keys)
+        identity = len(present) == keys.max() + 1
+        seg_level = None if identity else present
+        nsegs = len(present)
+        for K, U in ((1, 2), (1, 4), (2, 2), (4, 2), (8, 2), (16, 2), (32, 2)):
+            P = 64 // K
+            UP = U * P
+            NW = max(UP // 64 + 1, 2)
+            R = 256 * int(rng.integers(1, 4))
+            for start in range(0, n, R):
+                end = min(start + R, n)
+                for lb in range(0, end - start, UP):
+                    for q in range(U):
+                        for psub in range(P):
+                            posn = start + lb + q * P + psub
+                            if posn >= end:
+                                continue
+                            got = _level_s(words, wprefix, seg_level, nsegs, start, end, lb, q, psub, P, NW)
+                            assert got == keys[posn], (trial, K, U, start, lb, q, psub)
