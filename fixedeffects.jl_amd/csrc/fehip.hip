@@ -104,6 +104,7 @@ struct fehip_solver {
     hipStream_t stream = nullptr;
     std::vector<FeInfo> fe;
     int32_t *perm_s = nullptr;  // internal position -> caller's observation
+    int32_t *ident = nullptr;   // 0..N-1: the "permutation" of the sorted FE (kernels load it unconditionally)
     double *w_orig = nullptr;   // caller's order (nullptr: UnitWeights)
     double *sqrtw_int = nullptr;
     // no FE has an interaction: cache_g = sqrt(w) * scale_g[ref], and the
@@ -400,6 +401,11 @@ __global__ void k_cache(const int32_t *__restrict__ perm_s, int64_t N, const int
     cache[j] = (T)((zi * sqrt(wi)) * (double)scale[ref_int[j]]);
 }
 
+__global__ void k_iota(int32_t *__restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (int32_t)i;
+}
+
 template <typename T>
 __global__ void k_copy_range(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -621,6 +627,8 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
         f.kind = KIND_SORTED;
         CHK(dalloc(h, &f.ref_int, N));
         CHK(dalloc(h, &h->perm_s, N));
+        CHK(dalloc(h, &h->ident, N));
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, h->ident, N);
         if (t_sec >= 0) {
             int32_t *sk = nullptr, *p2 = nullptr, *k1 = nullptr;
             CHK(dalloc(h, &sk, N));
@@ -1319,12 +1327,12 @@ struct Engine {
         for (int q = 0; q < P->nseg; q++) {
             const FeInfo &f = h->fe[P->seg_fe[q]];
             SegFE<T> &s = p.fe[q];
-            s.perm = f.kind == KIND_SORTED ? nullptr : f.iperm;
+            s.perm = f.kind == KIND_SORTED ? h->ident : f.iperm;
             s.cache = (const T *)f.cache_csr[pidx<T>()];
             s.heads = f.heads;
             s.wprefix = f.wprefix;
             s.nsegs = std::max<int32_t>(P->seg_nsegs[q], 1);
-            s.seg_level = P->seg_nsegs[q] == f.n ? nullptr : f.seg_level;
+            s.seg_level = f.seg_level;
             s.coef_off = h->off[P->seg_fe[q]];
             s.tp = P->seg_tp[q];
             s.tile0 = P->tile0[q];
@@ -1367,18 +1375,30 @@ struct Engine {
         p.N = h->N;
         p.R = P->R;
         p.G = h->G;
-        p.s = h->s;
+        p.s = 0;
         p.ntiles = P->ntiles_s;
-        for (int g = 0; g < h->G; g++) {
-            p.refs[g] = h->fe[g].ref_int;
-            p.cache[g] = (const T *)h->fe[g].cache_int[pidx<T>()];
-            p.coef_off[g] = h->off[g];
+        // kernel FE order: FE s first; the first few-level FE of the pass
+        // last (compile-time positions in k_fused); the rest between
+        const bool dense_in = P->fused_dense && P->ndense > 0;
+        const int dlast = dense_in ? P->dense_fe[0] : -1;
+        int kord[kMaxG], kpos[kMaxG], nk = 0;
+        kord[nk++] = h->s;
+        for (int g = 0; g < h->G; g++)
+            if (g != h->s && g != dlast) kord[nk++] = g;
+        if (dlast >= 0) kord[nk++] = dlast;
+        for (int k = 0; k < h->G; k++) {
+            const int g = kord[k];
+            kpos[g] = k;
+            p.refs[k] = h->fe[g].ref_int;
+            p.cache[k] = (const T *)h->fe[g].cache_int[pidx<T>()];
+            p.coef_off[k] = h->off[g];
+            p.gsrc[k] = (h->factored ? (const T *)P->vt : coef) + h->off[g] * K;
         }
-        if (P->fused_dense) {
+        if (dense_in) {
             p.ndense = P->ndense;
             p.nslot = P->nslot;
             for (int q = 0; q < P->ndense; q++) {
-                p.dense_g[q] = P->dense_fe[q];
+                p.dense_g[q] = kpos[P->dense_fe[q]];
                 p.dense_slot0[q] = P->slot0[q];
                 p.dense_row0[q] = P->drow0[q];
             }
@@ -1386,7 +1406,7 @@ struct Engine {
         const FeInfo &fs = h->fe[h->s];
         p.heads = fs.heads;
         p.wprefix = fs.wprefix;
-        p.seg_level = P->seg_nsegs[0] == fs.n ? nullptr : fs.seg_level;
+        p.seg_level = fs.seg_level;
         p.nsegs_s = std::max<int32_t>(P->seg_nsegs[0], 1);
         p.coef = coef;
         p.target = target;
@@ -1399,13 +1419,12 @@ struct Engine {
         p.all_done = P->all_done;
         p.sqrtw = (const T *)h->sqrtw_T[pidx<T>()];
         p.vt = (const T *)P->vt;
-        for (int g = 0; g < h->G; g++) p.gsrc[g] = (h->factored ? (const T *)P->vt : coef) + h->off[g] * K;
         if (pm == 1) {
             p.ndense = 0;
             p.nslot = 0;
             launch_fused<1>(h, P, p);
         } else if (pm == 2) {
-            p.gsrc[h->s] = (const T *)P->zs;
+            p.gsrc[0] = (const T *)P->zs;
             launch_fused<2>(h, P, p);
         } else {
             launch_fused<0>(h, P, p);
@@ -1424,12 +1443,12 @@ struct Engine {
         q.first_tile = 0;
         q.total_tiles = P->ntiles_s;
         SegFE<T> &sf = q.fe[0];
-        sf.perm = f.kind == KIND_SORTED ? nullptr : f.iperm;
+        sf.perm = f.kind == KIND_SORTED ? h->ident : f.iperm;
         sf.cache = (const T *)f.cache_csr[pidx<T>()];
         sf.heads = f.heads;
         sf.wprefix = f.wprefix;
         sf.nsegs = std::max<int32_t>(P->seg_nsegs[0], 1);
-        sf.seg_level = P->seg_nsegs[0] == f.n ? nullptr : f.seg_level;
+        sf.seg_level = f.seg_level;
         sf.coef_off = 0;
         sf.tp = P->seg_tp[0];
         sf.tile0 = 0;

@@ -432,11 +432,11 @@ __global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
 // ---------------------------------------------------------------------------
 template <typename T>
 struct SegFE {
-    const int32_t *perm;      // CSR position -> internal observation; nullptr = identity
+    const int32_t *perm;      // CSR position -> internal observation (0..N-1 for the sorted FE)
     const T *cache;           // cache in CSR order
     const uint64_t *heads;    // segment head bits, 1 beyond N
     const uint32_t *wprefix;  // heads before each 64-position word
-    const int32_t *seg_level; // ordinal of non-empty segment -> level; nullptr = identity (no empty level)
+    const int32_t *seg_level; // ordinal of non-empty segment -> level
     int64_t coef_off;
     int64_t tp;               // positions per tile (a multiple of 256)
     int32_t tile0, ntiles;
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
                 const int64_t base = b0 + q * P;
                 const int64_t pos = base + psub;
                 const int64_t pc = pos < end ? pos : start;
-                in[q] = fe.perm ? __builtin_nontemporal_load(fe.perm + pc) : (int32_t)pc;
+                in[q] = __builtin_nontemporal_load(fe.perm + pc);
                 hwn[q] = hnn[q] = 0;
                 wpn[q] = 0;
                 if (base < end) {  // wave-uniform
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
                 const int64_t pos = base0 + q * P + psub;
                 uint32_t ord = wpn[q] + (uint32_t)__popcll(hw[q] & lanes_le((int)(pos & 63))) - 1u;
                 ord = ord < (uint32_t)fe.nsegs ? ord : (uint32_t)fe.nsegs - 1u;
-                lev[q] = fe.seg_level ? fe.seg_level[ord] : (int32_t)ord;
+                lev[q] = fe.seg_level[ord];
             }
             // unconditional loads (positions past the tile read its first
             // position): every load of the step is in flight together
@@ -751,7 +751,7 @@ struct FusedParams {
     int64_t dense_row0[kMaxG];   // its first dpart row
     const uint64_t *heads;       // FE s
     const uint32_t *wprefix;
-    const int32_t *seg_level;    // nullptr = identity (every FE s level observed)
+    const int32_t *seg_level;    // ordinal of an observed FE s level -> level (loads stay unconditional)
     int32_t nsegs_s;             // observed levels of FE s
     const T *coef;     // v
     T *target;         // u
@@ -793,6 +793,9 @@ __device__ __forceinline__ void sto(V *base, uint32_t i, V v) {
     *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(V)) = v;
 }
 
+// FE order (host-arranged): FE s is index 0; with GT > 0 and few-level FEs in
+// the pass, the first of them (dense_g[0]) is index GT-1 -- compile-time
+// positions, so no load sits behind a runtime FE test.
 // PM (projected iteration, FE s eliminated): 0 = plain pass; 1 = sums only:
 // the FE s segment sums of cache_s * (D*v over the other FEs), no u load,
 // store or norm, no dense sums; 2 = the u pass without FE s segment sums.
@@ -872,7 +875,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const int bit = (int)((ob + (uint32_t)psub) & 63u);
             uint32_t ord = wp[jw] + (uint32_t)__popcll(hw[jw] & lanes_le(bit)) - 1u;
             ord = ord < (uint32_t)p.nsegs_s ? ord : (uint32_t)p.nsegs_s - 1u;
-            return p.seg_level ? p.seg_level[ord] : (int32_t)ord;
+            return p.seg_level[ord];
         };
         auto compute = [&](const Step &st, uint32_t lb) {
             T un[U];
@@ -938,7 +941,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 for (int g = 0; g < GT; g++) {
                     const int32_t *rb = p.refs[g] + start;
 #pragma unroll
-                    for (int q = 0; q < U; q++) rr.r[g][q] = g == p.s ? 0 : ldo(rb, loc(lb, q));
+                    for (int q = 0; q < U; q++) rr.r[g][q] = g == 0 ? 0 : ldo(rb, loc(lb, q));
                 }
                 load_words_wp(lb, rr.hw, rr.wp);
             };
@@ -948,12 +951,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             };
             auto issue_data = [&](uint32_t lb, Refs &rr, InFlight &d) {
 #pragma unroll
-                for (int g = 0; g < GT; g++) {
-                    if (g == p.s) {
-#pragma unroll
-                        for (int q = 0; q < U; q++) rr.r[g][q] = level_s(lb, q, rr.hw, rr.wp);
-                    }
-                }
+                for (int q = 0; q < U; q++) rr.r[0][q] = level_s(lb, q, rr.hw, rr.wp);
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     const uint32_t i = loc(lb, q);
@@ -986,7 +984,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                     for (int g = 0; g < GT; g++)
 #pragma unroll
-                        for (int q = 0; q < U; q++) st.sum[q] += (PM == 1 && g == p.s) ? (T)0 : d.gv[g][q];
+                        for (int q = 0; q < U; q++) st.sum[q] += (PM == 1 && g == 0) ? (T)0 : d.gv[g][q];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
                         st.sum[q] = d.sw[q] * st.sum[q];
@@ -997,18 +995,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     for (int g = 0; g < GT; g++)
 #pragma unroll
                         for (int q = 0; q < U; q++)
-                            st.sum[q] += (PM == 1 && g == p.s) ? (T)0 : d.cv[g][q] * d.gv[g][q];
+                            st.sum[q] += (PM == 1 && g == 0) ? (T)0 : d.cv[g][q] * d.gv[g][q];
                 }
 #pragma unroll
                 for (int g = 0; g < GT; g++) {
-                    if (g == p.s) {
+                    if (g == 0) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
                             if constexpr (!FACT) st.cs[q] = d.cv[g][q];
                             st.rs[q] = rr.r[g][q];
                         }
                     }
-                    if (p.ndense && g == p.dense_g[0]) {
+                    if (g == GT - 1 && p.ndense) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
                             if constexpr (!FACT) st.cd[q] = d.cv[g][q];
