@@ -866,13 +866,19 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             P->slot0.push_back(P->nslot);
             P->nslot += (int32_t)h->fe[g].n;
         }
-    P->fused_dense = P->nslot > 0 && (size_t)P->nslot * kWave * accsz * kWavesPerBlock <= 65536 && fuse_dense_on(h);
+    // accumulators per slot and wave: one per lane, or one per column
+    // (dense_phased: FP64, K >= 16)
+    const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * sizeof(double) : (size_t)kWave * accsz;
+    // (k_dtx_dense then runs on the fused partition, 4 waves per workgroup,
+    // with lane-private accumulators: those must fit 64 KB as well)
+    P->fused_dense = P->nslot > 0 && (size_t)P->nslot * per_slot * kWavesPerBlock <= 65536 &&
+                     (size_t)maxlev * kWave * accsz * kWavesPerBlock <= 65536 && fuse_dense_on(h);
     P->dense_lds_wave = (int32_t)(accsz * kWave * maxlev);
     int64_t chunk;
     if (P->fused_dense) {
         chunk = P->R;
         P->dense_wpb = kWavesPerBlock;
-        P->fused_lds = (size_t)P->nslot * kWave * accsz * kWavesPerBlock;
+        P->fused_lds = (size_t)P->nslot * per_slot * kWavesPerBlock;
     } else {
         chunk = (N + 4095) / 4096;
         chunk = std::max<int64_t>(chunk, 16 * Pq);
@@ -974,15 +980,19 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     return 0;
 }
 
-// LDS of the fused pass: lane-private accumulators of the few-level FEs
-// when they fit 64 KB per workgroup (the rule get_plan applies)
-static size_t fused_lds_bytes(const fehip_solver *h, size_t ts) {
+// LDS of the fused pass: the few-level FEs' accumulators when they fit 64 KB
+// per workgroup (the rule get_plan applies)
+static size_t fused_lds_bytes(const fehip_solver *h, size_t ts, int K) {
     const size_t accsz = ts == 8 ? sizeof(double) : sizeof(F32x2);
+    const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * sizeof(double) : (size_t)kWave * accsz;
     int64_t nslot = 0;
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) nslot += h->fe[g].n;
-    const size_t b = (size_t)nslot * kWave * accsz * kWavesPerBlock;
-    return (b <= 65536 && fuse_dense_on(h)) ? b : 0;
+    int64_t maxlev = 1;
+    for (int g = 0; g < h->G; g++)
+        if (h->fe[g].kind == KIND_DENSE) maxlev = std::max<int64_t>(maxlev, h->fe[g].n);
+    const size_t b = (size_t)nslot * per_slot * kWavesPerBlock;
+    return (b <= 65536 && (size_t)maxlev * kWave * accsz * kWavesPerBlock <= 65536 && fuse_dense_on(h)) ? b : 0;
 }
 
 // resident waves of a kernel on this device (all CUs, kBlock threads)
@@ -1012,7 +1022,7 @@ static int get_plan_for(fehip_solver *h, Plan **out) {
                                  : (h->G == 3   ? (const void *)k_fused<T, Acc, K, 3, false>
                                     : h->G == 2 ? (const void *)k_fused<T, Acc, K, 2, false>
                                                 : (const void *)k_fused<T, Acc, K, 0, false>);
-    const int64_t wf = resident_waves(kf, fused_lds_bytes(h, sizeof(T)));
+    const int64_t wf = resident_waves(kf, fused_lds_bytes(h, sizeof(T), K));
     const int64_t wc = resident_waves((const void *)k_dtx_seg<T, Acc, K>, 0);
     return get_plan(h, K, sizeof(T), out, wf, wc);
 }
@@ -1353,16 +1363,17 @@ struct Engine {
     template <int PM>
     static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         const dim3 gr(P->fused_blocks), bl(kBlock);
+        const size_t lds = PM == 1 ? 0 : P->fused_lds;  // the sums-only pass has no few-level sums
         if (h->factored) {
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, PM>), gr, bl, P->fused_lds, h->stream, p);
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, PM>), gr, bl, lds, h->stream, p);
             else if (h->G == 2)
-                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, PM>), gr, bl, P->fused_lds, h->stream, p);
-            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, true, PM>), gr, bl, P->fused_lds, h->stream, p);
+                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, PM>), gr, bl, lds, h->stream, p);
+            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, true, PM>), gr, bl, lds, h->stream, p);
         } else {
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, false, PM>), gr, bl, P->fused_lds, h->stream, p);
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, false, PM>), gr, bl, lds, h->stream, p);
             else if (h->G == 2)
-                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, false, PM>), gr, bl, P->fused_lds, h->stream, p);
-            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, false, PM>), gr, bl, P->fused_lds, h->stream, p);
+                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, false, PM>), gr, bl, lds, h->stream, p);
+            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, false, PM>), gr, bl, lds, h->stream, p);
         }
     }
 
@@ -1656,7 +1667,7 @@ struct Engine {
         Plan *P = nullptr;
         CHK((get_plan_for<T, Acc, K>(h, &P)));
         const int64_t wf = P->waves_f, wc = P->waves_c;
-        const bool dense_choice = P->nslot > 0 && fused_lds_bytes(h, sizeof(T)) > 0;
+        const bool dense_choice = P->nslot > 0 && fused_lds_bytes(h, sizeof(T), K) > 0;
         auto measure = [&](int64_t ts, int64_t tc, int fd, double &ms) -> int {
             h->sched_tiles_s = ts;
             h->sched_tiles_c = tc;

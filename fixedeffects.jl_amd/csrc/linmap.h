@@ -16,6 +16,8 @@
 // results are bitwise reproducible run to run.
 #pragma once
 
+#include <type_traits>
+
 #include "fehip_common.h"
 
 namespace fehip {
@@ -793,6 +795,14 @@ __device__ __forceinline__ void sto(V *base, uint32_t i, V v) {
     *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(V)) = v;
 }
 
+// Few-level FE accumulators of the fused pass: per level and column, added to
+// in ordered phases, for FP64 batches of 16 or more columns (P <= 4 phases);
+// lane-private otherwise.  The host sizes the LDS with the same rule.
+template <int K, typename Acc>
+__host__ __device__ constexpr bool dense_phased() {
+    return K >= 16 && sizeof(Acc) == sizeof(double) && !std::is_same<Acc, F32x2>::value;
+}
+
 // FE order (host-arranged): FE s is index 0; with GT > 0 and few-level FEs in
 // the pass, the first of them (dense_g[0]) is index GT-1 -- compile-time
 // positions, so no load sits behind a runtime FE test.
@@ -819,8 +829,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const bool on = p.ops->on[c] != 0;
     const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
     const bool keep = p.ops->b[c] != 0.0;
-    Acc *acc = reinterpret_cast<Acc *>(fused_smem) + (size_t)wave * p.nslot * kWave;
-    for (int k = 0; k < p.nslot; k++) acc[k * kWave + lane] = Acc(0.f);
+    // few-level FE sums: lane-private accumulators (nslot x 64 per wave), or
+    // for wide batches in FP64 (dense_phased) one accumulator per level and
+    // column (nslot x K per wave) that the P lanes of a column add to in P
+    // ordered phases with LDS adds -- a quarter to a half of the LDS
+    constexpr bool PHASED = dense_phased<K, Acc>();
+    Acc *acc = reinterpret_cast<Acc *>(fused_smem) + (size_t)wave * p.nslot * (PHASED ? K : kWave);
+    if constexpr (PHASED) {
+        for (int e = lane; e < p.nslot * K; e += kWave) acc[e] = Acc(0.f);
+    } else {
+        for (int k = 0; k < p.nslot; k++) acc[k * kWave + lane] = Acc(0.f);
+    }
+    auto dense_add = [&](const int32_t (&sl)[U], const Acc (&x)[U]) {
+        if constexpr (PHASED) {
+#pragma unroll
+            for (int q = 0; q < U; q++)
+#pragma unroll
+                for (int ph = 0; ph < P; ph++)
+                    if (psub == ph) atomicAdd(reinterpret_cast<double *>(acc) + sl[q] * K + c, (double)x[q]);
+        } else {
+            lds_accumulate<U, Acc>(acc, sl, x, lane);
+        }
+    };
     const int t = blockIdx.x * kWavesPerBlock + wave;
     double sq = 0.0;
     if (t < p.ntiles) {  // wave-uniform
@@ -913,7 +943,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     sl[q] = s0 + st.rd[q];
                     x[q] = mul_acc<Acc, T>(st.cd[q], un[q]);
                 }
-                lds_accumulate<U, Acc>(acc, sl, x, lane);
+                dense_add(sl, x);
                 for (int d = 1; d < p.ndense; d++) {
                     const int32_t *rf = p.refs[p.dense_g[d]] + start;
                     const T *cf = p.cache[p.dense_g[d]] + start;
@@ -924,7 +954,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         sl[q] = sd + ldo(rf, i);
                         x[q] = mul_acc<Acc, T>(FACT ? st.cs[q] : ldo(cf, i), un[q]);
                     }
-                    lds_accumulate<U, Acc>(acc, sl, x, lane);
+                    dense_add(sl, x);
                 }
             }
         };
@@ -1098,14 +1128,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (PM != 1 && p.ndense) {
         // per-wave slot sums (written in place, see k_dtx_dense), then the
         // waves of the workgroup in order -> one dpart row per workgroup
-        double *ws = reinterpret_cast<double *>(acc);
-        for (int k = 0; k < p.nslot; k++) {
-            Acc x = acc[k * kWave + lane];
-            x = column_allreduce<K>(x);
-            if (psub == 0) ws[k * K + c] = acc_to_double(x);
+        if constexpr (!PHASED) {
+            double *ws = reinterpret_cast<double *>(acc);
+            for (int k = 0; k < p.nslot; k++) {
+                Acc x = acc[k * kWave + lane];
+                x = column_allreduce<K>(x);
+                if (psub == 0) ws[k * K + c] = acc_to_double(x);
+            }
         }
         __syncthreads();
-        const size_t region = (size_t)p.nslot * kWave * sizeof(Acc) / sizeof(double);
+        const size_t region = (size_t)p.nslot * (PHASED ? K : kWave) * sizeof(Acc) / sizeof(double);
         const double *base_ws = reinterpret_cast<const double *>(fused_smem);
         for (int e = threadIdx.x; e < p.nslot * K; e += blockDim.x) {
             double s = 0.0;
