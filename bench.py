@@ -45,8 +45,8 @@ def log(*a):
 # (scaled by a plain device copy of known size), by (workload, precision):
 # profiles/worker_firm_1e7_fp64_k1.md
 PMC_TRAFFIC = {
-    ("worker_firm_1e7", "fp64"): {"k_fused": 408.3e6, "k_dtx_csr": 352.7e6, "k_update": 88.1e6,
-                                  "k_dtx_fixup": 35.2e6},
+    ("worker_firm_1e7", "fp64"): {"k_fused": 369.5e6, "k_dtx_csr": 353.2e6, "k_update": 88.1e6,
+                                  "k_dtx_fixup": 25.3e6},
 }
 
 
@@ -60,10 +60,11 @@ def kernel_bytes(N, G, n, kinds, K, s, factored, weighted):
     ndense_lev = sum(k for k, kd in zip(n, kinds) if kd == "dense")
     fused_dense = 0 < ndense_lev <= 32
     out = {}
-    # fused pass: refs of every FE, caches (or sqrt(w)), u read + write, each
-    # coefficient once, FE s head bits, raw FE s sums (double) out
-    per_obs = N * G * (4 + s) if not factored else N * G * 4 + (N * s if weighted else 0)
-    out["k_fused"] = per_obs + K * N * 2 * s + K * p * s + N // 8 + 8 * K * n[gs]
+    # fused pass: refs of every FE but s (whose level comes from its head bits
+    # and their word prefixes), caches (or sqrt(w)), u read + write, each
+    # coefficient once, raw FE s sums (double) out
+    per_obs = N * G * s + N * (G - 1) * 4 if not factored else N * (G - 1) * 4 + (N * s if weighted else 0)
+    out["k_fused"] = per_obs + K * N * 2 * s + K * p * s + N // 8 + N // 16 + 8 * K * n[gs]
     csr = 0
     dense = 0
     for g in range(G):
