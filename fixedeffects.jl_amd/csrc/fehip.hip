@@ -1113,6 +1113,13 @@ __global__ void k_copy(const T *__restrict__ a, T *__restrict__ b, int64_t n) {
     if (e < n) b[e] = a[e];
 }
 
+// y[i*K + c] *= d[i]
+template <typename T>
+__global__ void k_scale_rows(T *__restrict__ y, const T *__restrict__ d, int64_t N, int K) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < N * K) y[e] = y[e] * d[e / K];
+}
+
 template <typename T>
 __global__ void k_square(const T *__restrict__ a, int64_t n, T *__restrict__ b) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1326,8 +1333,10 @@ struct Engine {
 
     // coef <- b*coef + a*A'y : segmented tiles, few-level chunks, fix-up.
     // csr_only: the tiles of the CSR FEs only (FE s is done by k_fused).
+    // raw: y is the iteration's stored sqrt(w) .* u (factored form): sums of y
+    // as it is, scaled per level (k_dtx_seg RAW)
     static void dtx_seg(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
-                        const int32_t *early_exit, bool csr_only = false) {
+                        const int32_t *early_exit, bool csr_only = false, bool raw = false) {
         if (!P->nseg || (csr_only && !P->csr_blocks)) return;
         DtxSegParams<T> p{};
         p.N = h->N;
@@ -1343,6 +1352,7 @@ struct Engine {
             s.wprefix = f.wprefix;
             s.nsegs = std::max<int32_t>(P->seg_nsegs[q], 1);
             s.seg_level = f.seg_level;
+            s.lscale = (const T *)f.scale_T[pidx<T>()];
             s.coef_off = h->off[P->seg_fe[q]];
             s.tp = P->seg_tp[q];
             s.tile0 = P->tile0[q];
@@ -1356,8 +1366,9 @@ struct Engine {
         p.partials = P->part_seg;
         p.all_done = early_exit;
         p.mode = mode;
-        hipLaunchKernelGGL((k_dtx_seg<T, Acc, K>), dim3(csr_only ? P->csr_blocks : P->seg_blocks), dim3(kBlock), 0,
-                           h->stream, p);
+        const dim3 gr(csr_only ? P->csr_blocks : P->seg_blocks);
+        if (raw) hipLaunchKernelGGL((k_dtx_seg<T, Acc, K, true>), gr, dim3(kBlock), 0, h->stream, p);
+        else hipLaunchKernelGGL((k_dtx_seg<T, Acc, K>), gr, dim3(kBlock), 0, h->stream, p);
     }
 
     template <int PM>
@@ -1527,7 +1538,8 @@ struct Engine {
         hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
     }
 
-    static void dtx_dense(fehip_solver *h, Plan *P, const ColOps *ops, const T *y, const int32_t *early_exit) {
+    static void dtx_dense(fehip_solver *h, Plan *P, const ColOps *ops, const T *y, const int32_t *early_exit,
+                          bool raw = false) {
         if (!P->ndense) return;
         DtxDenseParams<T> p{};
         p.N = h->N;
@@ -1549,8 +1561,12 @@ struct Engine {
         p.ops = ops;
         p.all_done = early_exit;
         const size_t lds = (size_t)P->dense_lds_wave * P->dense_wpb;
-        hipLaunchKernelGGL((k_dtx_dense<T, Acc, K>), dim3(P->dense_blocks), dim3(kWave * P->dense_wpb), lds,
-                           h->stream, p);
+        if (raw)
+            hipLaunchKernelGGL((k_dtx_dense<T, Acc, K, true>), dim3(P->dense_blocks), dim3(kWave * P->dense_wpb), lds,
+                               h->stream, p);
+        else
+            hipLaunchKernelGGL((k_dtx_dense<T, Acc, K>), dim3(P->dense_blocks), dim3(kWave * P->dense_wpb), lds,
+                               h->stream, p);
     }
 
     // proj: the projected iteration's form -- the items of the FEs other
@@ -1577,9 +1593,10 @@ struct Engine {
                 f.s = h->s;
             }
             if (h->factored) {
-                if (!proj) f.fscale[h->s] = (const T *)h->fe[h->s].scale_T[pidx<T>()];
-                if (P->fused_dense)
-                    for (int g : P->dense_fe) f.fscale[g] = (const T *)h->fe[g].scale_T[pidx<T>()];
+                // the iteration's sums are of sqrt(w) .* u for every FE: they
+                // take their scale here
+                for (int g = 0; g < h->G; g++)
+                    if (g != h->s || !proj) f.fscale[g] = (const T *)h->fe[g].scale_T[pidx<T>()];
             }
         }
         hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
@@ -1626,10 +1643,10 @@ struct Engine {
         out_ms[0] = timed([&] { fused(h, P, P->ops_once, (const T *)P->v, (T *)P->u); });
         out_ms[1] = timed([&] { fin_db(h, P, MODE_ITER, nullptr, true); });
         out_ms[2] = P->csr_blocks
-                        ? timed([&] { dtx_seg(h, P, MODE_ITER, P->ops_once, (const T *)P->u, (T *)P->v, nullptr, true); })
+                        ? timed([&] { dtx_seg(h, P, MODE_ITER, P->ops_once, (const T *)P->u, (T *)P->v, nullptr, true, h->factored); })
                         : 0.0;
         out_ms[3] = (P->ndense && !P->fused_dense)
-                        ? timed([&] { dtx_dense(h, P, P->ops_once, (const T *)P->u, nullptr); })
+                        ? timed([&] { dtx_dense(h, P, P->ops_once, (const T *)P->u, nullptr, h->factored); })
                         : 0.0;
         out_ms[4] = timed([&] { dtx_fixup(h, P, MODE_ITER, P->ops_once, (T *)P->v, nullptr, true); });
         out_ms[5] = timed([&] { fin_dtx(h, P, MODE_ITER, nullptr, true); });
@@ -1742,8 +1759,8 @@ struct Engine {
             fix_s(h, P);
             fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u, 2);
             fin_db(h, P, MODE_ITER, P->all_done, true);
-            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done);
-            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true);
+            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, h->factored);
+            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, h->factored);
             dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, true);
             fin_dtx(h, P, MODE_ITER, P->all_done, true);
             update(h, P);
@@ -1752,8 +1769,8 @@ struct Engine {
         }
         fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
         fin_db(h, P, MODE_ITER, P->all_done, true);
-        if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done);
-        dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true);
+        if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, h->factored);
+        dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, h->factored);
         dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
         fin_dtx(h, P, MODE_ITER, P->all_done, true);
         update(h, P);
@@ -1829,6 +1846,10 @@ struct Engine {
         hipLaunchKernelGGL((k_init_vectors<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, h->p * K,
                            (T *)P->v, (T *)P->hh, (T *)P->hbar, (const void *)P->state,
                            (const T *)h->scale_all[pidx<T>()], (T *)P->vt);
+        // the factored iteration keeps u as sqrt(w) .* u
+        if (h->factored && h->sqrtw_T[pidx<T>()])
+            hipLaunchKernelGGL(k_scale_rows<T>, dim3(grid_for(h->N * K, kBlock)), dim3(kBlock), 0, h->stream,
+                               (T *)P->u, (const T *)h->sqrtw_T[pidx<T>()], h->N, K);
         HIPCHK(hipGetLastError());
         CHK(ensure_graph(h, P));
         // Two replays in flight: while the host waits on replay i's copy of

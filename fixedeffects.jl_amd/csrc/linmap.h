@@ -439,6 +439,7 @@ struct SegFE {
     const uint64_t *heads;    // segment head bits, 1 beyond N
     const uint32_t *wprefix;  // heads before each 64-position word
     const int32_t *seg_level; // ordinal of non-empty segment -> level
+    const T *lscale;          // RAW form: per-level scale applied to each level's sum (cache not read)
     int64_t coef_off;
     int64_t tp;               // positions per tile (a multiple of 256)
     int32_t tile0, ntiles;
@@ -532,7 +533,11 @@ __device__ __forceinline__ int32_t seg_level_of(uint32_t wp, const int32_t *seg_
     return seg_level[wp + (uint32_t)__popcll(w & lanes_le((int)(pos & 63))) - 1u];
 }
 
-template <typename T, typename Acc, int K>
+// RAW (the factored form's iteration, where u is stored as sqrt(w) .* u):
+// the segment sums are of the stored vector as it is, and each level's sum
+// takes the FE's scale (lscale; crossing segments take it in the fix-up), so
+// the CSR-ordered cache is not read.
+template <typename T, typename Acc, int K, bool RAW = false>
 __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
     if (p.all_done && *p.all_done) return;
     constexpr int P = kWave / K;
@@ -599,7 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
             for (int q = 0; q < U; q++) {
                 const int64_t pos = base0 + q * P + psub;
                 const int64_t pc = pos < end ? pos : start;
-                cv[q] = __builtin_nontemporal_load(fe.cache + pc);
+                cv[q] = RAW ? fe.lscale[lev[q]] : __builtin_nontemporal_load(fe.cache + pc);
                 uv[q] = p.u[(int64_t)iq[q] * K + c];
                 ov[q] = v[(int64_t)lev[q] * K + c];
             }
@@ -608,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const int64_t pos = base0 + q * P + psub;
-                const Acc pr = mul_acc<Acc, T>(cv[q], uv[q]);
+                const Acc pr = RAW ? Acc(uv[q]) : mul_acc<Acc, T>(cv[q], uv[q]);
                 val[q] = (pos < end && on) ? pr : Acc(0.f);
             }
             // every em[q] is defined before the scan, which stops at the
@@ -626,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 if (em[q]) {
-                    T out = a * acc_to_T<T, Acc>(val[q]);
+                    T out = a * (RAW ? cv[q] * acc_to_T<T, Acc>(val[q]) : acc_to_T<T, Acc>(val[q]));
                     if (keep) out = b * ov[q] + out;
                     v[(int64_t)lev[q] * K + c] = out;
                     sq += (double)out * (double)out;
@@ -670,7 +675,8 @@ struct DtxDenseParams {
     const int32_t *all_done;
 };
 
-template <typename T, typename Acc, int K>
+// RAW: sums of the stored sqrt(w) .* u without the cache (the fix-up scales them)
+template <typename T, typename Acc, int K, bool RAW = false>
 __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
     if (p.all_done && *p.all_done) return;
     extern __shared__ __align__(16) unsigned char dense_smem[];
@@ -697,9 +703,13 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
                 const int64_t pos = base0 + q * P + psub;
                 const int64_t pc = pos < end ? pos : start;
                 r[q] = __builtin_nontemporal_load(fe.refs + pc);
-                const T cv = __builtin_nontemporal_load(fe.cache + pc);
                 const T uv = p.u[pc * K + c];
-                x[q] = pos < end ? mul_acc<Acc, T>(cv, uv) : Acc(0.f);
+                if constexpr (RAW) {
+                    x[q] = pos < end ? Acc(uv) : Acc(0.f);
+                } else {
+                    const T cv = __builtin_nontemporal_load(fe.cache + pc);
+                    x[q] = pos < end ? mul_acc<Acc, T>(cv, uv) : Acc(0.f);
+                }
             }
             lds_accumulate<U, Acc>(acc, r, x, lane);
         }
@@ -914,10 +924,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const uint32_t o = lb + (uint32_t)(q * P + psub);
                 const bool ok = on && o < len;
                 T out = a * st.sum[q];
-                if constexpr (PM != 1) out = keep ? b * st.uo[q] + out : out;
+                // FACT: the stored vector is sqrt(w) .* u (st.cs = sqrt(w)),
+                // which the D'u passes then sum without caches
+                if constexpr (PM != 1) out = keep ? b * (FACT ? st.uo[q] / st.cs[q] : st.uo[q]) + out : out;
                 un[q] = ok ? out : (T)0;
                 if constexpr (PM != 1) {
-                    if (ok) sto(ub, o * K + c, out);
+                    if (ok) sto(ub, o * K + c, FACT ? st.cs[q] * out : out);
                     sq += (double)un[q] * (double)un[q];
                 }
             }
