@@ -1371,6 +1371,9 @@ struct Engine {
         else hipLaunchKernelGGL((k_dtx_seg<T, Acc, K>), gr, dim3(kBlock), 0, h->stream, p);
     }
 
+    // u stored as sqrt(w) .* u in this engine's iteration (k_fused's rule)
+    static bool ut(const fehip_solver *h) { return h->factored && stores_scaled_u<K, true>(); }
+
     template <int PM>
     static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         const dim3 gr(P->fused_blocks), bl(kBlock);
@@ -1444,6 +1447,8 @@ struct Engine {
         if (pm == 1) {
             p.ndense = 0;
             p.nslot = 0;
+            p.zsw = (T *)P->zs;
+            p.gsrc[0] = (const T *)(h->factored ? h->scale2_T[pidx<T>()] : h->fe[h->s].scale_T[pidx<T>()]);
             launch_fused<1>(h, P, p);
         } else if (pm == 2) {
             p.gsrc[0] = (const T *)P->zs;
@@ -1513,10 +1518,11 @@ struct Engine {
         hipLaunchKernelGGL((k_db<T, K>), dim3(P->db_blocks), dim3(kBlock), 0, h->stream, d);
     }
 
-    // projected iteration, after the sums-only fused pass: zs = -A_s' t for
-    // t = A_o v (FE s sums completed from the tiles; under the factored form
-    // the raw sums of sqrt(w)*t take scale_s^2, the FE s scale once for A_s'
-    // and once for the A_s that the second fused pass applies to zs)
+    // projected iteration, after the sums-only fused pass (which wrote zs for
+    // the FE s levels inside a tile): zs = -A_s' t, t = A_o v, for the levels
+    // whose segment crosses tiles.  Under the factored form the raw sums of
+    // sqrt(w)*t take scale_s^2: the FE s scale once for A_s' and once for the
+    // A_s that the second fused pass applies to zs.
     static void fix_s(fehip_solver *h, Plan *P) {
         FixParams<T> f{};
         f.nitems = P->nitems_s;
@@ -1529,10 +1535,6 @@ struct Engine {
         f.partials = P->part_fix;
         f.all_done = P->all_done;
         f.mode = MODE_NONE;
-        f.wsum = P->wsum;
-        f.sdirect = P->sdirect;
-        f.ns = h->n[h->s];
-        f.s_off = 0;
         f.s = h->s;
         if (h->factored) f.fscale[h->s] = (const T *)h->scale2_T[pidx<T>()];
         hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
@@ -1593,10 +1595,14 @@ struct Engine {
                 f.s = h->s;
             }
             if (h->factored) {
-                // the iteration's sums are of sqrt(w) .* u for every FE: they
-                // take their scale here
-                for (int g = 0; g < h->G; g++)
-                    if (g != h->s || !proj) f.fscale[g] = (const T *)h->fe[g].scale_T[pidx<T>()];
+                // raw sums of sqrt(w) .* u take their scale here: FE s's and the
+                // fused few-level FEs' always, the CSR FEs' and the separate
+                // few-level pass's when u is stored scaled
+                for (int g = 0; g < h->G; g++) {
+                    const bool dense = h->fe[g].kind == KIND_DENSE;
+                    const bool raw = g == h->s ? !proj : (dense ? (P->fused_dense || ut(h)) : ut(h));
+                    if (raw) f.fscale[g] = (const T *)h->fe[g].scale_T[pidx<T>()];
+                }
             }
         }
         hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
@@ -1643,10 +1649,10 @@ struct Engine {
         out_ms[0] = timed([&] { fused(h, P, P->ops_once, (const T *)P->v, (T *)P->u); });
         out_ms[1] = timed([&] { fin_db(h, P, MODE_ITER, nullptr, true); });
         out_ms[2] = P->csr_blocks
-                        ? timed([&] { dtx_seg(h, P, MODE_ITER, P->ops_once, (const T *)P->u, (T *)P->v, nullptr, true, h->factored); })
+                        ? timed([&] { dtx_seg(h, P, MODE_ITER, P->ops_once, (const T *)P->u, (T *)P->v, nullptr, true, ut(h)); })
                         : 0.0;
         out_ms[3] = (P->ndense && !P->fused_dense)
-                        ? timed([&] { dtx_dense(h, P, P->ops_once, (const T *)P->u, nullptr, h->factored); })
+                        ? timed([&] { dtx_dense(h, P, P->ops_once, (const T *)P->u, nullptr, ut(h)); })
                         : 0.0;
         out_ms[4] = timed([&] { dtx_fixup(h, P, MODE_ITER, P->ops_once, (T *)P->v, nullptr, true); });
         out_ms[5] = timed([&] { fin_dtx(h, P, MODE_ITER, nullptr, true); });
@@ -1759,8 +1765,8 @@ struct Engine {
             fix_s(h, P);
             fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u, 2);
             fin_db(h, P, MODE_ITER, P->all_done, true);
-            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, h->factored);
-            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, h->factored);
+            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
+            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
             dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, true);
             fin_dtx(h, P, MODE_ITER, P->all_done, true);
             update(h, P);
@@ -1769,8 +1775,8 @@ struct Engine {
         }
         fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
         fin_db(h, P, MODE_ITER, P->all_done, true);
-        if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, h->factored);
-        dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, h->factored);
+        if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
+        dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
         dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
         fin_dtx(h, P, MODE_ITER, P->all_done, true);
         update(h, P);
@@ -1846,8 +1852,8 @@ struct Engine {
         hipLaunchKernelGGL((k_init_vectors<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, h->p * K,
                            (T *)P->v, (T *)P->hh, (T *)P->hbar, (const void *)P->state,
                            (const T *)h->scale_all[pidx<T>()], (T *)P->vt);
-        // the factored iteration keeps u as sqrt(w) .* u
-        if (h->factored && h->sqrtw_T[pidx<T>()])
+        // the factored iteration with K < 8 keeps u as sqrt(w) .* u
+        if (ut(h) && h->sqrtw_T[pidx<T>()])
             hipLaunchKernelGGL(k_scale_rows<T>, dim3(grid_for(h->N * K, kBlock)), dim3(kBlock), 0, h->stream,
                                (T *)P->u, (const T *)h->sqrtw_T[pidx<T>()], h->N, K);
         HIPCHK(hipGetLastError());

@@ -771,6 +771,10 @@ struct FusedParams {
     // from the FE's first level); the projected iteration points FE s at
     // its correction vector instead
     const T *gsrc[kMaxG];
+    // PM = 1: the FE s correction -scale_s^2 * sum (factored) or -sum of each
+    // segment inside a tile goes straight to zsw (levels of FE s x K);
+    // gsrc[0] is then the per-level scale_s^2 (read at the level, not x K)
+    T *zsw;
     // FACT: cache_g = sqrt(w) * scale_g[ref] for every FE (no interactions),
     // so D*v = sqrt(w) * sum_g vt_g[ref_g] with vt = scale .* v (kept by the
     // vector update) and the raw D'u sums are of sqrt(w)*u (scaled in the
@@ -813,6 +817,17 @@ __host__ __device__ constexpr bool dense_phased() {
     return K >= 16 && sizeof(Acc) == sizeof(double) && !std::is_same<Acc, F32x2>::value;
 }
 
+// The factored iteration with fewer than 8 columns stores u as sqrt(w) .* u:
+// the CSR and few-level D'u passes then read no cache (80 MB per column at
+// N = 1e7).  Wider batches keep u (their fused pass is VALU-bound, and the
+// division back to u would cost more than the cache read saves).  The host
+// applies the same rule (raw D'u forms, fix-up scales, the conversion at
+// LSMR start).
+template <int K, bool FACT>
+__host__ __device__ constexpr bool stores_scaled_u() {
+    return FACT && K < 8;
+}
+
 // FE order (host-arranged): FE s is index 0; with GT > 0 and few-level FEs in
 // the pass, the first of them (dense_g[0]) is index GT-1 -- compile-time
 // positions, so no load sits behind a runtime FE test.
@@ -844,6 +859,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // column (nslot x K per wave) that the P lanes of a column add to in P
     // ordered phases with LDS adds -- a quarter to a half of the LDS
     constexpr bool PHASED = dense_phased<K, Acc>();
+    constexpr bool UT = stores_scaled_u<K, FACT>();
     Acc *acc = reinterpret_cast<Acc *>(fused_smem) + (size_t)wave * p.nslot * (PHASED ? K : kWave);
     if constexpr (PHASED) {
         for (int e = lane; e < p.nslot * K; e += kWave) acc[e] = Acc(0.f);
@@ -874,7 +890,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         // one loop step, ready to compute: D*v sums, old u, FE s and first
         // dense FE cache/ref, head words
         struct Step {
-            T sum[U], uo[U], cs[U], cd[U];
+            T sum[U], uo[U], cs[U], cd[U], g0[U];
             int32_t rs[U], rd[U];
             uint64_t hw[NW];
         };
@@ -924,12 +940,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 const uint32_t o = lb + (uint32_t)(q * P + psub);
                 const bool ok = on && o < len;
                 T out = a * st.sum[q];
-                // FACT: the stored vector is sqrt(w) .* u (st.cs = sqrt(w)),
+                // UT: the stored vector is sqrt(w) .* u (st.cs = sqrt(w)),
                 // which the D'u passes then sum without caches
-                if constexpr (PM != 1) out = keep ? b * (FACT ? st.uo[q] / st.cs[q] : st.uo[q]) + out : out;
+                if constexpr (PM != 1) out = keep ? b * (UT ? st.uo[q] / st.cs[q] : st.uo[q]) + out : out;
                 un[q] = ok ? out : (T)0;
                 if constexpr (PM != 1) {
-                    if (ok) sto(ub, o * K + c, FACT ? st.cs[q] * out : out);
+                    if (ok) sto(ub, o * K + c, UT ? st.cs[q] * out : out);
                     sq += (double)un[q] * (double)un[q];
                 }
             }
@@ -942,8 +958,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 Acc x = mul_acc<Acc, T>(st.cs[q], un[q]);
                 const int jw = (q * P) / kWave;  // constant after unrolling
                 if (seg_step<K>(st.hw[jw], st.hw[jw + 1], (int)(ob & 63u), ob + (uint32_t)psub < len, psub, c, on, x,
-                                cur, &p.tileA[(int64_t)t * K + c]))
-                    sto(p.wsum, (uint32_t)st.rs[q] * K + c, acc_to_double(x));
+                                cur, &p.tileA[(int64_t)t * K + c])) {
+                    if constexpr (PM == 1)
+                        sto(p.zsw, (uint32_t)st.rs[q] * K + c, (T)(-(FACT ? (double)st.g0[q] : 1.0) * acc_to_double(x)));
+                    else
+                        sto(p.wsum, (uint32_t)st.rs[q] * K + c, acc_to_double(x));
+                }
             }
             // dense FEs: lane-private LDS accumulators
             if (PM != 1 && p.ndense) {
@@ -1008,7 +1028,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     const T *cf = p.gsrc[g];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        d.gv[g][q] = ldo(cf, (uint32_t)rr.r[g][q] * K + c);
+                        d.gv[g][q] = (PM == 1 && g == 0) ? ldo(cf, (uint32_t)rr.r[g][q])
+                                                         : ldo(cf, (uint32_t)rr.r[g][q] * K + c);
                         if constexpr (!FACT) d.cv[g][q] = ldo(p.cache[g] + start, loc(lb, q));
                     }
                 }
@@ -1018,6 +1039,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     st.sum[q] = (T)0;
+                    st.g0[q] = d.gv[0][q];
                     st.uo[q] = d.uo[q];
                     st.cs[q] = st.cd[q] = (T)0;
                     st.rs[q] = st.rd[q] = 0;
@@ -1080,6 +1102,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     st.sum[q] = (T)0;
+                    st.g0[q] = (T)0;
                     st.uo[q] = PM == 1 ? (T)0 : ldo(ub, loc(lb, q) * K + c);
                     st.cs[q] = st.cd[q] = (T)0;
                     st.rs[q] = st.rd[q] = 0;
@@ -1103,6 +1126,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         cv[q] = FACT ? (T)1 : ldo(p.cache[g] + start, loc(lb, q));
                     }
                     const T *cf = p.gsrc[g];
+                    if (PM == 1 && g == p.s) {
+#pragma unroll
+                        for (int q = 0; q < U; q++) st.g0[q] = ldo(cf, (uint32_t)r[q]);
+                    }
                     if (PM != 1 || g != p.s) {
 #pragma unroll
                         for (int q = 0; q < U; q++) {
