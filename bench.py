@@ -61,9 +61,11 @@ def kernel_bytes(N, G, n, kinds, K, s, factored, weighted):
     fused_dense = 0 < ndense_lev <= 32
     out = {}
     # fused pass: refs of every FE but s (whose level comes from its head bits
-    # and their word prefixes), caches (or sqrt(w)), u read + write, each
+    # and their word prefixes; with G = 3 and one few-level FE the other two
+    # share one packed stream), caches (or sqrt(w)), u read + write, each
     # coefficient once, raw FE s sums (double) out
-    per_obs = N * G * s + N * (G - 1) * 4 if not factored else N * (G - 1) * 4 + (N * s if weighted else 0)
+    nref = G - 2 if (G == 3 and kinds.count("dense") == 1 and fused_dense) else G - 1
+    per_obs = N * G * s + N * nref * 4 if not factored else N * nref * 4 + (N * s if weighted else 0)
     out["k_fused"] = per_obs + K * N * 2 * s + K * p * s + N // 8 + N // 16 + 8 * K * n[gs]
     csr = 0
     dense = 0

@@ -105,6 +105,11 @@ struct fehip_solver {
     std::vector<FeInfo> fe;
     int32_t *perm_s = nullptr;  // internal position -> caller's observation
     int32_t *ident = nullptr;   // 0..N-1: the "permutation" of the sorted FE (kernels load it unconditionally)
+    // the fused pass's last two FEs in one ref stream (G = 3): the first
+    // few-level FE (pack_d, <= 64 levels) in the top 6 bits over the other
+    // FE (pack_g, < 2^26 levels); nullptr when the shape does not allow it
+    int32_t *packed = nullptr;
+    int pack_g = -1, pack_d = -1;
     double *w_orig = nullptr;   // caller's order (nullptr: UnitWeights)
     double *sqrtw_int = nullptr;
     // no FE has an interaction: cache_g = sqrt(w) * scale_g[ref], and the
@@ -399,6 +404,13 @@ __global__ void k_cache(const int32_t *__restrict__ perm_s, int64_t N, const int
     const double zi = z ? z[i] : 1.0;
     const double wi = w ? w[i] : 1.0;
     cache[j] = (T)((zi * sqrt(wi)) * (double)scale[ref_int[j]]);
+}
+
+// packed[i] = lo[i] | hi[i] << 26 (lo < 2^26, hi < 64)
+__global__ void k_pack_refs(const int32_t *__restrict__ lo, const int32_t *__restrict__ hi, int64_t n,
+                            int32_t *__restrict__ packed) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) packed[i] = (int32_t)((uint32_t)lo[i] | ((uint32_t)hi[i] << 26));
 }
 
 __global__ void k_iota(int32_t *__restrict__ a, int64_t n) {
@@ -715,6 +727,23 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
         HIPCHK(hipMemcpy(o.data(), f.ooff, sizeof(int32_t) * (f.n + 1), hipMemcpyDeviceToHost));
         for (int64_t k = 0; k < f.n; k++)
             if (o[k + 1] - o[k] > kScaleBig) f.big_levels.push_back((int32_t)k);
+    }
+    if (G == 3) {
+        int d = -1, g2 = -1;
+        for (int g = 0; g < G; g++)
+            if (h->fe[g].kind == KIND_DENSE && d < 0) d = g;
+        for (int g = 0; g < G; g++)
+            if (g != h->s && g != d) g2 = g;
+        // (g2 not few-level itself: the fused pass reads a second few-level
+        // FE's refs directly)
+        if (d >= 0 && g2 >= 0 && h->fe[g2].kind != KIND_DENSE && h->n[d] <= 64 && h->n[g2] <= (int64_t(1) << 26)) {
+            CHK(dalloc(h, &h->packed, N));
+            hipLaunchKernelGGL(k_pack_refs, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
+                               (const int32_t *)h->fe[g2].ref_int, (const int32_t *)h->fe[d].ref_int, N, h->packed);
+            HIPCHK(hipGetLastError());
+            h->pack_g = g2;
+            h->pack_d = d;
+        }
     }
     CHK(do_update_weights(h, weights));
     cleanup.armed = false;
@@ -1418,6 +1447,12 @@ struct Engine {
             p.cache[k] = (const T *)h->fe[g].cache_int[pidx<T>()];
             p.coef_off[k] = h->off[g];
             p.gsrc[k] = (h->factored ? (const T *)P->vt : coef) + h->off[g] * K;
+        }
+        // one ref stream for the last two FEs when they are the packed pair
+        // (the kernel reads it twice at the same addresses and unpacks)
+        if (dense_in && h->G == 3 && h->packed && kord[1] == h->pack_g && kord[2] == h->pack_d) {
+            p.refs[1] = p.refs[2] = h->packed;
+            p.packed = 1;
         }
         if (dense_in) {
             p.ndense = P->ndense;

@@ -765,6 +765,7 @@ struct FusedParams {
     const uint32_t *wprefix;
     const int32_t *seg_level;    // ordinal of an observed FE s level -> level (loads stay unconditional)
     int32_t nsegs_s;             // observed levels of FE s
+    int32_t packed;              // GT = 3: refs[1] == refs[2] hold ref_1 | ref_2 << 26
     const T *coef;     // v
     T *target;         // u
     // per FE: the vector gathered at ref_g (scale .* v under FACT, else v,
@@ -1014,6 +1015,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             auto issue_data = [&](uint32_t lb, Refs &rr, InFlight &d) {
 #pragma unroll
                 for (int q = 0; q < U; q++) rr.r[0][q] = level_s(lb, q, rr.hw, rr.wp);
+                if constexpr (GT == 3) {
+                    // packed pair: branch-free unpack (the same loads either way)
+                    const uint32_t mask = p.packed ? 0x3FFFFFFu : 0xFFFFFFFFu;
+                    const uint32_t shift = p.packed ? 26u : 0u;
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        rr.r[1][q] = (int32_t)((uint32_t)rr.r[1][q] & mask);
+                        rr.r[2][q] = (int32_t)((uint32_t)rr.r[2][q] >> shift);
+                    }
+                }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     const uint32_t i = loc(lb, q);
