@@ -45,7 +45,7 @@ def log(*a):
 # (scaled by a plain device copy of known size), by (workload, precision):
 # profiles/worker_firm_1e7_fp64_k1.md
 PMC_TRAFFIC = {
-    ("worker_firm_1e7", "fp64"): {"k_fused": 369.5e6, "k_dtx_csr": 353.2e6, "k_update": 88.1e6,
+    ("worker_firm_1e7", "fp64"): {"k_fused": 324.5e6, "k_dtx_csr": 277.3e6, "k_update": 88.1e6,
                                   "k_dtx_fixup": 25.3e6},
 }
 

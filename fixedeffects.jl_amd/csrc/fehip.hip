@@ -1652,64 +1652,78 @@ struct Engine {
 
     // Average device time of each kernel of one iteration over `reps`
     // isolated launches (HIP events on the handle's stream).
+    // Average device time of each launch of the LSMR iteration, measured in
+    // place: `reps` iterations run eagerly (not from the graph), in their
+    // order, with a HIP event between consecutive launches on the handle's
+    // stream, so each kernel meets the cache state the real iteration leaves
+    // it.  The iterations run a real LSMR start on filler data (b = 1, x = 0)
+    // with zero tolerances, so no column stops.
     static int time_kernels(fehip_solver *h, int reps, double *out_ms) {
         Plan *P = nullptr;
         CHK((get_plan_for<T, Acc, K>(h, &P)));
         const int64_t NK = h->N * K, pK = h->p * K;
         hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->u, NK, (T)1);
-        void *pv[] = {P->x, P->v, P->hh, P->hbar, P->vt};
-        for (void *q : pv)
-            if (q)
-                hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(pK, kBlock)), dim3(kBlock), 0, h->stream, (T *)q, pK,
-                                   (T)0.5);
+        hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(pK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->x, pK, (T)0);
         hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, K, 0.0,
                            0.0, 0.0, (int64_t)1 << 40, P->all_done);
-        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.5, K, K);
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, K);
+        db(h, P, MODE_INIT, P->ops_once, (const T *)P->x, (T *)P->u, nullptr);
+        fin_db(h, P, MODE_INIT, nullptr);
+        dtx(h, P, MODE_INIT, P->ops_dtx, (const T *)P->u, (T *)P->v, nullptr);
+        fin_dtx(h, P, MODE_INIT, nullptr);
+        hipLaunchKernelGGL((k_init_vectors<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, pK, (T *)P->v,
+                           (T *)P->hh, (T *)P->hbar, (const void *)P->state, (const T *)h->scale_all[pidx<T>()],
+                           (T *)P->vt);
+        if (ut(h) && h->sqrtw_T[pidx<T>()])
+            hipLaunchKernelGGL(k_scale_rows<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->u,
+                               (const T *)h->sqrtw_T[pidx<T>()], h->N, K);
         HIPCHK(hipGetLastError());
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        auto timed = [&](auto launch) -> double {
-            launch();  // warm-up
-            (void)hipEventRecord(e0, h->stream);
-            for (int r = 0; r < reps; r++) launch();
-            (void)hipEventRecord(e1, h->stream);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            return (double)ms / reps;
-        };
-        // the launches of one LSMR iteration, in their iterating mode; state
-        // and vectors hold finite filler
-        out_ms[0] = timed([&] { fused(h, P, P->ops_once, (const T *)P->v, (T *)P->u); });
-        out_ms[1] = timed([&] { fin_db(h, P, MODE_ITER, nullptr, true); });
-        out_ms[2] = P->csr_blocks
-                        ? timed([&] { dtx_seg(h, P, MODE_ITER, P->ops_once, (const T *)P->u, (T *)P->v, nullptr, true, ut(h)); })
-                        : 0.0;
-        out_ms[3] = (P->ndense && !P->fused_dense)
-                        ? timed([&] { dtx_dense(h, P, P->ops_once, (const T *)P->u, nullptr, ut(h)); })
-                        : 0.0;
-        out_ms[4] = timed([&] { dtx_fixup(h, P, MODE_ITER, P->ops_once, (T *)P->v, nullptr, true); });
-        out_ms[5] = timed([&] { fin_dtx(h, P, MODE_ITER, nullptr, true); });
-        hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, K, 0.0,
-                           0.0, 0.0, (int64_t)1 << 40, P->all_done);
-        out_ms[6] = timed([&] { update(h, P); });
-        out_ms[7] = 0.0;  // k_fin_update may stop the filler state; timed once below
-        {
-            hipLaunchKernelGGL((k_lsmr_setup<Tr>), dim3(1), dim3(64), 0, h->stream, (LsmrState<Tr> *)P->state, K, K,
-                               0.0, 0.0, 0.0, (int64_t)1 << 40, P->all_done);
-            (void)hipEventRecord(e0, h->stream);
+        hipEvent_t ev[9];
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        // launch order of iteration(); out_ms slots: fused, fin_db, CSR D'u,
+        // separate few-level D'u, fix-up, fin_dtx, update, fin_update
+        static const int slot_of_gap[8] = {0, 1, 3, 2, 4, 5, 6, 7};
+        auto one = [&](bool rec) {
+            auto mark = [&](int q) {
+                if (rec) (void)hipEventRecord(ev[q], h->stream);
+            };
+            mark(0);
+            fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
+            mark(1);
+            fin_db(h, P, MODE_ITER, P->all_done, true);
+            mark(2);
+            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
+            mark(3);
+            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
+            mark(4);
+            dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
+            mark(5);
+            fin_dtx(h, P, MODE_ITER, P->all_done, true);
+            mark(6);
+            update(h, P);
+            mark(7);
             fin_update(h, P);
-            (void)hipEventRecord(e1, h->stream);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            out_ms[7] = ms;
+            mark(8);
+        };
+        one(false);
+        one(false);
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int r = 0; r < reps; r++) {
+            one(true);
+            HIPCHK(hipEventSynchronize(ev[8]));
+            for (int q = 0; q < 8; q++) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+                acc[slot_of_gap[q]] += ms;
+            }
         }
-        HIPCHK(hipGetLastError());
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        HIPCHK(hipStreamSynchronize(h->stream));
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        int32_t stopped = 0;
+        HIPCHK(hipMemcpy(&stopped, P->all_done, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (stopped) return fail(FEHIP_EHIP, "kernel timing: the filler LSMR stopped early");
+        for (int q = 0; q < 8; q++) out_ms[q] = acc[q] / std::max(reps, 1);
+        if (!P->csr_blocks) out_ms[2] = 0.0;
+        if (P->fused_dense || !P->ndense) out_ms[3] = 0.0;
         return 0;
     }
 

@@ -255,9 +255,11 @@ int fehip_mul_At(fehip_t *h, double *x, const double *y, double a, double b);
 int fehip_lsmr_info(fehip_t *h, int32_t col, double *out);
 
 /* Measurement hook (no reference counterpart): average device time in ms of
- * each launch of one LSMR iteration on K columns, from HIP events around
- * `reps` isolated launches on the handle's stream, in the launch
- * configuration the solve uses, with the solve vectors holding filler:
+ * each launch of one LSMR iteration on K columns.  `reps` iterations of a
+ * real LSMR start on filler data (b = 1, x = 0, zero tolerances) run eagerly
+ * in their order, with a HIP event between consecutive launches on the
+ * handle's stream, in the launch configuration the solve uses -- so each
+ * kernel meets the cache state the iteration leaves it:
  *   out_ms[0] one pass: u = A v - alpha u (scatter!, src/SolverCPU.jl:29-34)
  *             with the raw D'u sums of the FE with most levels and of the
  *             few-level FEs (gather!, src/SolverCPU.jl:21-27)
