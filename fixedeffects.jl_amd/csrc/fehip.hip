@@ -131,6 +131,10 @@ struct fehip_solver {
     int64_t sched_tiles_s = 0, sched_tiles_c = 0;
     int sched_fuse_dense = -1;
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    // second stream of the captured iteration: the FE s fix-up runs beside
+    // the CSR D'u pass (fork after beta, join before alpha)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::vector<double> last_info;  // LSMR scalars of the last solve, 9 per column
     // warm start: per-level degrees (lazy), feature / prediction scratch
     int32_t *degree = nullptr;   // p
@@ -153,6 +157,12 @@ struct fehip_solver {
     int32_t *cc_start = nullptr;  // p + 2
     uint32_t *cc_ncomp = nullptr; // 1
 };
+
+// From this batch width on, the plain iteration's FE s fix-up runs on a second
+// branch of the captured graph, beside the CSR pass.  Measured on N = 1e7:
+// +4..7% at K = 32, while at K = 1 the cross-stream join costs about 18 us
+// per iteration, more than the overlap gains.
+constexpr int kSplitFixK = 16;
 
 // the projected LSMR applies to the residual solves of a call (G >= 2)
 struct FwlScope {
@@ -597,10 +607,16 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
             if (h->h_flag) (void)hipHostFree(h->h_flag);
             for (int e = 0; e < 2; e++)
                 if (h->poll_ev[e]) (void)hipEventDestroy(h->poll_ev[e]);
+            if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+            if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+            if (h->stream2) (void)hipStreamDestroy(h->stream2);
             if (h->stream) (void)hipStreamDestroy(h->stream);
         }
     } cleanup{h};
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void **)&h->h_flag, sizeof(int32_t) * 4, hipHostMallocDefault));
     for (int e = 0; e < 2; e++) HIPCHK(hipEventCreateWithFlags(&h->poll_ev[e], hipEventDisableTiming));
     CHK(dalloc(h, &h->d_flag, 4));
@@ -806,6 +822,11 @@ struct Plan {
     // 0), the other FEs' items, zs = the FE s correction (n_s x K)
     FixItem *items_s = nullptr, *items_o = nullptr;
     int32_t nitems_s = 0, nitems_o = 0;
+    // plain iteration, split fix-up: FE s (and the fused few-level FEs')
+    // items beside the CSR pass, the CSR FEs' (and a separate few-level
+    // pass's) items after it
+    FixItem *items_sd = nullptr, *items_c = nullptr;
+    int32_t nitems_sd = 0, nitems_c = 0;
     void *zs = nullptr;
     ColOps *ops_one = nullptr, *ops_neg = nullptr, *ops_sub = nullptr;
 };
@@ -954,7 +975,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     CHK(dalloc(h, &P->items, std::max<int32_t>(P->nitems, 1)));
     if (P->nitems)
         HIPCHK(hipMemcpyAsync(P->items, items.data(), sizeof(FixItem) * P->nitems, hipMemcpyHostToDevice, h->stream));
-    std::vector<FixItem> its, ito;  // alive until the synchronize below
+    std::vector<FixItem> its, ito, isd, ic;  // alive until the synchronize below
     {
         for (const FixItem &fi : items) {
             if (fi.fe != h->s) {
@@ -965,6 +986,21 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
                 its.push_back(z);
             }
         }
+        for (const FixItem &fi : items) {
+            const bool dense = h->fe[fi.fe].kind == KIND_DENSE;
+            if (fi.fe == h->s || (dense && P->fused_dense)) isd.push_back(fi);
+            else ic.push_back(fi);
+        }
+        P->nitems_sd = (int32_t)isd.size();
+        P->nitems_c = (int32_t)ic.size();
+        CHK(dalloc(h, &P->items_sd, std::max<int32_t>(P->nitems_sd, 1)));
+        CHK(dalloc(h, &P->items_c, std::max<int32_t>(P->nitems_c, 1)));
+        if (P->nitems_sd)
+            HIPCHK(hipMemcpyAsync(P->items_sd, isd.data(), sizeof(FixItem) * P->nitems_sd, hipMemcpyHostToDevice,
+                                  h->stream));
+        if (P->nitems_c)
+            HIPCHK(hipMemcpyAsync(P->items_c, ic.data(), sizeof(FixItem) * P->nitems_c, hipMemcpyHostToDevice,
+                                  h->stream));
         P->nitems_s = (int32_t)its.size();
         P->nitems_o = (int32_t)ito.size();
         CHK(dalloc(h, &P->items_s, std::max<int32_t>(P->nitems_s, 1)));
@@ -989,7 +1025,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     CHK(dalloc(h, &P->part_db, (size_t)P->db_blocks * K));
     CHK(dalloc(h, &P->part_fused, (size_t)P->fused_blocks * K));
     CHK(dalloc(h, &P->part_seg, (size_t)P->seg_blocks * K));
-    CHK(dalloc(h, &P->part_fix, (size_t)P->fix_blocks * K));
+    CHK(dalloc(h, &P->part_fix, (size_t)2 * P->fix_blocks * K));  // two halves: the split fix-up
     CHK(dalloc(h, &P->part_upd, (size_t)P->upd_blocks * K));
     CHK(dalloc_bytes(h, &P->state, sizeof(LsmrState<double>) * K));
     CHK(dalloc(h, &P->ops_db, 1));
@@ -1347,11 +1383,13 @@ struct Engine {
 
     // alpha = ||v|| and the plane rotations (partials of k_dtx_seg over all
     // segmented FEs, or over the CSR FEs only after k_fused, and the fix-up)
-    static void fin_dtx(fehip_solver *h, Plan *P, int mode, const int32_t *early_exit, bool fused = false) {
+    // split: the partials of the split fix-up (both halves)
+    static void fin_dtx(fehip_solver *h, Plan *P, int mode, const int32_t *early_exit, bool fused = false,
+                        bool split = false) {
         const int64_t n0 = fused ? P->csr_blocks : (P->nseg ? P->seg_blocks : 0);
         hipLaunchKernelGGL((k_fin_dtx<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream, (const double *)P->part_seg,
-                           n0, (const double *)P->part_fix, (int64_t)P->fix_blocks, mode, P->state, P->ops_db,
-                           P->all_done, early_exit);
+                           n0, (const double *)P->part_fix, (int64_t)P->fix_blocks * (split ? 2 : 1), mode, P->state,
+                           P->ops_db, P->all_done, early_exit);
     }
 
     // ||x|| and the stopping rules
@@ -1608,21 +1646,32 @@ struct Engine {
 
     // proj: the projected iteration's form -- the items of the FEs other
     // than s only, FE s untouched (its coefficients stay zero)
+    // part (plain fused iteration only): 1 = FE s streaming and the FE s /
+    // fused few-level items (on `st`, beside the CSR pass), 2 = the other
+    // items into the second half of the partials
     static void dtx_fixup(fehip_solver *h, Plan *P, int mode, const ColOps *ops, T *coef,
-                          const int32_t *early_exit, bool fused = false, bool proj = false) {
+                          const int32_t *early_exit, bool fused = false, bool proj = false, int part = 0,
+                          hipStream_t st = nullptr) {
         FixParams<T> f{};
         f.nitems = proj ? P->nitems_o : P->nitems;
         f.items = proj ? P->items_o : P->items;
+        if (part == 1) {
+            f.nitems = P->nitems_sd;
+            f.items = P->items_sd;
+        } else if (part == 2) {
+            f.nitems = P->nitems_c;
+            f.items = P->items_c;
+        }
         f.tileA = P->tileA;
         f.tileB = P->tileB;
         f.dpart = P->dpart;
         f.v = coef;
         f.ops = ops;
-        f.partials = P->part_fix;
+        f.partials = P->part_fix + (part == 2 ? (size_t)P->fix_blocks * K : 0);
         f.all_done = early_exit;
         f.mode = mode;
         if (fused) {
-            if (!proj) {
+            if (!proj && part != 2) {
                 f.wsum = P->wsum;
                 f.sdirect = P->sdirect;
                 f.ns = h->n[h->s];
@@ -1640,7 +1689,7 @@ struct Engine {
                 }
             }
         }
-        hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, f);
+        hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, st ? st : h->stream, f);
     }
 
     static void dtx(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
@@ -1678,11 +1727,12 @@ struct Engine {
             hipLaunchKernelGGL(k_scale_rows<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->u,
                                (const T *)h->sqrtw_T[pidx<T>()], h->N, K);
         HIPCHK(hipGetLastError());
-        hipEvent_t ev[9];
+        hipEvent_t ev[10];
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-        // launch order of iteration(); out_ms slots: fused, fin_db, CSR D'u,
-        // separate few-level D'u, fix-up, fin_dtx, update, fin_update
-        static const int slot_of_gap[8] = {0, 1, 3, 2, 4, 5, 6, 7};
+        // the launches of iteration(), one after another on one stream;
+        // out_ms slots: fused, fin_db, CSR D'u, separate few-level D'u,
+        // fix-up (both parts), fin_dtx, update, fin_update
+        static const int slot_of_gap[9] = {0, 1, 4, 3, 2, 4, 5, 6, 7};
         auto one = [&](bool rec) {
             auto mark = [&](int q) {
                 if (rec) (void)hipEventRecord(ev[q], h->stream);
@@ -1692,26 +1742,29 @@ struct Engine {
             mark(1);
             fin_db(h, P, MODE_ITER, P->all_done, true);
             mark(2);
-            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
+            if (K >= kSplitFixK) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 1);
             mark(3);
-            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
+            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
             mark(4);
-            dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
+            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
             mark(5);
-            fin_dtx(h, P, MODE_ITER, P->all_done, true);
+            if (K >= kSplitFixK) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 2);
+            else dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
             mark(6);
-            update(h, P);
+            fin_dtx(h, P, MODE_ITER, P->all_done, true, K >= kSplitFixK);
             mark(7);
-            fin_update(h, P);
+            update(h, P);
             mark(8);
+            fin_update(h, P);
+            mark(9);
         };
         one(false);
         one(false);
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int r = 0; r < reps; r++) {
             one(true);
-            HIPCHK(hipEventSynchronize(ev[8]));
-            for (int q = 0; q < 8; q++) {
+            HIPCHK(hipEventSynchronize(ev[9]));
+            for (int q = 0; q < 9; q++) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
                 acc[slot_of_gap[q]] += ms;
@@ -1824,10 +1877,26 @@ struct Engine {
         }
         fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
         fin_db(h, P, MODE_ITER, P->all_done, true);
+        if (K < kSplitFixK) {
+            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
+            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
+            dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
+            fin_dtx(h, P, MODE_ITER, P->all_done, true);
+            update(h, P);
+            fin_update(h, P);
+            return;
+        }
+        // wide batches: the FE s fix-up needs only the fused pass and beta,
+        // so it runs as a second branch beside the CSR pass
+        (void)hipEventRecord(h->fork_ev, h->stream);
+        (void)hipStreamWaitEvent(h->stream2, h->fork_ev, 0);
+        dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 1, h->stream2);
         if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
         dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
-        dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
-        fin_dtx(h, P, MODE_ITER, P->all_done, true);
+        dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 2);
+        (void)hipEventRecord(h->join_ev, h->stream2);
+        (void)hipStreamWaitEvent(h->stream, h->join_ev, 0);
+        fin_dtx(h, P, MODE_ITER, P->all_done, true, true);
         update(h, P);
         fin_update(h, P);
     }
@@ -3126,6 +3195,12 @@ void fehip_destroy(fehip_t *h) {
     if (h->h_flag) (void)hipHostFree(h->h_flag);
     for (int e = 0; e < 2; e++)
         if (h->poll_ev[e]) (void)hipEventDestroy(h->poll_ev[e]);
+    if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+    if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+    if (h->stream2) {
+        (void)hipStreamSynchronize(h->stream2);
+        (void)hipStreamDestroy(h->stream2);
+    }
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
