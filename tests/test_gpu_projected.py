@@ -168,3 +168,22 @@ def test_projected_panels_and_x0(gpu):
     finally:
         for s in solvers:
             s.close()
+
+
+@pytest.mark.parametrize("prec,K,inter", [("fp64", 8, (2,)), ("fp64", 16, ()), ("fp32", 4, ())])
+def test_projected_batches_of_other_forms(gpu, prec, K, inter):
+    # batched projected solves in the non-factored form (interaction), at the
+    # width where the fix-up splits (K = 16) and in FP32: converged, every
+    # column within the precision's bar of the exact residual
+    _say(f"case: projected {prec} K={K} interactions {inter}")
+    rng = np.random.default_rng(100 + K)
+    refs, n, z, w, y = _rand_problem(90 + K, 25_000, (2500, 250, 12), interaction=inter)
+    Y = np.column_stack([y[:, 0]] + [rng.standard_normal(25_000) for _ in range(K - 1)])
+    o = oracle.OracleSolver(refs, n, z, w, nthreads=8)
+    tight = [o.solve_residuals(Y[:, c].copy(), tol=1e-13, maxiter=100_000)[0] for c in range(K)]
+    tol, bar = (1e-12, RES_RTOL) if prec == "fp64" else (1e-6, 1e-3)
+    rp, ip, cp = _solve(refs, n, z, w, Y, "projected", prec, tol)
+    assert all(cp)
+    worst = max(_rel(rp[c], tight[c]) for c in range(K))
+    _say("  iters", ip[:4], "worst rel err", worst)
+    assert worst <= bar
