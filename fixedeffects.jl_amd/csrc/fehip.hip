@@ -1536,6 +1536,12 @@ struct Engine {
     // mode MODE_INIT leaves the ||y||^2 partials of k_db for fin_db.  Uses
     // ops_one = (1, 0) and ops_sub = (-1, 1), set by start().
     static void proj_s(fehip_solver *h, Plan *P, T *y, int mode) {
+        s_sums(h, P, y);
+        s_apply(h, P, y, mode);
+    }
+
+    // zs = A_s' y (the FE s block of A'y; ops_one)
+    static void s_sums(fehip_solver *h, Plan *P, const T *y) {
         const FeInfo &f = h->fe[h->s];
         DtxSegParams<T> q{};
         q.N = h->N;
@@ -1576,6 +1582,11 @@ struct Engine {
         fx.all_done = nullptr;
         fx.mode = MODE_NONE;
         hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, h->stream, fx);
+    }
+
+    // y <- y - A_s zs (ops_sub); mode MODE_INIT leaves ||y||^2 partials
+    static void s_apply(fehip_solver *h, Plan *P, T *y, int mode) {
+        const FeInfo &f = h->fe[h->s];
         DbParams<T> d{};
         d.N = h->N;
         d.G = 1;
@@ -2094,16 +2105,7 @@ struct Engine {
                            h->perm_s, h->sqrtw_int, h->N, (T *)P->r, (T *)P->u);
         if (h->fwl_now) {
             // projected problem: b <- (I - A_s A_s') b
-            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_one, 1.0, 0.0, K, Kreal);
-            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_neg, -1.0, 0.0, K, Kreal);
-            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_sub, -1.0, 1.0, K, Kreal);
-            if (h->factored) {
-                constexpr int q = pidx<T>();
-                const int64_t ns = h->n[h->s];
-                if (!h->scale2_T[q]) CHK(dalloc_bytes(h, &h->scale2_T[q], sizeof(T) * ns));
-                hipLaunchKernelGGL(k_square<T>, dim3(grid_for(ns, kBlock)), dim3(kBlock), 0, h->stream,
-                                   (const T *)h->fe[h->s].scale_T[q], ns, (T *)h->scale2_T[q]);
-            }
+            CHK(fwl_setup(h, P, Kreal));
             proj_s(h, P, (T *)P->r, MODE_NONE);
             hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (const T *)P->r,
                                (T *)P->u, NK);
@@ -2111,6 +2113,22 @@ struct Engine {
         if (x_from_At) {
             hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, 1.0, 0.0, K, Kreal);
             dtx(h, P, MODE_NONE, P->ops_once, (const T *)P->r, (T *)P->x, nullptr);
+        }
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+
+    // the projected iteration's constant operators and scale_s^2
+    static int fwl_setup(fehip_solver *h, Plan *P, int Kreal) {
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_one, 1.0, 0.0, K, Kreal);
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_neg, -1.0, 0.0, K, Kreal);
+        hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_sub, -1.0, 1.0, K, Kreal);
+        if (h->factored) {
+            constexpr int q = pidx<T>();
+            const int64_t ns = h->n[h->s];
+            if (!h->scale2_T[q]) CHK(dalloc_bytes(h, &h->scale2_T[q], sizeof(T) * ns));
+            hipLaunchKernelGGL(k_square<T>, dim3(grid_for(ns, kBlock)), dim3(kBlock), 0, h->stream,
+                               (const T *)h->fe[h->s].scale_T[q], ns, (T *)h->scale2_T[q]);
         }
         HIPCHK(hipGetLastError());
         return 0;
@@ -2294,10 +2312,26 @@ struct Engine {
             hipLaunchKernelGGL((k_x0_from_coef<T>), dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream,
                                (const double *)h->ws_coef, ws_params(h), h->p, (T *)P->x, K, 0);
         }
+        if (h->fwl_now) {
+            // projected: b kept in r; LSMR on (I - A_s A_s') b
+            hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (const T *)P->u,
+                               (T *)P->r, NK);
+            CHK(fwl_setup(h, P, 1));
+            proj_s(h, P, (T *)P->u, MODE_NONE);
+        }
         CHK(lsmr(h, P, 1, tol, maxiter));
         std::vector<int64_t> it;
         std::vector<int> cv;
         CHK(read_state(h, P, 1, it, cv));
+        if (h->fwl_now) {
+            // the eliminated FE's coefficients: the least-squares fit of the
+            // rest, x_s = A_s'(b - A_o x_o) (A_s has orthonormal columns)
+            hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, 1);
+            db(h, P, MODE_NONE, P->ops_once, (const T *)P->x, (T *)P->r, nullptr);
+            s_sums(h, P, (const T *)P->r);
+            hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(h->n[h->s] * K, kBlock)), dim3(kBlock), 0, h->stream,
+                               (const T *)P->zs, (T *)P->x + h->off[h->s] * K, h->n[h->s] * K);
+        }
         for (int g = 0; g < h->G; g++)
             hipLaunchKernelGGL(k_scale_coef<T>, dim3(grid_for(h->n[g], kBlock)), dim3(kBlock), 0, h->stream,
                                (const T *)P->x + h->off[g], (const T *)h->fe[g].scale_T[pidx<T>()], h->n[g],
@@ -2480,6 +2514,8 @@ static int solve_residuals_dev(fehip_solver *h, double *dY, int Kreal, int64_t l
 // LSMR from zero, coefficients unscaled into dc (p doubles, device).
 static int solve_coefficients_dev(fehip_solver *h, double *dy, double tol, int64_t maxiter, double *dc,
                                   int64_t *iters, uint8_t *converged) {
+    FwlScope fwl(h);
+    if (h->precision == FEHIP_MIXED) h->fwl_now = false;  // its two phases keep the reference LSMR
     if (h->precision == FEHIP_MIXED)
         return for_K_mixed(1, [&](auto e) {
             return decltype(e)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged);
@@ -3038,6 +3074,7 @@ int fehip_solve_coefficients_x0(fehip_t *h, const double *y, double tol, int64_t
     hipError_t e = hipMemcpyAsync(dy, y, sizeof(double) * h->N, hipMemcpyHostToDevice, h->stream);
     if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
     const int prec = h->precision == FEHIP_MIXED ? FEHIP_FP64 : h->precision;
+    FwlScope fwl(h);
     if (rc == 0)
         rc = dispatch(prec, 1, [&](auto eng) {
             return decltype(eng)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged, coef0);

@@ -325,8 +325,8 @@ class FixedEffectSolverROCm:
 
     def set_algorithm(self, algorithm: str) -> "FixedEffectSolverROCm":
         """"lsmr" (the reference's LSMR on all FEs) or "projected" (LSMR with
-        the FE of most levels eliminated; same least-squares residual, fewer
-        iterations; residual solves only)."""
+        the FE of most levels eliminated: the same least-squares residual and,
+        after normalize!, coefficients; fewer iterations)."""
         if algorithm not in _ALGORITHMS:
             raise ValueError(f"algorithm must be one of {sorted(_ALGORITHMS)}")
         check(lib.fehip_set_algorithm(self._h, _ALGORITHMS[algorithm]))

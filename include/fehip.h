@@ -145,8 +145,10 @@ int fehip_solve_coefficients_full(fehip_t *h, const double *y, double tol,
  * exact within-level demeaning (the columns of A_s are orthonormal).  The
  * least-squares residual is the same; LSMR needs fewer iterations (a smaller
  * condition number) and the same stopping rule applies to the projected
- * problem.  Applies to fehip_solve_residuals*, panels and the _x0/_learned
- * forms; the coefficient solves always use FEHIP_LSMR. */
+ * problem.  Applies to fehip_solve_residuals*, panels, the _x0/_learned
+ * forms and, in FEHIP_FP64 / FEHIP_FP32, to the coefficient solves: there
+ * the eliminated FE's coefficients are the least-squares fit of the rest,
+ * A_s'(b - A_o x_o), and iters is div(projected LSMR iterations, 2). */
 #define FEHIP_LSMR 0
 #define FEHIP_LSMR_PROJECTED 1
 int fehip_set_algorithm(fehip_t *h, int32_t algorithm);

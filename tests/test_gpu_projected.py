@@ -187,3 +187,26 @@ def test_projected_batches_of_other_forms(gpu, prec, K, inter):
     worst = max(_rel(rp[c], tight[c]) for c in range(K))
     _say("  iters", ip[:4], "worst rel err", worst)
     assert worst <= bar
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_projected_coefficients(gpu, prec):
+    # solve_coefficients! on the projected problem: FE s's coefficients are
+    # recovered as the least-squares fit of the rest; after normalize! the
+    # coefficients and the fitted values match the oracle's reference solve
+    _say(f"case: projected coefficients {prec}")
+    refs, n, z, w, y = _rand_problem(61, 20_000, (2000, 150))
+    y = y[:, 0]
+    o = oracle.OracleSolver(refs, n, z, w)
+    tol = 1e-10 if prec == "fp64" else 1e-6
+    co, io, cvo = o.solve_coefficients(y.copy(), tol=tol, maxiter=10000)
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, prec, algorithm="projected") as s:
+        cg, ig, cvg = s.solve_coefficients(y.copy(), tol=tol, maxiter=10000)
+        s.set_algorithm("lsmr")
+        _, il, _ = s.solve_coefficients(y.copy(), tol=tol, maxiter=10000)
+    _say("  iters projected", ig, "plain", il, "oracle", io)
+    assert cvg and cvo and ig < il
+    bar = 1e-8 if prec == "fp64" else 1e-4
+    assert _rel(sum(cg), sum(co)) <= bar
+    for a, b in zip(cg, co):
+        np.testing.assert_allclose(a, b, rtol=1e-6 if prec == "fp64" else 1e-3, atol=1e-6 if prec == "fp64" else 1e-3)
