@@ -395,16 +395,17 @@ def _write_back(xs, cols):
 
 
 def AbstractFixedEffectSolver(fes, weights=None, method: str = "rocm", double_precision: bool = True,
-                              precision: Optional[str] = None) -> FixedEffectSolverROCm:
+                              precision: Optional[str] = None, algorithm: str = "lsmr") -> FixedEffectSolverROCm:
     """Backend constructor AbstractFixedEffectSolver{T}(fes, w, Val{method})
     (src/SolverCPU.jl:55, src/AbstractFixedEffectSolver.jl:43).  Only
     method = "rocm" exists here; precision="mixed" selects the FP32-stream /
-    FP64-recurrence mode (not a reference option)."""
+    FP64-recurrence mode and algorithm="projected" the LSMR with the FE of
+    most levels eliminated (neither is a reference option)."""
     if method != "rocm":
         raise ValueError(f"method {method!r} is not provided by this package (method = 'rocm')")
     if precision is None:
         precision = "fp64" if double_precision else "fp32"
-    return FixedEffectSolverROCm(fes, weights, precision)
+    return FixedEffectSolverROCm(fes, weights, precision, algorithm=algorithm)
 
 
 def _outer(method, double_precision, tol):
@@ -435,7 +436,7 @@ def solve_residuals_panels(solvers, dptrs, Ks, lds, tol: float = 1e-8, maxiter: 
 
 def solve_residuals(y, fes, w=None, *, method: str = "rocm", double_precision: Optional[bool] = None,
                     tol: Optional[float] = None, maxiter: int = 10000, precision: Optional[str] = None,
-                    progress_bar: bool = True):
+                    progress_bar: bool = True, algorithm: str = "lsmr"):
     """solve_residuals!(y, fes, w; method, double_precision, tol, maxiter)
     (src/AbstractFixedEffectSolver.jl:37-45), and the solver-object forms
     solve_residuals!(y, feM) / solve_residuals!(xs, feM) (:49-97).
@@ -453,14 +454,15 @@ def solve_residuals(y, fes, w=None, *, method: str = "rocm", double_precision: O
     if any(len(fe) != N for fe in fes):
         raise ValueError("FixedEffects must have the same length as y")  # :42
     dp, tol = _outer(method, double_precision, tol)
-    with AbstractFixedEffectSolver(fes, w, method, dp, precision) as feM:
+    with AbstractFixedEffectSolver(fes, w, method, dp, precision, algorithm) as feM:
         if _is_single(y):
             return feM.solve_residuals(y, tol=tol, maxiter=maxiter)
         return feM.solve_residuals_many(y, tol=tol, maxiter=maxiter, progress_bar=progress_bar)
 
 
 def solve_coefficients(y, fes, w=None, *, method: str = "rocm", double_precision: Optional[bool] = None,
-                       tol: Optional[float] = None, maxiter: int = 10000, precision: Optional[str] = None):
+                       tol: Optional[float] = None, maxiter: int = 10000, precision: Optional[str] = None,
+                       algorithm: str = "lsmr"):
     """solve_coefficients!(y, fes, w; ...) (src/AbstractFixedEffectSolver.jl:137-145)."""
     if isinstance(fes, FixedEffectSolverROCm):
         kw = {} if tol is None else {"tol": tol}
@@ -469,7 +471,7 @@ def solve_coefficients(y, fes, w=None, *, method: str = "rocm", double_precision
     if any(len(fe) != len(y) for fe in fes):
         raise ValueError("FixedEffects must have the same length as y")  # :142
     dp, tol = _outer(method, double_precision, tol)
-    with AbstractFixedEffectSolver(fes, w, method, dp, precision) as feM:
+    with AbstractFixedEffectSolver(fes, w, method, dp, precision, algorithm) as feM:
         return feM.solve_coefficients(y, tol=tol, maxiter=maxiter)
 
 

@@ -19,7 +19,7 @@ from . import api
 
 
 def partial_out(Y, fes: Sequence[api.FixedEffect], weights=None, esample=None, *, precision: str = "fp64",
-                tol: float = 1e-8, maxiter: int = 10_000):
+                tol: float = 1e-8, maxiter: int = 10_000, algorithm: str = "lsmr"):
     """Residualize every column of Y (N x K) on the fixed effects.
 
     esample: boolean mask or index array of the observations kept (rows of
@@ -27,6 +27,8 @@ def partial_out(Y, fes: Sequence[api.FixedEffect], weights=None, esample=None, *
     numbering as in getindex(fe, esample)).  Returns (residuals (n x K,
     float64), iterations per column, converged per column)."""
     Y = np.asarray(Y, np.float64)
+    # algorithm="projected": LSMR with the FE of most levels eliminated (the
+    # same residuals, fewer iterations)
     if Y.ndim == 1:
         Y = Y[:, None]
     if esample is not None:
@@ -36,13 +38,13 @@ def partial_out(Y, fes: Sequence[api.FixedEffect], weights=None, esample=None, *
         if weights is not None:
             weights = np.asarray(weights, np.float64)[idx]
     cols = [np.ascontiguousarray(Y[:, k]) for k in range(Y.shape[1])]
-    with api.FixedEffectSolverROCm(list(fes), weights, precision) as s:
+    with api.FixedEffectSolverROCm(list(fes), weights, precision, algorithm=algorithm) as s:
         _, iters, conv = s.solve_residuals_many(cols, tol=tol, maxiter=maxiter)
     return np.stack(cols, axis=1), iters, conv
 
 
 def fe_ols(y, X, fes: Sequence[api.FixedEffect], weights=None, esample=None, *, precision: str = "fp64",
-           tol: float = 1e-8, maxiter: int = 10_000):
+           tol: float = 1e-8, maxiter: int = 10_000, algorithm: str = "lsmr"):
     """Least squares of y on X with the fixed effects absorbed: partial the
     fixed effects out of [y X] on the GPU, then solve the weighted normal
     equations of the residualized columns.  Returns (coefficients,
@@ -52,7 +54,7 @@ def fe_ols(y, X, fes: Sequence[api.FixedEffect], weights=None, esample=None, *, 
     if X.ndim == 1:
         X = X[:, None]
     R, iters, conv = partial_out(np.column_stack([y, X]), fes, weights, esample, precision=precision, tol=tol,
-                                 maxiter=maxiter)
+                                 maxiter=maxiter, algorithm=algorithm)
     w = None
     if weights is not None:
         w = np.asarray(weights, np.float64)

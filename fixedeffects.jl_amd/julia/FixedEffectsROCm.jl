@@ -236,4 +236,14 @@ function fixedeffect(keys::AbstractVector{<:Integer}...; interaction::AbstractVe
     return FixedEffect{typeof(refs), typeof(interaction)}(refs, interaction, Int(ng[]))
 end
 
+# LSMR algorithm of this solver's solves (no reference counterpart):
+# :lsmr (the reference's) or :projected (the FE of most levels eliminated --
+# the same residuals and normalized coefficients, fewer iterations).
+function set_algorithm!(feM::FixedEffectSolverROCm, algorithm::Symbol)
+    code = algorithm === :lsmr ? Int32(0) : algorithm === :projected ? Int32(1) :
+           throw(ArgumentError("algorithm must be :lsmr or :projected"))
+    check(ccall((:fehip_set_algorithm, libfehip), Cint, (Ptr{Cvoid}, Int32), feM.handle, code))
+    return feM
+end
+
 end # module
