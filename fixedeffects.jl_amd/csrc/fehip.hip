@@ -917,13 +917,13 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             P->nslot += (int32_t)h->fe[g].n;
         }
     // accumulators per slot and wave: one per lane, or one per column
-    // (dense_phased: FP64, K >= 16)
-    const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * sizeof(double) : (size_t)kWave * accsz;
-    // (k_dtx_dense then runs on the fused partition, 4 waves per workgroup,
-    // with lane-private accumulators: those must fit 64 KB as well)
+    // (dense_phased: FP64, K >= 16), in k_fused and k_dtx_dense alike
+    const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * accsz : (size_t)kWave * accsz;
+    P->dense_lds_wave = (int32_t)(per_slot * maxlev);
+    // (k_dtx_dense then runs on the fused partition, 4 waves per workgroup:
+    // its accumulators must fit 64 KB as well)
     P->fused_dense = P->nslot > 0 && (size_t)P->nslot * per_slot * kWavesPerBlock <= 65536 &&
-                     (size_t)maxlev * kWave * accsz * kWavesPerBlock <= 65536 && fuse_dense_on(h);
-    P->dense_lds_wave = (int32_t)(accsz * kWave * maxlev);
+                     (size_t)P->dense_lds_wave * kWavesPerBlock <= 65536 && fuse_dense_on(h);
     int64_t chunk;
     if (P->fused_dense) {
         chunk = P->R;
@@ -1049,7 +1049,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
 // per workgroup (the rule get_plan applies)
 static size_t fused_lds_bytes(const fehip_solver *h, size_t ts, int K) {
     const size_t accsz = ts == 8 ? sizeof(double) : sizeof(F32x2);
-    const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * sizeof(double) : (size_t)kWave * accsz;
+    const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * accsz : (size_t)kWave * accsz;
     int64_t nslot = 0;
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) nslot += h->fe[g].n;
@@ -1057,7 +1057,7 @@ static size_t fused_lds_bytes(const fehip_solver *h, size_t ts, int K) {
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) maxlev = std::max<int64_t>(maxlev, h->fe[g].n);
     const size_t b = (size_t)nslot * per_slot * kWavesPerBlock;
-    return (b <= 65536 && (size_t)maxlev * kWave * accsz * kWavesPerBlock <= 65536 && fuse_dense_on(h)) ? b : 0;
+    return (b <= 65536 && (size_t)maxlev * per_slot * kWavesPerBlock <= 65536 && fuse_dense_on(h)) ? b : 0;
 }
 
 // resident waves of a kernel on this device (all CUs, kBlock threads)

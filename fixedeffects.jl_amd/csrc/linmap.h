@@ -675,6 +675,30 @@ struct DtxDenseParams {
     const int32_t *all_done;
 };
 
+// Few-level FE accumulators of the fused pass and of k_dtx_dense: per level
+// and column, added to in ordered phases, for FP64 batches of 16 or more
+// columns (P <= 4 phases); lane-private otherwise.  The host sizes the LDS
+// with the same rule.
+template <int K, typename Acc>
+__host__ __device__ constexpr bool dense_phased() {
+    return K >= 16 && std::is_same<Acc, double>::value;
+}
+
+// One phased add of U steps into per-wave accumulators acc[slot*K + c] (FP64):
+// the lanes of phase ph (psub == ph) hold distinct columns, so no two lanes of
+// an instruction meet; LDS adds in a fixed order of steps and phases.
+template <int U, int K, typename Acc>
+__device__ __forceinline__ void phased_add(Acc *acc, const int32_t (&slot)[U], const Acc (&x)[U], int psub, int c) {
+    static_assert(std::is_same<Acc, double>::value, "phased accumulators are FP64 only");
+    constexpr int P = kWave / K;
+#pragma unroll
+    for (int q = 0; q < U; q++)
+#pragma unroll
+        for (int ph = 0; ph < P; ph++)
+            if (psub == ph) atomicAdd(acc + slot[q] * K + c, x[q]);
+}
+
+
 // RAW: sums of the stored sqrt(w) .* u without the cache (the fix-up scales them)
 template <typename T, typename Acc, int K, bool RAW = false>
 __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
@@ -688,8 +712,13 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
     while (f + 1 < p.ndense && (int)blockIdx.x >= p.fe[f + 1].blk0) f++;
     const DenseFE<T> fe = p.fe[f];
     const int bl = blockIdx.x - fe.blk0;
-    Acc *acc = reinterpret_cast<Acc *>(dense_smem) + (size_t)wave * fe.nlev * kWave;
-    for (int k = 0; k < fe.nlev; k++) acc[k * kWave + lane] = Acc(0.f);
+    constexpr bool PHASED = dense_phased<K, Acc>();  // per-column accumulators, as in k_fused
+    Acc *acc = reinterpret_cast<Acc *>(dense_smem) + (size_t)wave * fe.nlev * (PHASED ? K : kWave);
+    if constexpr (PHASED) {
+        for (int e = lane; e < fe.nlev * K; e += kWave) acc[e] = Acc(0.f);
+    } else {
+        for (int k = 0; k < fe.nlev; k++) acc[k * kWave + lane] = Acc(0.f);
+    }
     const int on = p.ops->on[c];
     const int64_t start = ((int64_t)bl * wpb + wave) * p.chunk;
     const int64_t end = start + p.chunk < p.N ? start + p.chunk : p.N;
@@ -711,22 +740,25 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
                     x[q] = pos < end ? mul_acc<Acc, T>(cv, uv) : Acc(0.f);
                 }
             }
-            lds_accumulate<U, Acc>(acc, r, x, lane);
+            if constexpr (PHASED) phased_add<U, K, Acc>(acc, r, x, psub, c);
+            else lds_accumulate<U, Acc>(acc, r, x, lane);
         }
     }
-    // Per-wave level sums over the lanes of each column.  The double result
-    // of level k goes to slot k*K + c of the wave's own region, which only
-    // overlaps accumulators of levels <= k, already consumed (K <= 32).
-    double *wsum = reinterpret_cast<double *>(acc);
-    for (int k = 0; k < fe.nlev; k++) {
-        Acc x = acc[k * kWave + lane];
-        x = column_allreduce<K>(x);
-        if (psub == 0) wsum[k * K + c] = acc_to_double(x);
+    if constexpr (!PHASED) {
+        // Per-wave level sums over the lanes of each column.  The double result
+        // of level k goes to slot k*K + c of the wave's own region, which only
+        // overlaps accumulators of levels <= k, already consumed (K <= 32).
+        double *wsum = reinterpret_cast<double *>(acc);
+        for (int k = 0; k < fe.nlev; k++) {
+            Acc x = acc[k * kWave + lane];
+            x = column_allreduce<K>(x);
+            if (psub == 0) wsum[k * K + c] = acc_to_double(x);
+        }
     }
     __syncthreads();
     // combine the waves in order; thread t handles entries t, t + blockDim, ...
     double *out = p.dpart + fe.dpart_row0 * K;
-    const size_t region = (size_t)fe.nlev * kWave * sizeof(Acc) / sizeof(double);
+    const size_t region = (size_t)fe.nlev * (PHASED ? K : kWave) * sizeof(Acc) / sizeof(double);
     const double *base_ws = reinterpret_cast<const double *>(dense_smem);
     for (int e = threadIdx.x; e < fe.nlev * K; e += blockDim.x) {
         double s = 0.0;
@@ -810,14 +842,6 @@ __device__ __forceinline__ void sto(V *base, uint32_t i, V v) {
     *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(V)) = v;
 }
 
-// Few-level FE accumulators of the fused pass: per level and column, added to
-// in ordered phases, for FP64 batches of 16 or more columns (P <= 4 phases);
-// lane-private otherwise.  The host sizes the LDS with the same rule.
-template <int K, typename Acc>
-__host__ __device__ constexpr bool dense_phased() {
-    return K >= 16 && sizeof(Acc) == sizeof(double) && !std::is_same<Acc, F32x2>::value;
-}
-
 // The factored iteration with fewer than 8 columns stores u as sqrt(w) .* u:
 // the CSR and few-level D'u passes then read no cache (80 MB per column at
 // N = 1e7).  Wider batches keep u (their fused pass is VALU-bound, and the
@@ -868,15 +892,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         for (int k = 0; k < p.nslot; k++) acc[k * kWave + lane] = Acc(0.f);
     }
     auto dense_add = [&](const int32_t (&sl)[U], const Acc (&x)[U]) {
-        if constexpr (PHASED) {
-#pragma unroll
-            for (int q = 0; q < U; q++)
-#pragma unroll
-                for (int ph = 0; ph < P; ph++)
-                    if (psub == ph) atomicAdd(reinterpret_cast<double *>(acc) + sl[q] * K + c, (double)x[q]);
-        } else {
-            lds_accumulate<U, Acc>(acc, sl, x, lane);
-        }
+        if constexpr (PHASED) phased_add<U, K, Acc>(acc, sl, x, psub, c);
+        else lds_accumulate<U, Acc>(acc, sl, x, lane);
     };
     const int t = blockIdx.x * kWavesPerBlock + wave;
     double sq = 0.0;

@@ -489,3 +489,24 @@ def test_panels_of_mixed_precisions(gpu):
     finally:
         for s in solvers:
             s.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "fp32"])
+def test_wide_batch_few_level_fes(gpu, prec):
+    # K = 16 with a 20-level FE and a 45-level interacted FE: the fused pass's
+    # per-column phased accumulators (LDS adds in FP64, compensated
+    # read-modify-write in FP32) and the set-up pass on the same partition
+    _say(f"case: K=16 few-level FEs {prec}")
+    rng = np.random.default_rng(77)
+    refs, n, z, w, y = _rand_problem(78, 30_000, (3000, 20, 45), interaction=(2,))
+    K = 16
+    Y = np.column_stack([y] + [rng.standard_normal(30_000) for _ in range(K - 1)])
+    o = oracle.OracleSolver(refs, n, z, w, nthreads=8)
+    tight = [o.solve_residuals(Y[:, c].copy(), tol=1e-13, maxiter=100_000)[0] for c in range(K)]
+    tol, bar = (1e-6, 1e-3) if prec == "fp32" else (1e-12, RES_RTOL)
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, prec) as s:
+        cols = [Y[:, c].copy() for c in range(K)]
+        _, its, cvs = s.solve_residuals_many(cols, tol=tol, maxiter=100_000)
+    worst = max(_rel(a, b) for a, b in zip(cols, tight))
+    _say("  iters", its[:4], "worst", worst)
+    assert all(cvs) and worst <= bar
