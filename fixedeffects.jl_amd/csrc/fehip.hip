@@ -126,6 +126,9 @@ struct fehip_solver {
     uint32_t *hist = nullptr, *sums = nullptr;
     int64_t sort_cap = 0;
     int poll_iters = 8;
+    // the iteration's CSR pass and the fused-pass fix-up items in one launch
+    // (FEHIP_SEG_FIX=0: separate launches)
+    bool seg_fix = true;
     // schedule overrides (fehip_set_schedule / fehip_autotune); 0 or -1 =
     // the default (environment, else sized by occupancy)
     int64_t sched_tiles_s = 0, sched_tiles_c = 0;
@@ -585,6 +588,7 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
     h->has64 = precision != FEHIP_FP32;
     h->has32 = precision != FEHIP_FP64;
     h->factored = env_int("FEHIP_FACTORED", 1, 0, 1) != 0;
+    h->seg_fix = env_int("FEHIP_SEG_FIX", 1, 0, 1) != 0;
     for (int g = 0; g < G; g++)
         if (interaction && interaction[g]) h->factored = false;
     if (const char *e = std::getenv("FEHIP_MIXED_TOL")) h->mixed_tol = std::atof(e);
@@ -792,7 +796,8 @@ struct Plan {
     int32_t total_tiles = 0, nseg = 0, ndense = 0, dense_blocks = 0, dense_chunk = 0, nitems = 0;
     int32_t dense_wpb = 1, dense_lds_wave = 0;
     std::vector<int> seg_fe, dense_fe;  // seg_fe[0] is FE s
-    std::vector<int64_t> seg_tp;        // positions per tile, per segmented FE
+    std::vector<int64_t> seg_ts0;       // per segmented FE: its first entry in tstart
+    int64_t *tstart = nullptr;          // tile start positions (+ N), FE after FE
     std::vector<int32_t> seg_nsegs;     // non-empty levels, per segmented FE
     std::vector<int32_t> tile0, blk0;
     std::vector<int64_t> drow0;
@@ -825,8 +830,8 @@ struct Plan {
     // plain iteration, split fix-up: FE s (and the fused few-level FEs')
     // items beside the CSR pass, the CSR FEs' (and a separate few-level
     // pass's) items after it
-    FixItem *items_sd = nullptr, *items_c = nullptr;
-    int32_t nitems_sd = 0, nitems_c = 0;
+    FixItem *items_sd = nullptr, *items_c = nullptr, *items_d = nullptr;
+    int32_t nitems_sd = 0, nitems_c = 0, nitems_d = 0;  // items_d: the fused few-level FEs' (projected part 1)
     void *zs = nullptr;
     ColOps *ops_one = nullptr, *ops_neg = nullptr, *ops_sub = nullptr;
 };
@@ -859,6 +864,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     P->fused_blocks = grid_for(P->ntiles_s, kWavesPerBlock);
     std::vector<FixItem> items;
     std::vector<uint8_t> sdirect(h->fe[h->s].n, 0);
+    std::vector<int64_t> tsh;  // tile starts (+ N) of every segmented FE, in order
     int32_t t0 = 0;
     std::vector<int> order{h->s};
     for (int g = 0; g < h->G; g++)
@@ -867,8 +873,29 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
         const FeInfo &f = h->fe[g];
         const int64_t tp = g == h->s ? P->R : Rc;
         P->seg_fe.push_back(g);
-        P->seg_tp.push_back(tp);
+        P->seg_ts0.push_back((int64_t)tsh.size());
         P->tile0.push_back(t0);
+        // FE s: fixed tiles of R positions (the fused pass's partition).  A
+        // CSR FE: cut at the last segment head at or before each target
+        // position start + Rc, unless that segment began at or before start
+        // (a segment longer than a tile): then it is cut at the target.
+        const size_t b0 = tsh.size();
+        if (g == h->s) {
+            for (int64_t q = 0; q < N; q += tp) tsh.push_back(q);
+        } else {
+            const int32_t *off = f.h_ioff.data();
+            for (int64_t q = 0; q < N;) {
+                tsh.push_back(q);
+                const int64_t tgt = q + tp;
+                if (tgt >= N) break;
+                const int64_t head = *(std::upper_bound(off, off + f.n + 1, tgt) - 1);
+                q = head > q ? head : tgt;
+            }
+        }
+        tsh.push_back(N);
+        const int64_t *tb = tsh.data() + b0;
+        const int64_t ntl = (int64_t)(tsh.size() - b0) - 1;
+        auto tile_of = [&](int64_t pos) -> int64_t { return (int64_t)(std::upper_bound(tb, tb + ntl + 1, pos) - tb) - 1; };
         int32_t nonempty = 0;
         for (int64_t k = 0; k < f.n; k++) nonempty += f.h_ioff[k + 1] > f.h_ioff[k] ? 1 : 0;
         P->seg_nsegs.push_back(nonempty);
@@ -883,7 +910,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
                 items.push_back(fi);
                 continue;
             }
-            const int64_t tfirst = st / tp, tlast = (en - 1) / tp;
+            const int64_t tfirst = tile_of(st), tlast = tile_of(en - 1);
             if (tfirst == tlast) {
                 if (g == h->s) sdirect[k] = 1;
                 continue;
@@ -897,8 +924,10 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             fi.count = (int32_t)(tlast - tfirst);
             items.push_back(fi);
         }
-        t0 += (int32_t)((N + tp - 1) / tp);
+        t0 += (int32_t)ntl;
     }
+    CHK(dalloc(h, &P->tstart, tsh.size()));
+    HIPCHK(hipMemcpyAsync(P->tstart, tsh.data(), sizeof(int64_t) * tsh.size(), hipMemcpyHostToDevice, h->stream));
     P->nseg = (int32_t)P->seg_fe.size();
     P->total_tiles = t0;
     // dense FEs.  If their levels fit the fused pass's LDS (<= 64 KB per
@@ -975,7 +1004,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     CHK(dalloc(h, &P->items, std::max<int32_t>(P->nitems, 1)));
     if (P->nitems)
         HIPCHK(hipMemcpyAsync(P->items, items.data(), sizeof(FixItem) * P->nitems, hipMemcpyHostToDevice, h->stream));
-    std::vector<FixItem> its, ito, isd, ic;  // alive until the synchronize below
+    std::vector<FixItem> its, ito, isd, ic, id;  // alive until the synchronize below
     {
         for (const FixItem &fi : items) {
             if (fi.fe != h->s) {
@@ -990,7 +1019,13 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             const bool dense = h->fe[fi.fe].kind == KIND_DENSE;
             if (fi.fe == h->s || (dense && P->fused_dense)) isd.push_back(fi);
             else ic.push_back(fi);
+            if (fi.fe != h->s && dense && P->fused_dense) id.push_back(fi);
         }
+        P->nitems_d = (int32_t)id.size();
+        CHK(dalloc(h, &P->items_d, std::max<int32_t>(P->nitems_d, 1)));
+        if (P->nitems_d)
+            HIPCHK(hipMemcpyAsync(P->items_d, id.data(), sizeof(FixItem) * P->nitems_d, hipMemcpyHostToDevice,
+                                  h->stream));
         P->nitems_sd = (int32_t)isd.size();
         P->nitems_c = (int32_t)ic.size();
         CHK(dalloc(h, &P->items_sd, std::max<int32_t>(P->nitems_sd, 1)));
@@ -1096,7 +1131,8 @@ static void destroy_plan(fehip_solver *h, Plan *P) {
     if (P->graph) (void)hipGraphExecDestroy(P->graph);
     void *ptrs[] = {P->items, P->tileA, P->tileB, P->dpart, P->part_db, P->part_seg, P->part_fix, P->part_upd,
                     P->part_fused, P->wsum, P->sdirect, P->vt, P->state, P->ops_db, P->ops_dtx, P->ops_once, P->all_done, P->u, P->r, P->x,
-                    P->v, P->hh, P->hbar, P->dY};
+                    P->v, P->hh, P->hbar, P->dY, P->tstart, P->items_s, P->items_o, P->items_sd, P->items_c,
+                    P->zs, P->ops_one, P->ops_neg, P->ops_sub, P->items_d};
     for (void *q : ptrs) dfree(h, q);
     delete P;
 }
@@ -1402,9 +1438,8 @@ struct Engine {
     // csr_only: the tiles of the CSR FEs only (FE s is done by k_fused).
     // raw: y is the iteration's stored sqrt(w) .* u (factored form): sums of y
     // as it is, scaled per level (k_dtx_seg RAW)
-    static void dtx_seg(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
-                        const int32_t *early_exit, bool csr_only = false, bool raw = false) {
-        if (!P->nseg || (csr_only && !P->csr_blocks)) return;
+    static DtxSegParams<T> seg_params(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
+                                      const int32_t *early_exit, bool csr_only) {
         DtxSegParams<T> p{};
         p.N = h->N;
         p.nseg = P->nseg;
@@ -1421,9 +1456,9 @@ struct Engine {
             s.seg_level = f.seg_level;
             s.lscale = (const T *)f.scale_T[pidx<T>()];
             s.coef_off = h->off[P->seg_fe[q]];
-            s.tp = P->seg_tp[q];
+            s.tstart = P->tstart + P->seg_ts0[q];
             s.tile0 = P->tile0[q];
-            s.ntiles = (int32_t)((h->N + s.tp - 1) / s.tp);
+            s.ntiles = (q + 1 < P->nseg ? P->tile0[q + 1] : P->total_tiles) - P->tile0[q];
         }
         p.u = y;
         p.v = coef;
@@ -1433,6 +1468,13 @@ struct Engine {
         p.partials = P->part_seg;
         p.all_done = early_exit;
         p.mode = mode;
+        return p;
+    }
+
+    static void dtx_seg(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
+                        const int32_t *early_exit, bool csr_only = false, bool raw = false) {
+        if (!P->nseg || (csr_only && !P->csr_blocks)) return;
+        const DtxSegParams<T> p = seg_params(h, P, mode, ops, y, coef, early_exit, csr_only);
         const dim3 gr(csr_only ? P->csr_blocks : P->seg_blocks);
         if (raw) hipLaunchKernelGGL((k_dtx_seg<T, Acc, K, true>), gr, dim3(kBlock), 0, h->stream, p);
         else hipLaunchKernelGGL((k_dtx_seg<T, Acc, K>), gr, dim3(kBlock), 0, h->stream, p);
@@ -1556,7 +1598,7 @@ struct Engine {
         sf.nsegs = std::max<int32_t>(P->seg_nsegs[0], 1);
         sf.seg_level = f.seg_level;
         sf.coef_off = 0;
-        sf.tp = P->seg_tp[0];
+        sf.tstart = P->tstart + P->seg_ts0[0];
         sf.tile0 = 0;
         sf.ntiles = P->ntiles_s;
         q.u = y;
@@ -1660,15 +1702,14 @@ struct Engine {
     // part (plain fused iteration only): 1 = FE s streaming and the FE s /
     // fused few-level items (on `st`, beside the CSR pass), 2 = the other
     // items into the second half of the partials
-    static void dtx_fixup(fehip_solver *h, Plan *P, int mode, const ColOps *ops, T *coef,
-                          const int32_t *early_exit, bool fused = false, bool proj = false, int part = 0,
-                          hipStream_t st = nullptr) {
+    static FixParams<T> fix_params(fehip_solver *h, Plan *P, int mode, const ColOps *ops, T *coef,
+                                   const int32_t *early_exit, bool fused, bool proj, int part) {
         FixParams<T> f{};
         f.nitems = proj ? P->nitems_o : P->nitems;
         f.items = proj ? P->items_o : P->items;
         if (part == 1) {
-            f.nitems = P->nitems_sd;
-            f.items = P->items_sd;
+            f.nitems = proj ? P->nitems_d : P->nitems_sd;
+            f.items = proj ? P->items_d : P->items_sd;
         } else if (part == 2) {
             f.nitems = P->nitems_c;
             f.items = P->items_c;
@@ -1700,8 +1741,32 @@ struct Engine {
                 }
             }
         }
+        return f;
+    }
+
+    static void dtx_fixup(fehip_solver *h, Plan *P, int mode, const ColOps *ops, T *coef,
+                          const int32_t *early_exit, bool fused = false, bool proj = false, int part = 0,
+                          hipStream_t st = nullptr) {
+        const FixParams<T> f = fix_params(h, P, mode, ops, coef, early_exit, fused, proj, part);
         hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->fix_blocks), dim3(kBlock), 0, st ? st : h->stream, f);
     }
+
+    // the iteration's CSR pass and the fix-up part that needs only the fused
+    // pass (part 1: FE s, few-level FEs) in one launch, fix-up blocks first;
+    // part 2 (segments crossing CSR tiles) is launched after it.  Norm
+    // partials as in the split form (fin_dtx split).
+    static void seg_fix(fehip_solver *h, Plan *P, const ColOps *ops, const T *y, T *coef,
+                        const int32_t *early_exit, bool proj = false) {
+        const DtxSegParams<T> p = seg_params(h, P, MODE_ITER, ops, y, coef, early_exit, true);
+        const FixParams<T> f = fix_params(h, P, MODE_ITER, ops, coef, early_exit, true, proj, 1);
+        const int nfix = (int)P->fix_blocks;
+        const dim3 gr(P->fix_blocks + P->csr_blocks);
+        if (ut(h)) hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true>), gr, dim3(kBlock), 0, h->stream, p, f, nfix);
+        else hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, false>), gr, dim3(kBlock), 0, h->stream, p, f, nfix);
+    }
+
+    // the iteration takes the merged launch (FEs beside FE s in CSR form)
+    static bool use_seg_fix(const fehip_solver *h, const Plan *P) { return h->seg_fix && P->csr_blocks; }
 
     static void dtx(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
                     const int32_t *early_exit) {
@@ -1752,17 +1817,21 @@ struct Engine {
             fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
             mark(1);
             fin_db(h, P, MODE_ITER, P->all_done, true);
+            const bool sf = use_seg_fix(h, P);
             mark(2);
-            if (K >= kSplitFixK) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 1);
+            if (K >= kSplitFixK && !sf) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 1);
             mark(3);
             if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
             mark(4);
-            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
+            // the merged launch counts as the CSR slot, its part 2 as the fix-up
+            if (use_seg_fix(h, P)) seg_fix(h, P, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done);
+            else dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
             mark(5);
-            if (K >= kSplitFixK) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 2);
-            else dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
+            if ((K >= kSplitFixK && !sf) || (sf && P->nitems_c))
+                dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 2);
+            else if (!sf) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
             mark(6);
-            fin_dtx(h, P, MODE_ITER, P->all_done, true, K >= kSplitFixK);
+            fin_dtx(h, P, MODE_ITER, P->all_done, true, sf ? P->nitems_c > 0 : K >= kSplitFixK);
             mark(7);
             update(h, P);
             mark(8);
@@ -1879,15 +1948,32 @@ struct Engine {
             fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u, 2);
             fin_db(h, P, MODE_ITER, P->all_done, true);
             if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
-            dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
-            dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, true);
-            fin_dtx(h, P, MODE_ITER, P->all_done, true);
+            if (use_seg_fix(h, P)) {
+                seg_fix(h, P, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true);
+                if (P->nitems_c) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, true, 2);
+                fin_dtx(h, P, MODE_ITER, P->all_done, true, P->nitems_c > 0);
+            } else {
+                dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));
+                dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, true);
+                fin_dtx(h, P, MODE_ITER, P->all_done, true);
+            }
             update(h, P);
             fin_update(h, P);
             return;
         }
         fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
         fin_db(h, P, MODE_ITER, P->all_done, true);
+        if (use_seg_fix(h, P)) {
+            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
+            // part 2 only if some CSR segment crosses tiles (or a level is
+            // empty, or the few-level FEs ran separately)
+            seg_fix(h, P, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done);
+            if (P->nitems_c) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 2);
+            fin_dtx(h, P, MODE_ITER, P->all_done, true, P->nitems_c > 0);
+            update(h, P);
+            fin_update(h, P);
+            return;
+        }
         if (K < kSplitFixK) {
             if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, ut(h));
             dtx_seg(h, P, MODE_ITER, P->ops_dtx, (const T *)P->u, (T *)P->v, P->all_done, true, ut(h));

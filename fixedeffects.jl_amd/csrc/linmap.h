@@ -426,11 +426,13 @@ __global__ __launch_bounds__(kBlock) void k_db(DbParams<T> p) {
 // D'y: coef[k,c] <- b[c]*coef[k,c] + a[c] * sum_{i in level k} cache[i]*y[i,c]
 // (reference gather!/mul!', src/SolverCPU.jl:21-27, src/AbstractFixedEffectLinearMap.jl:26-35)
 //
-// Segmented FEs (FE s and CSR FEs) are cut into tiles of tp CSR positions
-// (one size per FE), one wave per tile.  A segment that lies inside a tile is reduced and
-// written by that wave.  A segment crossing tile boundaries leaves a partial
-// in its first tile (tileB) and in each later tile it covers (tileA); the
-// fix-up launch combines them in tile order.
+// Segmented FEs (FE s and CSR FEs) are cut into tiles of CSR positions, one
+// wave per tile: FE s into tiles of a fixed multiple of 256 positions (the
+// fused pass's partition), a CSR FE at segment heads near a target size, so
+// that only segments longer than a tile cross tiles.  A segment that lies
+// inside a tile is reduced and written by that wave.  A segment crossing tile
+// boundaries leaves a partial in its first tile (tileB) and in each later tile
+// it covers (tileA); the fix-up launch combines them in tile order.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct SegFE {
@@ -440,8 +442,8 @@ struct SegFE {
     const uint32_t *wprefix;  // heads before each 64-position word
     const int32_t *seg_level; // ordinal of non-empty segment -> level
     const T *lscale;          // RAW form: per-level scale applied to each level's sum (cache not read)
+    const int64_t *tstart;    // tile i covers positions [tstart[i], tstart[i+1])
     int64_t coef_off;
-    int64_t tp;               // positions per tile (a multiple of 256)
     int32_t tile0, ntiles;
     int32_t nsegs, pad;       // non-empty segments
 };
@@ -537,9 +539,10 @@ __device__ __forceinline__ int32_t seg_level_of(uint32_t wp, const int32_t *seg_
 // the segment sums are of the stored vector as it is, and each level's sum
 // takes the FE's scale (lscale; crossing segments take it in the fix-up), so
 // the CSR-ordered cache is not read.
-template <typename T, typename Acc, int K, bool RAW = false>
-__global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
-    if (p.all_done && *p.all_done) return;
+// the body of one launch block (bid of nblk): launched alone (k_dtx_seg) or
+// as the later blocks of k_dtx_seg_fix
+template <typename T, typename Acc, int K, bool RAW>
+__device__ __forceinline__ void dtx_seg_body(const DtxSegParams<T> &p, int bid, int nblk) {
     constexpr int P = kWave / K;
     constexpr int U = 4;  // wave steps whose loads are issued together
     const int lane = threadIdx.x & (kWave - 1);
@@ -547,15 +550,26 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
     const int on = p.ops->on[c];
     const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
     const bool keep = p.ops->b[c] != 0.0;
-    const int gw = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
-    const int nw = gridDim.x * kWavesPerBlock;
+    const int gw = bid * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
+    const int nw = nblk * kWavesPerBlock;
     double sq = 0.0;
     for (int t = p.first_tile + gw; t < p.total_tiles; t += nw) {
         int f = 0;
         while (f + 1 < p.nseg && t >= p.fe[f + 1].tile0) f++;
         const SegFE<T> fe = p.fe[f];
-        const int64_t start = (int64_t)(t - fe.tile0) * fe.tp;
-        const int64_t end = start + fe.tp < p.N ? start + fe.tp : p.N;
+        const int64_t start = fe.tstart[t - fe.tile0];
+        const int64_t end = fe.tstart[t - fe.tile0 + 1];
+        // the walk starts at the word holding start; the scan sees the head
+        // bits of positions [start, end] only (end's bit ends the tile's last
+        // segment), so positions outside the tile neither open nor close runs
+        const int64_t wbeg = start & ~(int64_t)63;
+        auto clip = [&](uint64_t w, int64_t wi) -> uint64_t {
+            const int64_t lo = start - (wi << 6), hi = end - (wi << 6);
+            if (hi < 0 || lo > 63) return 0ull;
+            uint64_t m = lo > 0 ? ~0ull << lo : ~0ull;
+            if (hi < 63) m &= (2ull << hi) - 1ull;
+            return w & m;
+        };
         T *v = p.v + fe.coef_off * K;
         SegCursor<Acc> cur{Acc(0.f), true, false};
         // per lane, one loop step ahead: the observation each position reads
@@ -569,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
             for (int q = 0; q < U; q++) {
                 const int64_t base = b0 + q * P;
                 const int64_t pos = base + psub;
-                const int64_t pc = pos < end ? pos : start;
+                const int64_t pc = (pos >= start && pos < end) ? pos : start;
                 in[q] = __builtin_nontemporal_load(fe.perm + pc);
                 hwn[q] = hnn[q] = 0;
                 wpn[q] = 0;
@@ -580,8 +594,8 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
                 }
             }
         };
-        meta(start);
-        for (int64_t base0 = start; base0 < end; base0 += U * P) {
+        meta(wbeg);
+        for (int64_t base0 = wbeg; base0 < end; base0 += U * P) {
             int32_t iq[U];
             uint64_t hw[U], hn[U];
             int32_t lev[U];
@@ -603,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const int64_t pos = base0 + q * P + psub;
-                const int64_t pc = pos < end ? pos : start;
+                const int64_t pc = (pos >= start && pos < end) ? pos : start;
                 cv[q] = RAW ? fe.lscale[lev[q]] : __builtin_nontemporal_load(fe.cache + pc);
                 uv[q] = p.u[(int64_t)iq[q] * K + c];
                 ov[q] = v[(int64_t)lev[q] * K + c];
@@ -614,7 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
             for (int q = 0; q < U; q++) {
                 const int64_t pos = base0 + q * P + psub;
                 const Acc pr = RAW ? Acc(uv[q]) : mul_acc<Acc, T>(cv[q], uv[q]);
-                val[q] = (pos < end && on) ? pr : Acc(0.f);
+                val[q] = (pos >= start && pos < end && on) ? pr : Acc(0.f);
             }
             // every em[q] is defined before the scan, which stops at the
             // first step past the tile's end
@@ -625,7 +639,9 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
             for (int q = 0; q < U; q++) {
                 const int64_t base = base0 + q * P;
                 if (base >= end) break;  // wave-uniform
-                em[q] = seg_step<K>(hw[q], hn[q], (int)(base & 63), base + psub < end, psub, c, on, val[q], cur,
+                const int64_t wi = base >> 6;
+                em[q] = seg_step<K>(clip(hw[q], wi), clip(hn[q], wi + 1), (int)(base & 63),
+                                    base + psub >= start && base + psub < end, psub, c, on, val[q], cur,
                                     &p.tileA[(int64_t)t * K + c]);
             }
 #pragma unroll
@@ -646,7 +662,13 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
     if (p.mode == MODE_NONE) return;
     __shared__ double red[kBlock];
     const double s = block_column_sum<K>(sq, red);
-    if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+    if (threadIdx.x < K) p.partials[(int64_t)bid * K + threadIdx.x] = s;
+}
+
+template <typename T, typename Acc, int K, bool RAW = false>
+__global__ __launch_bounds__(kBlock) void k_dtx_seg(DtxSegParams<T> p) {
+    if (p.all_done && *p.all_done) return;
+    dtx_seg_body<T, Acc, K, RAW>(p, blockIdx.x, gridDim.x);
 }
 
 // FEs with few levels: one wave per chunk of observations (internal order,
@@ -1256,32 +1278,58 @@ struct FixParams {
 };
 
 template <typename T, int K>
-__global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
-    if (p.all_done && *p.all_done) return;
+__device__ __forceinline__ void dtx_fixup_body(const FixParams<T> &p, int bid, int nblk) {
     constexpr int P = kWave / K;
     const int lane = threadIdx.x & (kWave - 1);
     const int psub = lane / K, c = lane % K;
-    const int gw = blockIdx.x * kWavesPerBlock + (threadIdx.x / kWave);
-    const int nw = gridDim.x * kWavesPerBlock;
+    const int gw = bid * kWavesPerBlock + (threadIdx.x / kWave);
+    const int nw = nblk * kWavesPerBlock;
     const int on = p.ops->on[c];
     const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
     const bool keep = p.ops->b[c] != 0.0;
     double sq = 0.0;
-    if (p.wsum) {
-        // stride is a multiple of K, so element e belongs to column c
-        const int64_t nsK = p.ns * K;
-        for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nsK; e += (int64_t)gridDim.x * kBlock) {
-            if (!on || !p.sdirect[e / K]) continue;
-            T *dst = p.v + p.s_off * K + e;
-            const double tot = p.fscale[p.s] ? (double)p.fscale[p.s][e / K] * p.wsum[e] : p.wsum[e];
-            T out = a * (T)tot;
-            if (keep) out = b * (*dst) + out;
-            *dst = out;
-            sq += (double)out * (double)out;
+    if (p.wsum && on) {
+        // stride is a multiple of K, so element e belongs to column c.  The
+        // loads of kU elements are issued together, none waiting on sdirect
+        // (every index is in range; values of levels not taken are dropped),
+        // so a thread pays one memory round trip per kU elements.
+        constexpr int kU = 4;
+        const int64_t nsK = p.ns * K, stride = (int64_t)nblk * kBlock;
+        const T *fs = p.fscale[p.s];
+        T *vs = p.v + p.s_off * K;
+        int64_t e = (int64_t)bid * kBlock + threadIdx.x;
+        for (; e < nsK; e += kU * stride) {
+            uint8_t dir[kU];
+            double ws[kU];
+            T f[kU], old[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int64_t q = e + u * stride < nsK ? e + u * stride : e;
+                dir[u] = e + u * stride < nsK ? p.sdirect[q / K] : 0;
+                ws[u] = p.wsum[q];
+                f[u] = fs ? fs[q / K] : (T)1;
+                old[u] = keep ? vs[q] : (T)0;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                if (!dir[u]) continue;
+                const double tot = fs ? (double)f[u] * ws[u] : ws[u];
+                T out = a * (T)tot;
+                if (keep) out = b * old[u] + out;
+                vs[e + u * stride] = out;
+                sq += (double)out * (double)out;
+            }
         }
     }
     for (int item = gw; item < p.nitems; item += nw) {
         const FixItem it = p.items[item];
+        // the loads that do not depend on the sum go out first; each level
+        // is one item's, so no other wave writes dst in this launch
+        T *dst = p.v + (it.coef_off + it.level) * K + c;
+        const T *fsp = it.kind != 2 ? p.fscale[it.fe] : nullptr;
+        const double tb = on && it.kind == 0 ? p.tileB[it.first * K + c] : 0.0;
+        const T fsv = on && fsp ? fsp[it.level] : (T)1;
+        const T old = on && keep ? *dst : (T)0;
         double s = 0.0;
         if (on && it.kind != 2) {
             // rows [lo, hi) of `src`, lane-strided, four independent chains
@@ -1301,11 +1349,10 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
         }
         s = column_allreduce<K>(s);
         if (on && psub == 0) {
-            if (it.kind == 0) s = p.tileB[it.first * K + c] + s;
-            if (it.kind != 2 && p.fscale[it.fe]) s = (double)p.fscale[it.fe][it.level] * s;
-            T *dst = p.v + (it.coef_off + it.level) * K + c;
+            if (it.kind == 0) s = tb + s;
+            if (fsp) s = (double)fsv * s;
             T out = a * (T)s;
-            if (keep) out = b * (*dst) + out;
+            if (keep) out = b * old + out;
             *dst = out;
             sq += (double)out * (double)out;
         }
@@ -1313,7 +1360,25 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
     if (p.mode == MODE_NONE) return;
     __shared__ double red[kBlock];
     const double s = block_column_sum<K>(sq, red);
-    if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+    if (threadIdx.x < K) p.partials[(int64_t)bid * K + threadIdx.x] = s;
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
+    if (p.all_done && *p.all_done) return;
+    dtx_fixup_body<T, K>(p, blockIdx.x, gridDim.x);
+}
+
+// The fix-up items that need only the fused pass (FE s, few-level FEs) and
+// the CSR pass in one launch: blocks [0, nfix) run the fix-up, the rest the
+// CSR tiles.  The fix-up's short dependent loads then overlap the CSR
+// pass's stream instead of running after it.  The two parts touch disjoint
+// levels of v and leave their norm partials in separate arrays.
+template <typename T, typename Acc, int K, bool RAW>
+__global__ __launch_bounds__(kBlock) void k_dtx_seg_fix(DtxSegParams<T> p, FixParams<T> f, int nfix) {
+    if (p.all_done && *p.all_done) return;
+    if ((int)blockIdx.x < nfix) dtx_fixup_body<T, K>(f, blockIdx.x, nfix);
+    else dtx_seg_body<T, Acc, K, RAW>(p, (int)blockIdx.x - nfix, (int)gridDim.x - nfix);
 }
 
 // ---------------------------------------------------------------------------

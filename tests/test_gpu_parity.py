@@ -510,3 +510,34 @@ def test_wide_batch_few_level_fes(gpu, prec):
     worst = max(_rel(a, b) for a, b in zip(cols, tight))
     _say("  iters", its[:4], "worst", worst)
     assert all(cvs) and worst <= bar
+
+
+@pytest.mark.parametrize("algorithm,K", [("lsmr", 1), ("lsmr", 4), ("lsmr", 16), ("projected", 1)])
+def test_csr_levels_longer_than_tiles(gpu, algorithm, K):
+    # a CSR FE with Zipf-sized levels, the largest spanning many CSR tiles:
+    # CSR tiles are cut at segment heads except inside such levels, whose
+    # partials the fix-up launch after the CSR pass combines (here also with
+    # small CSR tiles, which cut many more levels); the converged residuals
+    # match the oracle's tight solve
+    _say(f"case: CSR levels longer than tiles {algorithm} K={K}")
+    rng = np.random.default_rng(40 + K)
+    N = 200_000 if K < 16 else 100_000
+    r0 = rng.integers(1, 20_001, N)  # most levels: FE s
+    sizes = synth.zipf_sizes(N, 3000, 1.1)
+    r1 = np.repeat(np.arange(1, 3001), sizes)
+    rng.shuffle(r1)
+    r2 = rng.integers(1, 13, N)
+    refs = [r0.astype(np.int64), r1.astype(np.int64), r2.astype(np.int64)]
+    n = [20_000, 3000, 12]
+    w = rng.lognormal(0, 0.5, N)
+    Y = rng.standard_normal((N, K))
+    o = oracle.OracleSolver(refs, n, None, w, nthreads=8)
+    tight = [o.solve_residuals(Y[:, c].copy(), tol=1e-13, maxiter=100_000)[0] for c in range(K)]
+    with api.FixedEffectSolverROCm(_fes(refs, n), w, "fp64", algorithm=algorithm) as s:
+        for sched in ((0, 0, -1, 0), (0, 3000, -1, 0)):
+            s.set_schedule(*sched)
+            cols = [np.ascontiguousarray(Y[:, c]).copy() for c in range(K)]
+            _, its, cvs = s.solve_residuals_many(cols, tol=1e-12, maxiter=100_000)
+            worst = max(_rel(cols[c], tight[c]) for c in range(K))
+            _say(f"  schedule {sched}: iters {its[:4]} worst rel err {worst:.2e}")
+            assert all(cvs) and worst <= RES_RTOL

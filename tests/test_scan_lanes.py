@@ -155,3 +155,124 @@ def test_level_from_head_words():
                                 continue
                             got = _level_s(words, wprefix, seg_level, nsegs, start, end, lb, q, psub, P, NW)
                             assert got == keys[posn], (trial, K, U, start, lb, q, psub)
+
+
+# -- CSR tile walk (linmap.h dtx_seg_body + seg_step + the fix-up's kind-0 items) --
+
+def _clip(w, wi, start, end):
+    """The head bits of positions [start, end] of word wi (dtx_seg_body clip)."""
+    lo, hi = int(start) - (int(wi) << 6), int(end) - (int(wi) << 6)
+    if hi < 0 or lo > 63:
+        return 0
+    m = ((1 << 64) - 1) << lo if lo > 0 else (1 << 64) - 1
+    if hi < 63:
+        m &= (2 << hi) - 1
+    return w & m & ((1 << 64) - 1)
+
+
+def _seg_step(w, wnext, sh, valid, x, cur, P):
+    """seg_step for one column, lane by lane: returns (scanned x, lanes that
+    emit, lanes that write tileA)."""
+    hb = w if P == 64 else (w >> sh) & ((1 << P) - 1)
+    nb = (w >> (sh + P)) & 1 if sh + P < 64 else wnext & 1
+    x = x.copy()
+    if not (hb & 1):
+        x[0] += cur["carry"]
+    out = np.zeros(P)
+    run = 0.0
+    for ps in range(P):
+        run = x[ps] if (hb >> ps) & 1 else run + x[ps]
+        out[ps] = run
+    em, ta = [], []
+    for ps in range(P):
+        hle = hb & ((2 << ps) - 1)
+        isend = nb if ps == P - 1 else (hb >> (ps + 1)) & 1
+        if valid[ps] and isend:
+            (ta if (cur["cont"] and hle == 0) else em).append(ps)
+    cur["open"] = nb == 0
+    cur["carry"] = out[P - 1] if cur["open"] else 0.0
+    cur["cont"] = cur["cont"] and hb == 0
+    return out, em, ta
+
+
+def _walk(words, val, level_of, tiles, P):
+    """Every tile walked as the kernel does (from the word holding its start,
+    P positions a step); returns the directly written level sums and the
+    tileA / tileB partials."""
+    direct, tileA, tileB = {}, {}, {}
+    for t, (start, end) in enumerate(tiles):
+        start, end = int(start), int(end)
+        cur = {"carry": 0.0, "cont": True, "open": False}
+        for base in range(start & ~63, end, P):
+            wi = base >> 6
+            w, wn = _clip(int(words[wi]), wi, start, end), _clip(int(words[wi + 1]), wi + 1, start, end)
+            pos = base + np.arange(P)
+            valid = (pos >= start) & (pos < end)
+            x = np.where(valid, val[np.clip(pos, 0, len(val) - 1)], 0.0)
+            out, em, ta = _seg_step(w, wn, base & 63, valid, x, cur, P)
+            for ps in em:
+                k = level_of[pos[ps]]
+                assert k not in direct
+                direct[k] = out[ps]
+            for ps in ta:
+                tileA[t] = out[ps]
+        if cur["open"]:
+            (tileA if cur["cont"] else tileB)[t] = cur["carry"]
+    return direct, tileA, tileB
+
+
+def _tiles_at_heads(off, n, tp):
+    """CSR tile starts as the plan cuts them: at the last head at or before
+    each target, or at the target inside a segment longer than a tile."""
+    q, starts = 0, []
+    while q < n:
+        starts.append(q)
+        tgt = q + tp
+        if tgt >= n:
+            break
+        head = off[np.searchsorted(off, tgt, side="right") - 1]
+        q = head if head > q else tgt
+    starts.append(n)
+    return starts
+
+
+def test_csr_tile_walk_at_heads_and_inside_segments():
+    # segment sums from the walk + the fix-up's combination equal the direct
+    # sums, for tiles cut at heads (the plan's cut), tiles cut anywhere, and
+    # the old fixed tiles; long segments span several tiles
+    rng = np.random.default_rng(11)
+    for trial in range(8):
+        n = int(rng.integers(500, 4000))
+        sizes = rng.geometric(1 / rng.choice([3, 20, 150]), size=n)
+        sizes[rng.integers(0, len(sizes), 2)] += int(rng.integers(300, 900))  # long segments
+        keys = np.repeat(np.arange(len(sizes)), sizes)[:n]
+        nlev = keys.max() + 1
+        off = np.concatenate([[0], np.cumsum(np.bincount(keys, minlength=nlev))])
+        words, _ = _heads_and_prefix(keys, n)
+        val = rng.standard_normal(n)
+        want = np.bincount(keys, weights=val, minlength=nlev)
+        tp = 256 * int(rng.integers(1, 3))
+        cuts = {
+            "heads": _tiles_at_heads(off, n, tp),
+            "anywhere": sorted(set([0, n] + list(rng.choice(np.arange(1, n), size=n // 97, replace=False)))),
+            "fixed": list(range(0, n, tp)) + [n],
+        }
+        for name, st in cuts.items():
+            tiles = list(zip(st[:-1], st[1:]))
+            tstart = np.array(st)
+            for K in (1, 4, 32):
+                P = 64 // K
+                direct, tileA, tileB = _walk(words, val, keys, tiles, P)
+                crossing = 0
+                for k in range(nlev):
+                    a, b = off[k], off[k + 1]
+                    tf = np.searchsorted(tstart, a, side="right") - 1
+                    tl = np.searchsorted(tstart, b - 1, side="right") - 1
+                    if tf == tl:
+                        got = direct[k]
+                    else:
+                        crossing += 1
+                        got = tileB[tf] + sum(tileA[q] for q in range(tf + 1, tl + 1))
+                    assert abs(got - want[k]) <= 1e-9 * (1 + abs(want[k])), (trial, name, K, k)
+                if name == "heads":  # only segments longer than a tile cross
+                    assert crossing <= sum((off[k + 1] - off[k]) > tp for k in range(nlev)), (trial, crossing)
