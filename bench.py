@@ -45,8 +45,7 @@ def log(*a):
 # (scaled by a plain device copy of known size), by (workload, precision):
 # profiles/worker_firm_1e7_fp64_k1.md
 PMC_TRAFFIC = {
-    ("worker_firm_1e7", "fp64"): {"k_fused": 324.5e6, "k_dtx_csr": 277.3e6, "k_update": 88.1e6,
-                                  "k_dtx_fixup": 25.3e6},
+    ("worker_firm_1e7", "fp64"): {"k_fused": 324.1e6, "k_dtx_csr": 311.1e6, "k_update": 88.1e6},
 }
 
 
@@ -76,8 +75,15 @@ def kernel_bytes(N, G, n, kinds, K, s, factored, weighted):
             dense += N * (4 + s) + K * N * s
     out["k_dtx_csr"] = csr
     out["k_dtx_dense"] = dense
-    # FE s levels: raw sum (double) + v read + write, one flag byte (+ scale)
-    out["k_dtx_fixup"] = K * n[gs] * (8 + 2 * s) + n[gs] + (n[gs] * s if factored else 0)
+    # FE s levels: raw sum (double) + v read + write, one flag byte (+ scale).
+    # With CSR FEs these items run as the first blocks of the CSR launch
+    # (k_dtx_seg_fix), whose slot then carries their bytes
+    fix = K * n[gs] * (8 + 2 * s) + n[gs] + (n[gs] * s if factored else 0)
+    if csr:
+        out["k_dtx_csr"] += fix
+        out["k_dtx_fixup"] = 0
+    else:
+        out["k_dtx_fixup"] = fix
     # v, h, hbar, x read + write (+ scale read, scale .* v written)
     out["k_update"] = 8 * K * p * s + ((K + 1) * p * s if factored else 0)
     # the pair formula of the survey: D*v + D'u over all G FEs with perm+cache each

@@ -1031,8 +1031,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
         };
         if constexpr (GT > 0) {
-            // stage A: refs of a step (FE s: none, its level comes from the
-            // head words, loaded here with their prefixes)
+            // A four-stage software pipeline over loop steps, so that every
+            // wait targets loads issued one loop step earlier (vector loads
+            // complete in order, so a wait for a load also waits for every
+            // older one):
+            //   A  refs of step +3 (FE s: none, its level comes from the head
+            //      words, loaded here with their prefixes)
+            //   L  FE s levels of step +2 (seg_level loads; packed refs split)
+            //   B  every other load of step +1 (gathers use its refs, levels)
+            //   compute step 0
             struct Refs {
                 int32_t r[GA][U];
                 uint64_t hw[NW];
@@ -1047,11 +1054,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
                 load_words_wp(lb, rr.hw, rr.wp);
             };
-            // stage B: every other load of a step (gathers use its refs)
-            struct InFlight {
-                T uo[U], cv[FACT ? 1 : GA][U], gv[GA][U], sw[U];
-            };
-            auto issue_data = [&](uint32_t lb, Refs &rr, InFlight &d) {
+            auto issue_level = [&](uint32_t lb, Refs &rr) {
 #pragma unroll
                 for (int q = 0; q < U; q++) rr.r[0][q] = level_s(lb, q, rr.hw, rr.wp);
                 if constexpr (GT == 3) {
@@ -1064,6 +1067,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         rr.r[2][q] = (int32_t)((uint32_t)rr.r[2][q] >> shift);
                     }
                 }
+            };
+            // a step's loads in flight, with the refs and head words that
+            // compute needs (copied, so the Refs can move on)
+            struct InFlight {
+                T uo[U], cv[FACT ? 1 : GA][U], gv[GA][U], sw[U];
+                int32_t rs[U], rd[U];
+                uint64_t hw[NW];
+            };
+            auto issue_data = [&](uint32_t lb, const Refs &rr, InFlight &d) {
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     const uint32_t i = loc(lb, q);
@@ -1083,16 +1095,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                         if constexpr (!FACT) d.cv[g][q] = ldo(p.cache[g] + start, loc(lb, q));
                     }
                 }
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    d.rs[q] = rr.r[0][q];
+                    d.rd[q] = rr.r[GT - 1][q];
+                }
+#pragma unroll
+                for (int j = 0; j < NW; j++) d.hw[j] = rr.hw[j];
             };
             // the step's loads have arrived: form the D*v sums (FEs in order)
-            auto settle = [&](const InFlight &d, const Refs &rr, Step &st) {
+            auto settle = [&](const InFlight &d, Step &st) {
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     st.sum[q] = (T)0;
                     st.g0[q] = d.gv[0][q];
                     st.uo[q] = d.uo[q];
                     st.cs[q] = st.cd[q] = (T)0;
-                    st.rs[q] = st.rd[q] = 0;
+                    st.rs[q] = d.rs[q];
+                    st.rd[q] = p.ndense ? d.rd[q] : 0;
                 }
                 if constexpr (FACT) {
 #pragma unroll
@@ -1110,41 +1130,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                         for (int q = 0; q < U; q++)
                             st.sum[q] += (PM == 1 && g == 0) ? (T)0 : d.cv[g][q] * d.gv[g][q];
-                }
 #pragma unroll
-                for (int g = 0; g < GT; g++) {
-                    if (g == 0) {
-#pragma unroll
-                        for (int q = 0; q < U; q++) {
-                            if constexpr (!FACT) st.cs[q] = d.cv[g][q];
-                            st.rs[q] = rr.r[g][q];
-                        }
-                    }
-                    if (g == GT - 1 && p.ndense) {
-#pragma unroll
-                        for (int q = 0; q < U; q++) {
-                            if constexpr (!FACT) st.cd[q] = d.cv[g][q];
-                            st.rd[q] = rr.r[g][q];
-                        }
+                    for (int q = 0; q < U; q++) {
+                        st.cs[q] = d.cv[0][q];
+                        if (p.ndense) st.cd[q] = d.cv[GT - 1][q];
                     }
                 }
 #pragma unroll
-                for (int j = 0; j < NW; j++) st.hw[j] = rr.hw[j];
+                for (int j = 0; j < NW; j++) st.hw[j] = d.hw[j];
             };
-            Refs r1, r2;
+            // prologue: data of step 0 in flight, levels of step 1 issued,
+            // refs of step 2 issued
+            Refs ra, rb, rc;
             InFlight d;
             Step st;
-            issue_refs(0u, r1);
-            issue_data(0u, r1, d);
-            issue_refs(UP, r2);
-            settle(d, r1, st);
-            r1 = r2;
+            issue_refs(0u, rc);
+            issue_refs(UP, rb);
+            issue_refs(2 * UP, ra);
+            issue_level(0u, rc);
+            issue_level(UP, rb);
+            issue_data(0u, rc, d);
+            rc = rb;
+            rb = ra;
             for (uint32_t lb = 0; lb < len; lb += UP) {
-                issue_refs(lb + 2 * UP, r2);
-                issue_data(lb + UP, r1, d);
+                issue_refs(lb + 3 * UP, ra);
+                issue_level(lb + 2 * UP, rb);
+                settle(d, st);
+                issue_data(lb + UP, rc, d);
                 compute(st, lb);
-                settle(d, r1, st);
-                r1 = r2;
+                rc = rb;
+                rb = ra;
             }
         } else {
             for (uint32_t lb = 0; lb < len; lb += UP) {
@@ -1413,22 +1428,40 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdParams<T> p) {
     const T ia = (T)st[c].inv_alpha, chb = (T)st[c].c_hbar, cx = (T)st[c].c_x, ch = (T)st[c].c_h;
     double sq = 0.0;
     if (act) {
-        const int64_t n = p.pK - p.skipn;
-        for (int64_t e2 = (int64_t)blockIdx.x * kBlock + threadIdx.x; e2 < n; e2 += (int64_t)gridDim.x * kBlock) {
-            const int64_t e = e2 < p.skip0 ? e2 : e2 + p.skipn;
-            T vn = p.v[e];
-            if (!vs) {
-                vn = vn * ia;
-                p.v[e] = vn;
+        // kU grid-stride elements per trip, their loads issued together (a
+        // thread's elements past n read element e0 and are not stored)
+        constexpr int kU = 4;
+        const int64_t n = p.pK - p.skipn, stride = (int64_t)gridDim.x * kBlock;
+        for (int64_t e0 = (int64_t)blockIdx.x * kBlock + threadIdx.x; e0 < n; e0 += kU * stride) {
+            int64_t ee[kU];
+            T vv[kU], hv[kU], hbv[kU], xv[kU], sc[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int64_t e2 = e0 + u * stride < n ? e0 + u * stride : e0;
+                ee[u] = e2 < p.skip0 ? e2 : e2 + p.skipn;
+                vv[u] = p.v[ee[u]];
+                hv[u] = p.h[ee[u]];
+                hbv[u] = p.hbar[ee[u]];
+                xv[u] = p.x[ee[u]];
+                sc[u] = p.vt ? p.scale[ee[u] / K] : (T)0;
             }
-            if (p.vt) p.vt[e] = p.scale[e / K] * vn;
-            const T hv = p.h[e];
-            const T hb = p.hbar[e] * chb + hv;
-            p.hbar[e] = hb;
-            const T xv = p.x[e] + cx * hb;
-            p.x[e] = xv;
-            p.h[e] = hv * ch + vn;
-            sq += (double)xv * (double)xv;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                if (e0 + u * stride >= n) continue;
+                const int64_t e = ee[u];
+                T vn = vv[u];
+                if (!vs) {
+                    vn = vn * ia;
+                    p.v[e] = vn;
+                }
+                if (p.vt) p.vt[e] = sc[u] * vn;
+                const T hb = hbv[u] * chb + hv[u];
+                p.hbar[e] = hb;
+                const T xn = xv[u] + cx * hb;
+                p.x[e] = xn;
+                p.h[e] = hv[u] * ch + vn;
+                sq += (double)xn * (double)xn;
+            }
         }
     }
     __shared__ double red[kBlock];
