@@ -541,3 +541,29 @@ def test_csr_levels_longer_than_tiles(gpu, algorithm, K):
             worst = max(_rel(cols[c], tight[c]) for c in range(K))
             _say(f"  schedule {sched}: iters {its[:4]} worst rel err {worst:.2e}")
             assert all(cvs) and worst <= RES_RTOL
+
+
+@pytest.mark.parametrize("algorithm,K", [("lsmr", 1), ("lsmr", 16), ("projected", 1)])
+def test_separate_csr_and_fixup_launches(gpu, monkeypatch, algorithm, K):
+    # FEHIP_SEG_FIX=0 (read when a handle is created): the CSR pass and the
+    # fix-up as separate launches (at K >= 16 the fix-up's FE s part on a
+    # second graph branch) give the merged launch's converged residuals
+    _say(f"case: separate CSR / fix-up launches {algorithm} K={K}")
+    p = synth.config("worker_firm_1e7", scale=0.01)
+    rng = np.random.default_rng(7)
+    Y = np.column_stack([p.Y[:, 0]] + [rng.standard_normal(p.N) for _ in range(K - 1)])
+    o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=8)
+    tight = [o.solve_residuals(Y[:, c].copy(), tol=1e-13, maxiter=100_000)[0] for c in range(K)]
+    out = {}
+    for sf in ("1", "0"):
+        monkeypatch.setenv("FEHIP_SEG_FIX", sf)
+        with api.FixedEffectSolverROCm(_fes(p.refs, p.n, p.interactions), p.weights, "fp64",
+                                       algorithm=algorithm) as s:
+            cols = [np.ascontiguousarray(Y[:, c]).copy() for c in range(K)]
+            _, its, cvs = s.solve_residuals_many(cols, tol=1e-12, maxiter=100_000)
+        worst = max(_rel(cols[c], tight[c]) for c in range(K))
+        _say(f"  FEHIP_SEG_FIX={sf}: iters {its[:4]} worst rel err {worst:.2e}")
+        assert all(cvs) and worst <= RES_RTOL
+        out[sf] = cols
+    for a, b in zip(out["1"], out["0"]):
+        assert _rel(a, b) <= 1e-9
