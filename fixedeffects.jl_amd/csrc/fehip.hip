@@ -937,7 +937,9 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     // iteration with ~4096 chunks per FE and waves per workgroup bounded by
     // 64 KB of LDS.
     const int Pq = kWave / K;
-    const size_t accsz = ts == 8 ? sizeof(double) : sizeof(F32x2);
+    // accumulator bytes: FP64 double; FP32 modes a float per lane and level
+    // (DenseAcc in linmap.h: phasing is FP64-only)
+    const size_t accsz = ts == 8 ? sizeof(double) : sizeof(float);
     int maxlev = 1;
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) {
@@ -1083,7 +1085,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
 // LDS of the fused pass: the few-level FEs' accumulators when they fit 64 KB
 // per workgroup (the rule get_plan applies)
 static size_t fused_lds_bytes(const fehip_solver *h, size_t ts, int K) {
-    const size_t accsz = ts == 8 ? sizeof(double) : sizeof(F32x2);
+    const size_t accsz = ts == 8 ? sizeof(double) : sizeof(float);
     const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * accsz : (size_t)kWave * accsz;
     int64_t nslot = 0;
     for (int g = 0; g < h->G; g++)

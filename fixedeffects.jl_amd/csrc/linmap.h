@@ -706,6 +706,23 @@ __host__ __device__ constexpr bool dense_phased() {
     return K >= 16 && std::is_same<Acc, double>::value;
 }
 
+// Lane-private few-level accumulators (the form that is not phased): each
+// sums the positions of one lane and level within a chunk, a small share of
+// the level, and the lanes' sums are combined in FP64.  FP32 modes keep them
+// as plain floats, half the LDS of a compensated pair.  The share is below
+// chunk/64 positions, so the rounding stays far below that of one FP32 sum
+// over the whole level.
+template <typename Acc>
+struct LaneAccOf {
+    using type = Acc;
+};
+template <>
+struct LaneAccOf<F32x2> {
+    using type = float;
+};
+template <int K, typename Acc>
+using DenseAcc = typename std::conditional<dense_phased<K, Acc>(), Acc, typename LaneAccOf<Acc>::type>::type;
+
 // One phased add of U steps into per-wave accumulators acc[slot*K + c] (FP64):
 // the lanes of phase ph (psub == ph) hold distinct columns, so no two lanes of
 // an instruction meet; LDS adds in a fixed order of steps and phases.
@@ -735,11 +752,12 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
     const DenseFE<T> fe = p.fe[f];
     const int bl = blockIdx.x - fe.blk0;
     constexpr bool PHASED = dense_phased<K, Acc>();  // per-column accumulators, as in k_fused
-    Acc *acc = reinterpret_cast<Acc *>(dense_smem) + (size_t)wave * fe.nlev * (PHASED ? K : kWave);
+    using DA = DenseAcc<K, Acc>;
+    DA *acc = reinterpret_cast<DA *>(dense_smem) + (size_t)wave * fe.nlev * (PHASED ? K : kWave);
     if constexpr (PHASED) {
-        for (int e = lane; e < fe.nlev * K; e += kWave) acc[e] = Acc(0.f);
+        for (int e = lane; e < fe.nlev * K; e += kWave) acc[e] = DA(0.f);
     } else {
-        for (int k = 0; k < fe.nlev; k++) acc[k * kWave + lane] = Acc(0.f);
+        for (int k = 0; k < fe.nlev; k++) acc[k * kWave + lane] = DA(0.f);
     }
     const int on = p.ops->on[c];
     const int64_t start = ((int64_t)bl * wpb + wave) * p.chunk;
@@ -748,7 +766,7 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
         constexpr int U = 4;
         for (int64_t base0 = start; base0 < end; base0 += U * P) {
             int32_t r[U];
-            Acc x[U];
+            DA x[U];
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const int64_t pos = base0 + q * P + psub;
@@ -756,14 +774,14 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
                 r[q] = __builtin_nontemporal_load(fe.refs + pc);
                 const T uv = p.u[pc * K + c];
                 if constexpr (RAW) {
-                    x[q] = pos < end ? Acc(uv) : Acc(0.f);
+                    x[q] = pos < end ? DA(uv) : DA(0.f);
                 } else {
                     const T cv = __builtin_nontemporal_load(fe.cache + pc);
-                    x[q] = pos < end ? mul_acc<Acc, T>(cv, uv) : Acc(0.f);
+                    x[q] = pos < end ? mul_acc<DA, T>(cv, uv) : DA(0.f);
                 }
             }
-            if constexpr (PHASED) phased_add<U, K, Acc>(acc, r, x, psub, c);
-            else lds_accumulate<U, Acc>(acc, r, x, lane);
+            if constexpr (PHASED) phased_add<U, K, DA>(acc, r, x, psub, c);
+            else lds_accumulate<U, DA>(acc, r, x, lane);
         }
     }
     if constexpr (!PHASED) {
@@ -772,15 +790,15 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
         // overlaps accumulators of levels <= k, already consumed (K <= 32).
         double *wsum = reinterpret_cast<double *>(acc);
         for (int k = 0; k < fe.nlev; k++) {
-            Acc x = acc[k * kWave + lane];
+            double x = acc_to_double(acc[k * kWave + lane]);
             x = column_allreduce<K>(x);
-            if (psub == 0) wsum[k * K + c] = acc_to_double(x);
+            if (psub == 0) wsum[k * K + c] = x;
         }
     }
     __syncthreads();
     // combine the waves in order; thread t handles entries t, t + blockDim, ...
     double *out = p.dpart + fe.dpart_row0 * K;
-    const size_t region = (size_t)fe.nlev * (PHASED ? K : kWave) * sizeof(Acc) / sizeof(double);
+    const size_t region = (size_t)fe.nlev * (PHASED ? K : kWave) * sizeof(DA) / sizeof(double);
     const double *base_ws = reinterpret_cast<const double *>(dense_smem);
     for (int e = threadIdx.x; e < fe.nlev * K; e += blockDim.x) {
         double s = 0.0;
@@ -907,15 +925,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // ordered phases with LDS adds -- a quarter to a half of the LDS
     constexpr bool PHASED = dense_phased<K, Acc>();
     constexpr bool UT = stores_scaled_u<K, FACT>();
-    Acc *acc = reinterpret_cast<Acc *>(fused_smem) + (size_t)wave * p.nslot * (PHASED ? K : kWave);
+    using DA = DenseAcc<K, Acc>;
+    DA *acc = reinterpret_cast<DA *>(fused_smem) + (size_t)wave * p.nslot * (PHASED ? K : kWave);
     if constexpr (PHASED) {
-        for (int e = lane; e < p.nslot * K; e += kWave) acc[e] = Acc(0.f);
+        for (int e = lane; e < p.nslot * K; e += kWave) acc[e] = DA(0.f);
     } else {
-        for (int k = 0; k < p.nslot; k++) acc[k * kWave + lane] = Acc(0.f);
+        for (int k = 0; k < p.nslot; k++) acc[k * kWave + lane] = DA(0.f);
     }
-    auto dense_add = [&](const int32_t (&sl)[U], const Acc (&x)[U]) {
-        if constexpr (PHASED) phased_add<U, K, Acc>(acc, sl, x, psub, c);
-        else lds_accumulate<U, Acc>(acc, sl, x, lane);
+    auto dense_add = [&](const int32_t (&sl)[U], const DA (&x)[U]) {
+        if constexpr (PHASED) phased_add<U, K, DA>(acc, sl, x, psub, c);
+        else lds_accumulate<U, DA>(acc, sl, x, lane);
     };
     const int t = blockIdx.x * kWavesPerBlock + wave;
     double sq = 0.0;
@@ -1009,11 +1028,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             if (PM != 1 && p.ndense) {
                 const int s0 = p.dense_slot0[0];
                 int32_t sl[U];
-                Acc x[U];
+                DA x[U];
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     sl[q] = s0 + st.rd[q];
-                    x[q] = mul_acc<Acc, T>(st.cd[q], un[q]);
+                    x[q] = mul_acc<DA, T>(st.cd[q], un[q]);
                 }
                 dense_add(sl, x);
                 for (int d = 1; d < p.ndense; d++) {
@@ -1024,7 +1043,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     for (int q = 0; q < U; q++) {
                         const uint32_t i = loc(lb, q);
                         sl[q] = sd + ldo(rf, i);
-                        x[q] = mul_acc<Acc, T>(FACT ? st.cs[q] : ldo(cf, i), un[q]);
+                        x[q] = mul_acc<DA, T>(FACT ? st.cs[q] : ldo(cf, i), un[q]);
                     }
                     dense_add(sl, x);
                 }
@@ -1235,13 +1254,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         if constexpr (!PHASED) {
             double *ws = reinterpret_cast<double *>(acc);
             for (int k = 0; k < p.nslot; k++) {
-                Acc x = acc[k * kWave + lane];
+                double x = acc_to_double(acc[k * kWave + lane]);
                 x = column_allreduce<K>(x);
-                if (psub == 0) ws[k * K + c] = acc_to_double(x);
+                if (psub == 0) ws[k * K + c] = x;
             }
         }
         __syncthreads();
-        const size_t region = (size_t)p.nslot * (PHASED ? K : kWave) * sizeof(Acc) / sizeof(double);
+        const size_t region = (size_t)p.nslot * (PHASED ? K : kWave) * sizeof(DA) / sizeof(double);
         const double *base_ws = reinterpret_cast<const double *>(fused_smem);
         for (int e = threadIdx.x; e < p.nslot * K; e += blockDim.x) {
             double s = 0.0;
