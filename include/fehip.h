@@ -266,10 +266,14 @@ int fehip_lsmr_info(fehip_t *h, int32_t col, double *out);
  *             with the raw D'u sums of the FE with most levels and of the
  *             few-level FEs (gather!, src/SolverCPU.jl:21-27)
  *   out_ms[1] beta = ||u|| finalize
- *   out_ms[2] D'u of the other many-level FEs (CSR-ordered gathers)
+ *   out_ms[2] D'u of the other many-level FEs (CSR-ordered gathers); by
+ *             default one launch whose first blocks also do the fix-up of
+ *             the first FE's levels and of the few-level FEs (v = A'u/beta -
+ *             beta v)
  *   out_ms[3] D'u of few-level FEs when they do not fit the pass (else 0)
- *   out_ms[4] fix-up: v = A'u/beta - beta v for the first FE's levels, and
- *             the segments split across tiles
+ *   out_ms[4] fix-up after the CSR pass: segments longer than a CSR tile,
+ *             levels without observations (about 0 when there are none);
+ *             with FEHIP_SEG_FIX=0, the whole fix-up
  *   out_ms[5] alpha = ||v|| and plane-rotation finalize
  *   out_ms[6] LSMR vector update (src/utils/lsmr.jl:138-142, :192)
  *   out_ms[7] ||x|| and stopping-rule finalize (one launch) */
