@@ -618,9 +618,6 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
         }
     } cleanup{h};
     HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void **)&h->h_flag, sizeof(int32_t) * 4, hipHostMallocDefault));
     for (int e = 0; e < 2; e++) HIPCHK(hipEventCreateWithFlags(&h->poll_ev[e], hipEventDisableTiming));
     CHK(dalloc(h, &h->d_flag, 4));
@@ -2003,6 +2000,14 @@ struct Engine {
     static int ensure_graph(fehip_solver *h, Plan *P) {
         const int M = h->poll_iters;
         if (P->graph && P->graph_iters == M && P->graph_fwl == h->fwl_now) return 0;
+        // the split fix-up's second stream exists only on handles whose graph
+        // uses it: every stream of the process shares a few hardware queues,
+        // and idle extra streams cost concurrency to many handles (panels)
+        if (K >= kSplitFixK && !use_seg_fix(h, P) && !h->fwl_now && !h->stream2) {
+            HIPCHK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming));
+        }
         if (P->graph) {
             (void)hipGraphExecDestroy(P->graph);
             P->graph = nullptr;
