@@ -4,7 +4,10 @@ row shifts and row broadcasts, and the K = 32 form built from a swap of the
 wave's 32-lane halves, emulated lane by lane with the documented move
 semantics and compared with a direct segmented inclusive scan.  (The
 hardware moves themselves are exercised by every D'u parity test on the
-GPU.)"""
+GPU.)  Also emulated: the FE s level from the head words (k_fused level_s),
+and the CSR tile walk (dtx_seg_body with clipped head bits, seg_step, and the
+fix-up's combination of tile partials) for tiles cut at segment heads, at
+arbitrary positions and at fixed sizes."""
 import numpy as np
 
 
