@@ -161,10 +161,12 @@ struct fehip_solver {
     uint32_t *cc_ncomp = nullptr; // 1
 };
 
-// From this batch width on, the plain iteration's FE s fix-up runs on a second
-// branch of the captured graph, beside the CSR pass.  Measured on N = 1e7:
-// +4..7% at K = 32, while at K = 1 the cross-stream join costs about 18 us
-// per iteration, more than the overlap gains.
+// With separate CSR and fix-up launches (FEHIP_SEG_FIX=0), from this batch
+// width on the plain iteration's FE s fix-up runs on a second branch of the
+// captured graph, beside the CSR pass.  Measured on N = 1e7: +4..7% at
+// K = 32, while at K = 1 the cross-stream join costs about 18 us per
+// iteration, more than the overlap gains.  The default merged launch
+// (k_dtx_seg_fix) overlaps the two without a second branch, at every width.
 constexpr int kSplitFixK = 16;
 
 // the projected LSMR applies to the residual solves of a call (G >= 2)
