@@ -493,9 +493,10 @@ def test_panels_of_mixed_precisions(gpu):
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed", "fp32"])
 def test_wide_batch_few_level_fes(gpu, prec):
-    # K = 16 with a 20-level FE and a 45-level interacted FE: the fused pass's
-    # per-column phased accumulators (LDS adds in FP64, compensated
-    # read-modify-write in FP32) and the set-up pass on the same partition
+    # K = 16 with a 20-level FE and a 45-level interacted FE: in FP64 the
+    # fused pass's per-column phased accumulators and the set-up pass on the
+    # same partition; in FP32 (65 lane-accumulator slots exceed 64 KB) the
+    # separate few-level pass
     _say(f"case: K=16 few-level FEs {prec}")
     rng = np.random.default_rng(77)
     refs, n, z, w, y = _rand_problem(78, 30_000, (3000, 20, 45), interaction=(2,))
@@ -567,3 +568,23 @@ def test_separate_csr_and_fixup_launches(gpu, monkeypatch, algorithm, K):
         out[sf] = cols
     for a, b in zip(out["1"], out["0"]):
         assert _rel(a, b) <= 1e-9
+
+
+@pytest.mark.parametrize("prec,K", [("fp32", 1), ("fp32", 16), ("mixed", 16)])
+def test_fp32_lane_accumulators_in_fused_pass(gpu, prec, K):
+    # one 50-level interacted FE beside FE s and a CSR FE: in FP32 its plain
+    # float lane accumulators (50 slots, 51 KB per workgroup) fit the fused
+    # pass; converged residuals within the precision's bar of the exact ones
+    _say(f"case: FP32 lane accumulators {prec} K={K}")
+    rng = np.random.default_rng(90 + K)
+    refs, n, z, w, y = _rand_problem(91, 40_000, (4000, 300, 50), interaction=(2,))
+    Y = np.column_stack([y] + [rng.standard_normal(40_000) for _ in range(K - 1)])
+    o = oracle.OracleSolver(refs, n, z, w, nthreads=8)
+    tight = [o.solve_residuals(Y[:, c].copy(), tol=1e-13, maxiter=100_000)[0] for c in range(K)]
+    tol, bar = (1e-6, 1e-3) if prec == "fp32" else (1e-12, RES_RTOL)
+    with api.FixedEffectSolverROCm(_fes(refs, n, z), w, prec) as s:
+        cols = [Y[:, c].copy() for c in range(K)]
+        _, its, cvs = s.solve_residuals_many(cols, tol=tol, maxiter=100_000)
+    worst = max(_rel(a, b) for a, b in zip(cols, tight))
+    _say("  iters", its[:4], "worst", worst)
+    assert all(cvs) and worst <= bar
