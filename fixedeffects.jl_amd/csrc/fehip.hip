@@ -9,8 +9,9 @@
 //   * scale (serial per level, in the reference's summation order) and cache.
 // A solve (fehip_solve_residuals*) runs the whole LSMR loop on the device:
 // each iteration is D*v, D'u (+ fix-up) and the fused vector update; the
-// scalar recurrences run in the last workgroup of each reduction, so the host
-// only replays a captured hipGraph of several iterations and polls one flag.
+// scalar recurrences run in a single-workgroup finalize launch after each
+// reduction, so the host only replays a captured hipGraph of several
+// iterations and polls one flag.
 
 #include <hip/hip_runtime.h>
 
@@ -1390,6 +1391,7 @@ struct Engine {
     using TT = T;
     using AccT = Acc;
     using TrT = Tr;
+    static constexpr int kK = K;
 
     // target <- b*target + a*A*coef ; mode != NONE leaves ||target||^2 partials
     static void db(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *coef, T *target,
@@ -3275,6 +3277,46 @@ int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms) {
     if (!h || !out_ms) return fail(FEHIP_EARG, "bad arguments");
     if (K < 1 || K > kMaxK || reps < 1) return fail(FEHIP_EARG, "K must be in 1..32 and reps >= 1");
     return dispatch(h->precision, padded_K(K), [&](auto e) { return decltype(e)::time_kernels(h, reps, out_ms); });
+}
+
+int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out) {
+    g_err.clear();
+    if (!h || !out) return fail(FEHIP_EARG, "bad arguments");
+    if (K < 1 || K > kMaxK) return fail(FEHIP_EARG, "K must be in 1..32");
+    if (fp32_storage ? !h->has32 : !h->has64) return fail(FEHIP_EARG, "the handle has no such storage precision");
+    auto fill = [&](Plan *P) {
+        out[0] = P->fused_dense ? 1 : 0;
+        out[1] = P->R;
+        out[2] = P->ntiles_s;
+        out[3] = P->total_tiles;
+        out[4] = P->nitems;
+        out[5] = P->nslot;
+        out[6] = 0;
+        out[7] = P->csr_blocks;
+        return 0;
+    };
+    const int Kp = padded_K(K);
+    if (fp32_storage) {
+        if (h->precision == FEHIP_FP32)
+            return for_K<float, float, float>(Kp, [&](auto e) {
+                using E = decltype(e);
+                Plan *P = nullptr;
+                CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
+                return fill(P);
+            });
+        return for_K<float, F32x2, double>(Kp, [&](auto e) {
+            using E = decltype(e);
+            Plan *P = nullptr;
+            CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
+            return fill(P);
+        });
+    }
+    return for_K<double, double, double>(Kp, [&](auto e) {
+        using E = decltype(e);
+        Plan *P = nullptr;
+        CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
+        return fill(P);
+    });
 }
 
 int fehip_set_algorithm(fehip_t *h, int32_t algorithm) {

@@ -13,7 +13,8 @@ constexpr int kMaxG = 16;                 // fixed effects per handle
 constexpr int kMaxK = 32;                 // columns per batched launch
 constexpr int kFinBlock = 1024;           // threads of a finalize launch
 
-// What a reduction's last workgroup does with the column norms it forms.
+// What a reduction launch and the finalize launch after it do with the
+// column norms the reduction forms.
 enum Mode : int {
     MODE_NONE = 0,  // plain mul!: no norm, no LSMR bookkeeping
     MODE_INIT = 1,  // LSMR set-up (src/utils/lsmr.jl:63-98)

@@ -172,8 +172,9 @@ __device__ __forceinline__ A seg_scan(A v, int psub, int rs) {
 
 // ---------------------------------------------------------------------------
 // LSMR scalar bookkeeping (src/utils/lsmr.jl), run by one thread per column
-// in the last workgroup of each reduction.  Contraction is disabled so the
-// scalar recurrences round like the reference's.
+// of the single-workgroup finalize launch that follows each reduction
+// (k_fin_db, k_fin_dtx, k_fin_update).  Contraction is disabled so the scalar
+// recurrences round like the reference's.
 // ---------------------------------------------------------------------------
 template <typename Tr>
 __device__ __forceinline__ Tr big_minrbar() { return (Tr)1e100; }
@@ -725,16 +726,22 @@ using DenseAcc = typename std::conditional<dense_phased<K, Acc>(), Acc, typename
 
 // One phased add of U steps into per-wave accumulators acc[slot*K + c] (FP64):
 // the lanes of phase ph (psub == ph) hold distinct columns, so no two lanes of
-// an instruction meet; LDS adds in a fixed order of steps and phases.
+// an instruction meet, and the adds go out phase by phase, steps in order
+// within a phase.  The phase loop is kept rolled: unrolled, the compiler
+// merged the P guarded adds into one unguarded instruction at K = 32 (lanes
+// c and 32 + c then meet in one ds_add_f64, whose order among them the
+// hardware picks), so only program order fixes the summation order here.
 template <int U, int K, typename Acc>
 __device__ __forceinline__ void phased_add(Acc *acc, const int32_t (&slot)[U], const Acc (&x)[U], int psub, int c) {
     static_assert(std::is_same<Acc, double>::value, "phased accumulators are FP64 only");
     constexpr int P = kWave / K;
+#pragma unroll 1
+    for (int ph = 0; ph < P; ph++) {
+        if (psub == ph) {
 #pragma unroll
-    for (int q = 0; q < U; q++)
-#pragma unroll
-        for (int ph = 0; ph < P; ph++)
-            if (psub == ph) atomicAdd(acc + slot[q] * K + c, x[q]);
+            for (int q = 0; q < U; q++) atomicAdd(acc + slot[q] * K + c, x[q]);
+        }
+    }
 }
 
 

@@ -161,7 +161,10 @@ class FixedEffectSolverROCm:
         self.n = np.array([fe.n for fe in fes], dtype=np.int64)
         self.p = int(self.n.sum())
         self.precision = precision
-        self.T = np.float64 if precision == "fp64" else np.float32
+        # element type T of the solver: Float32 only for the reference Float32
+        # mode; the mixed mode returns FP64-accurate results (FP64 recurrences
+        # and refinement), so it keeps Float64's defaults
+        self.T = _precision_T(precision)
         self._refs = [np.ascontiguousarray(fe.refs, dtype=np.int64) for fe in fes]
         self._z = [fe.interaction for fe in fes]
         self.weights = None
@@ -213,9 +216,7 @@ class FixedEffectSolverROCm:
         return self
 
     def _default_tol(self, tol):
-        if tol is None:
-            return float(np.sqrt(np.finfo(self.T).eps))  # sqrt(eps(T)) (:49)
-        return float(tol)
+        return _inner_default_tol(self.T, tol)
 
     def solve_residuals(self, y, tol: Optional[float] = None, maxiter: int = 100_000):
         """Inner solve_residuals!(r, feM; tol, maxiter) (:49-68) on one vector,
@@ -344,6 +345,14 @@ class FixedEffectSolverROCm:
         return {"tiles_s": int(out[0]), "tiles_c": int(out[1]), "fuse_dense": int(out[2]),
                 "poll_iters": int(out[3]), "ms_per_iter": float(out[4]), "default_ms_per_iter": float(out[5])}
 
+    def plan_info(self, K: int = 1, fp32_storage: bool = False) -> dict:
+        """Launch plan of K columns (test hook fehip_plan_info)."""
+        out = np.zeros(8, np.int64)
+        check(lib.fehip_plan_info(self._h, int(K), 1 if fp32_storage else 0, _ptr(out, C.c_int64)))
+        keys = ("fused_dense", "tile_positions", "fused_tiles", "dtx_tiles", "fixup_items", "dense_slots",
+                "pieces", "csr_blocks")
+        return {k: int(v) for k, v in zip(keys, out)}
+
     def lsmr_info(self, col: int = 0) -> dict:
         """LSMR scalars of column col of the last solve."""
         out = np.zeros(9, np.float64)
@@ -408,11 +417,29 @@ def AbstractFixedEffectSolver(fes, weights=None, method: str = "rocm", double_pr
     return FixedEffectSolverROCm(fes, weights, precision, algorithm=algorithm)
 
 
-def _outer(method, double_precision, tol):
+def _precision_T(precision: str):
+    """Element type of a solver of this precision: Float32 for "fp32" (the
+    reference's double_precision = false), Float64 for "fp64" and "mixed"."""
+    return np.float32 if precision == "fp32" else np.float64
+
+
+def _inner_default_tol(T, tol):
+    """tol default of the inner solve_residuals!/solve_coefficients!:
+    sqrt(eps(T)) (src/AbstractFixedEffectSolver.jl:49, :147)."""
+    if tol is None:
+        return float(np.sqrt(np.finfo(T).eps))
+    return float(tol)
+
+
+def _outer(method, double_precision, tol, precision=None):
+    """Defaults of the outer keyword methods (src/AbstractFixedEffectSolver.jl
+    :37-40): double_precision = (method == :cpu), tol = 1e-8 in double
+    precision, else 1e-6.  precision = "mixed" (no reference counterpart)
+    computes FP64-accurate results and takes the double-precision tol."""
     if double_precision is None:
         double_precision = method == "cpu"  # :39 -- GPU methods default to Float32
     if tol is None:
-        tol = 1e-8 if double_precision else 1e-6  # :40
+        tol = 1e-8 if (double_precision or precision in ("fp64", "mixed")) else 1e-6  # :40
     return double_precision, tol
 
 
@@ -453,7 +480,7 @@ def solve_residuals(y, fes, w=None, *, method: str = "rocm", double_precision: O
     N = len(y) if _is_single(y) else (y.shape[0] if isinstance(y, np.ndarray) else len(y[0]))
     if any(len(fe) != N for fe in fes):
         raise ValueError("FixedEffects must have the same length as y")  # :42
-    dp, tol = _outer(method, double_precision, tol)
+    dp, tol = _outer(method, double_precision, tol, precision)
     with AbstractFixedEffectSolver(fes, w, method, dp, precision, algorithm) as feM:
         if _is_single(y):
             return feM.solve_residuals(y, tol=tol, maxiter=maxiter)
@@ -470,7 +497,7 @@ def solve_coefficients(y, fes, w=None, *, method: str = "rocm", double_precision
     fes = list(fes)
     if any(len(fe) != len(y) for fe in fes):
         raise ValueError("FixedEffects must have the same length as y")  # :142
-    dp, tol = _outer(method, double_precision, tol)
+    dp, tol = _outer(method, double_precision, tol, precision)
     with AbstractFixedEffectSolver(fes, w, method, dp, precision, algorithm) as feM:
         return feM.solve_coefficients(y, tol=tol, maxiter=maxiter)
 

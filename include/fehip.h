@@ -256,6 +256,17 @@ int fehip_mul_At(fehip_t *h, double *x, const double *y, double a, double b);
  * out[5] condA, out[6] normx, out[7] alpha, out[8] beta. */
 int fehip_lsmr_info(fehip_t *h, int32_t col, double *out);
 
+/* Test hook (no reference counterpart): the launch plan of K columns in one
+ * storage precision (fp32_storage 0: the FP64 streams of FEHIP_FP64 and of
+ * FEHIP_MIXED's FP64 phase; 1: the FP32 streams of FEHIP_FP32 and of
+ * FEHIP_MIXED's FP32 phase), built if not yet there.
+ *   out[0] 1 if the few-level FEs are summed inside the fused pass
+ *   out[1] positions per fused-pass tile      out[2] fused-pass tiles
+ *   out[3] tiles of the D'u passes (all FEs)  out[4] fix-up items
+ *   out[5] few-level accumulator slots        out[6] reserved (0)
+ *   out[7] workgroups of the CSR D'u launch */
+int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out);
+
 /* Measurement hook (no reference counterpart): average device time in ms of
  * each launch of one LSMR iteration on K columns.  `reps` iterations of a
  * real LSMR start on filler data (b = 1, x = 0, zero tolerances) run eagerly

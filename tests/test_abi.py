@@ -95,3 +95,25 @@ def test_host_normalize_matches_oracle():
     b = oracle.normalize([c.copy() for c in coefs], refs, n, [True, False, True])
     for x, y in zip(a, b):
         np.testing.assert_allclose(x, y, rtol=1e-12, atol=1e-12)
+
+
+def test_default_tolerances_per_precision():
+    """Inner tol default sqrt(eps(T)) (src/AbstractFixedEffectSolver.jl:49,147)
+    with T = Float32 only for the reference Float32 mode; the mixed mode is
+    FP64-accurate and keeps Float64's defaults.  Outer defaults (:39-40):
+    1e-8 in double precision, 1e-6 otherwise; mixed takes 1e-8."""
+    assert api._precision_T("fp64") is np.float64
+    assert api._precision_T("mixed") is np.float64
+    assert api._precision_T("fp32") is np.float32
+    assert api._inner_default_tol(api._precision_T("fp64"), None) == pytest.approx(np.sqrt(2.0 ** -52))
+    assert api._inner_default_tol(api._precision_T("mixed"), None) == pytest.approx(np.sqrt(2.0 ** -52))
+    assert api._inner_default_tol(api._precision_T("fp32"), None) == pytest.approx(np.sqrt(2.0 ** -23))
+    assert api._inner_default_tol(np.float32, 1e-3) == 1e-3
+    # outer: method :rocm defaults to Float32 semantics unless told otherwise
+    assert api._outer("rocm", None, None) == (False, 1e-6)
+    assert api._outer("rocm", True, None) == (True, 1e-8)
+    assert api._outer("rocm", None, None, "mixed") == (False, 1e-8)
+    assert api._outer("rocm", None, None, "fp64") == (False, 1e-8)
+    assert api._outer("rocm", None, None, "fp32") == (False, 1e-6)
+    assert api._outer("rocm", None, 1e-4, "mixed") == (False, 1e-4)
+    assert api._outer("cpu", None, None) == (True, 1e-8)

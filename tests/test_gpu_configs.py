@@ -1,0 +1,216 @@
+"""GPU parity of the measured workloads that the single-case tests do not
+cover, at sizes the CPU oracle finishes in seconds, through the C-ABI:
+
+* the N = 1e8 worker/firm/year workload (Zipf workers, mobility 0.02 firms,
+  50 years interacted with z ~ U(0.5, 1.5), K = 16) at 1/500 scale, in the
+  mixed mode and in FP64, every column against the FP64 oracle;
+* many uneven panels (N log-uniform, 2-3 FEs, 1-8 columns) through the
+  many-panel entry point and through the learned start with the committed
+  network weights, every column against the oracle;
+* bitwise repeatability of wide FP64 batches with a few-level FE (the phased
+  LDS accumulators);
+* the reference Float32 mode with a 33-64-level FE inside the fused pass,
+  against the oracle's Float32 restatement.
+
+The solve under test is the inner solve_residuals! (src/AbstractFixedEffectSolver.jl
+:49-68), its multi-column loop (:74-97), and lsmr!'s start hook
+(src/utils/lsmr.jl:43).  Bars (per column):
+* tol 1e-12: ||r_gpu - r_oracle|| / ||r_oracle|| <= 1e-8;
+* tol 1e-8: the GPU residual is at least as close to the converged residual
+  as the oracle's own tol-1e-8 residual, within a factor 3, and the relative
+  difference to the oracle's tol-1e-8 residual is printed.
+"""
+import functools
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from fixedeffects_rocm import api, device, synth, warmstart
+
+pytestmark = pytest.mark.gpu
+
+RES_RTOL = 1e-8
+NTHREADS = 8
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _fes(p):
+    return [api.FixedEffect.from_refs(r, k, z) for r, k, z in zip(p.refs, p.n, p.interactions)]
+
+
+@functools.lru_cache(maxsize=None)
+def _interacted_k16():
+    """The N = 1e8 interacted workload at 1/500 scale (N = 2e5, 1e4 workers,
+    1e3 firms, 50 interacted years, K = 16) and the FP64 oracle's residuals
+    of every column at tol 1e-12 (converged) and at tol 1e-8."""
+    p = synth.config("worker_firm_1e8_k16", scale=0.002)
+    o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=NTHREADS)
+    conv = [o.solve_residuals(p.Y[:, c].copy(), tol=1e-12, maxiter=100_000) for c in range(p.K)]
+    loose = [o.solve_residuals(p.Y[:, c].copy(), tol=1e-8, maxiter=10_000) for c in range(p.K)]
+    return p, conv, loose
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_interacted_k16_workload(gpu, prec):
+    p, conv, loose = _interacted_k16()
+    _say(f"case: N=1e8 workload at 1/500 scale, {prec}, N={p.N} n={p.n} K={p.K}")
+    assert p.K == 16 and p.interactions[2] is not None and p.n[2] == 50
+    with api.FixedEffectSolverROCm(_fes(p), p.weights, prec) as s:
+        cols = [np.ascontiguousarray(p.Y[:, c]).copy() for c in range(p.K)]
+        _, its, cvs = s.solve_residuals_many(cols, tol=1e-12, maxiter=100_000)
+        assert all(cvs)
+        errs = [_rel(cols[c], conv[c][0]) for c in range(p.K)]
+        _say(f"  tol 1e-12 per-column rel err vs the FP64 oracle: {' '.join(f'{e:.1e}' for e in errs)}")
+        _say(f"  iterations gpu {its} oracle {[c[1] for c in conv]}")
+        assert max(errs) <= RES_RTOL
+        cols = [np.ascontiguousarray(p.Y[:, c]).copy() for c in range(p.K)]
+        _, its8, cvs8 = s.solve_residuals_many(cols, tol=1e-8, maxiter=10_000)
+        assert all(cvs8)
+    d_or = [_rel(cols[c], loose[c][0]) for c in range(p.K)]
+    d_ex = [_rel(cols[c], conv[c][0]) for c in range(p.K)]
+    d_oe = [_rel(loose[c][0], conv[c][0]) for c in range(p.K)]
+    _say(f"  tol 1e-8 per-column rel err vs the FP64 oracle (tol 1e-8): {' '.join(f'{e:.1e}' for e in d_or)}")
+    _say(f"  tol 1e-8 distance to converged: gpu {max(d_ex):.2e} oracle {max(d_oe):.2e}; "
+         f"iterations gpu {its8} oracle {[c[1] for c in loose]}")
+    for c in range(p.K):
+        assert d_ex[c] <= 3 * d_oe[c] + 1e-12, (c, d_ex[c], d_oe[c])
+
+
+# -- many uneven panels ---------------------------------------------------------------
+
+PANEL_SEEDS = list(range(910_000, 910_008))
+
+
+@functools.lru_cache(maxsize=None)
+def _panels():
+    """Eight uneven panels (N log-uniform in [1e4, 1.2e5], 2-3 FEs, 1-8
+    columns) and the oracle's residuals of every column at tol 1e-12 and 1e-8."""
+    panels = [synth.uneven_panel(s, 1e4, 1.2e5) for s in PANEL_SEEDS]
+    ref = []
+    for p in panels:
+        o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=NTHREADS)
+        conv = [o.solve_residuals(p.Y[:, c].copy(), tol=1e-12, maxiter=100_000)[0] for c in range(p.K)]
+        loose = [o.solve_residuals(p.Y[:, c].copy(), tol=1e-8, maxiter=10_000) for c in range(p.K)]
+        ref.append((conv, loose))
+    return panels, ref
+
+
+def _check_panel_columns(label, cols, conv, loose):
+    worst_or, worst_ex = 0.0, 0.0
+    for c, r in enumerate(cols):
+        d_ex, d_oe = _rel(r, conv[c]), _rel(loose[c][0], conv[c])
+        worst_or = max(worst_or, _rel(r, loose[c][0]))
+        worst_ex = max(worst_ex, d_ex)
+        assert d_ex <= 3 * d_oe + 1e-12, (label, c, d_ex, d_oe)
+    return worst_or, worst_ex
+
+
+def test_uneven_panels_against_oracle(gpu):
+    panels, ref = _panels()
+    _say(f"case: {len(panels)} uneven panels, N={[p.N for p in panels]}, K={[p.K for p in panels]}, "
+         f"G={[len(p.n) for p in panels]}")
+    solvers = [api.FixedEffectSolverROCm(_fes(p), p.weights, "fp64") for p in panels]
+    net = warmstart.WarmStartNet.load()
+    try:
+        bufs = []
+        for p in panels:
+            b = device.DeviceBuffer(p.N * p.K)
+            b.upload(np.asfortranarray(p.Y).ravel(order="F"))
+            bufs.append(b)
+        its, cvs = api.solve_residuals_panels(solvers, [b.address for b in bufs], [p.K for p in panels],
+                                              [p.N for p in panels], tol=1e-8, maxiter=10_000)
+        tot_ref, tot_lrn = 0, 0
+        for i, (p, b, s, (conv, loose)) in enumerate(zip(panels, bufs, solvers, ref)):
+            assert all(cvs[i])
+            flat = b.download(p.N * p.K)
+            cols = [flat[c * p.N:(c + 1) * p.N] for c in range(p.K)]
+            w_or, w_ex = _check_panel_columns(f"panel {i}", cols, conv, loose)
+            lrn = [np.ascontiguousarray(p.Y[:, c]).copy() for c in range(p.K)]
+            it_l, cv_l = warmstart.solve_residuals_learned(s, lrn, net, tol=1e-8, maxiter=10_000)
+            assert all(cv_l)
+            l_or, l_ex = _check_panel_columns(f"panel {i} learned", lrn, conv, loose)
+            tot_ref += sum(its[i]) - p.K
+            tot_lrn += sum(it_l) - p.K
+            _say(f"  panel {i}: N={p.N} K={p.K} LSMR iters reference start {[x - 1 for x in its[i]]} "
+                 f"learned {[x - 1 for x in it_l]}; rel err vs oracle {w_or:.1e} (learned {l_or:.1e}), "
+                 f"vs converged {w_ex:.1e} (learned {l_ex:.1e})")
+        _say(f"  LSMR iterations in total: reference start {tot_ref}, learned start {tot_lrn} "
+             f"({100.0 * (tot_ref - tot_lrn) / tot_ref:+.1f}% saved)")
+    finally:
+        for s in solvers:
+            s.close()
+        net.close()
+
+
+# -- determinism of the phased few-level accumulators ---------------------------------------
+
+@pytest.mark.parametrize("K", [16, 32])
+def test_wide_batches_bitwise_repeatable(gpu, K):
+    # FP64 at K >= 16 sums few-level FEs with per-column phased LDS adds in the
+    # fused pass and the set-up pass; two solves must agree bit for bit
+    _say(f"case: bitwise repeatability K={K}")
+    rng = np.random.default_rng(60 + K)
+    N = 60_000
+    refs = [rng.integers(1, 6001, N), rng.integers(1, 401, N), rng.integers(1, 49, N)]
+    n = [6000, 400, 48]
+    for r, k in zip(refs, n):
+        r[:k] = np.arange(1, k + 1)
+    w = rng.lognormal(0, 0.5, N)
+    Y = rng.standard_normal((N, K))
+    fes = [api.FixedEffect.from_refs(r.astype(np.int64), k) for r, k in zip(refs, n)]
+    with api.FixedEffectSolverROCm(fes, w, "fp64") as s:
+        info = s.plan_info(K)
+        assert info["fused_dense"] == 1 and info["dense_slots"] == 48, info
+        out = []
+        for _ in range(2):
+            cols = [np.ascontiguousarray(Y[:, c]).copy() for c in range(K)]
+            _, its, _ = s.solve_residuals_many(cols, tol=1e-10, maxiter=100_000)
+            out.append((cols, its))
+    assert out[0][1] == out[1][1]
+    for a, b in zip(out[0][0], out[1][0]):
+        np.testing.assert_array_equal(a, b)
+
+
+# -- Float32 reference mode with a wide few-level FE in the fused pass ----------------------
+
+@pytest.mark.parametrize("K,nlev", [(1, 40), (16, 60)])
+def test_fp32_few_level_fe_against_float32_oracle(gpu, K, nlev):
+    # the FP32 lane accumulators of a 33-64-level FE ride in the fused pass
+    # (plan hook); every column matches the oracle's Float32 restatement at
+    # the same tol (the bar of test/solve.jl:37 is sqrt(eps(Float32)))
+    _say(f"case: fp32 {nlev}-level FE K={K} vs the Float32 oracle")
+    rng = np.random.default_rng(170 + K)
+    N = 40_000
+    refs = [rng.integers(1, 4001, N), rng.integers(1, 301, N), rng.integers(1, nlev + 1, N)]
+    n = [4000, 300, nlev]
+    for r, k in zip(refs, n):
+        r[:k] = np.arange(1, k + 1)
+    refs = [r.astype(np.int64) for r in refs]
+    z = [None, None, rng.uniform(0.5, 1.5, N)]
+    w = rng.lognormal(0, 0.5, N)
+    Y = rng.standard_normal((N, K))
+    o = oracle.OracleSolver(refs, n, z, w, double_precision=False, nthreads=NTHREADS)
+    want = [o.solve_residuals(Y[:, c].copy(), tol=1e-6, maxiter=10_000) for c in range(K)]
+    fes = [api.FixedEffect.from_refs(r, k, zz) for r, k, zz in zip(refs, n, z)]
+    with api.FixedEffectSolverROCm(fes, w, "fp32") as s:
+        info = s.plan_info(K, fp32_storage=True)
+        assert info["fused_dense"] == 1 and info["dense_slots"] == nlev, info
+        cols = [np.ascontiguousarray(Y[:, c]).copy() for c in range(K)]
+        _, its, cvs = s.solve_residuals_many(cols, tol=1e-6, maxiter=10_000)
+    errs = [_rel(cols[c], want[c][0]) for c in range(K)]
+    _say(f"  iters gpu {its[:8]} oracle {[x[1] for x in want][:8]}; worst rel err {max(errs):.2e}")
+    for c in range(K):
+        ro, io, co = want[c]
+        assert cvs[c] == co
+        assert errs[c] <= 1e-4, (c, errs[c])
+        assert abs(its[c] - io) <= max(3, io // 10), (c, its[c], io)

@@ -181,8 +181,11 @@ def _check_solve(refs, n, z, w, Y, prec="fp64", tol=1e-8, iter_slack=2):
             _, its, cvs = s.solve_residuals_many(cols, tol=tol, maxiter=10000)
             got = list(zip(cols, its, cvs))
     worst = 0.0
-    for (rg, ig, cg), (ro, io, co), rt in zip(got, want, tight):
+    for c, ((rg, ig, cg), (ro, io, co), rt) in enumerate(zip(got, want, tight)):
         worst = max(worst, _rel(rg, ro))
+        if c < 4:
+            _say(f"  col {c} tol {tol:g}: iters gpu {ig} oracle {io}; rel err vs oracle {_rel(rg, ro):.2e}, "
+                 f"to the tight solve gpu {_rel(rg, rt):.2e} oracle {_rel(ro, rt):.2e}")
         assert cg == co
         assert abs(ig - io) <= iter_slack, (ig, io)
         assert _rel(rg, rt) <= 3 * _rel(ro, rt) + 1e-12, (_rel(rg, rt), _rel(ro, rt))
@@ -352,10 +355,14 @@ def test_deterministic(gpu):
     np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("k", [1, 5, 12])
+@pytest.mark.parametrize("k", [1, 5, 12, 30, 60])
 def test_fixed_iteration_parity(gpu, k):
     # tol = 0: neither side stops early, so the k-th LSMR iterates and scalars
-    # are compared directly (x0 = A'b as solve_residuals! uses)
+    # are compared directly (x0 = A'b as solve_residuals! uses).  Early on the
+    # bar is 1e-12.  Mid-run, LSMR's iterates depend sharply on the rounding
+    # of its sums (src/utils/lsmr.jl:113-192): the bar is then 10x how far the
+    # oracle moves when only its row order changes (the same run on permuted
+    # rows), per quantity -- the spread any reordering of the sums produces.
     _say(f"case: fixed iterations k={k}")
     p = synth.config("worker_firm_1e7", scale=0.02)
     y = p.Y[:, 0].copy()
@@ -363,16 +370,27 @@ def test_fixed_iteration_parity(gpu, k):
     ro, io, _ = o.solve_residuals(y.copy(), tol=0.0, maxiter=k + 1)
     b = y * np.sqrt(p.weights)
     _, info = o.lsmr(o.mul_At(b), b, atol=0.0, btol=0.0, conlim=1e8, maxiter=k)
+    perm = np.random.default_rng(5).permutation(p.N)
+    op = oracle.OracleSolver([r[perm] for r in p.refs], p.n, [None] * 3, p.weights[perm], nthreads=8)
+    rp = np.empty(p.N)
+    rp[perm] = op.solve_residuals(y[perm].copy(), tol=0.0, maxiter=k + 1)[0]
+    bp = b[perm]
+    _, info_p = op.lsmr(op.mul_At(bp), bp, atol=0.0, btol=0.0, conlim=1e8, maxiter=k)
     with api.FixedEffectSolverROCm(_fes(p.refs, p.n), p.weights, "fp64") as s:
         rg, ig, _ = s.solve_residuals(y.copy(), tol=0.0, maxiter=k + 1)
         gi = s.lsmr_info(0)
-    _say(f"  iters {ig} {io} rel err {_rel(rg, ro):.3e}")
+    spread = _rel(rp, ro)
+    bar = 1e-12 if k <= 12 else max(1e-12, 10 * spread)
+    _say(f"  iters {ig} {io} rel err {_rel(rg, ro):.3e} (permuted-row oracle {spread:.3e}, bar {bar:.1e})")
     for key in ("normr", "normAr", "normA", "condA", "normx"):
-        _say(f"  {key}: gpu {gi[key]!r} oracle {getattr(info, key)!r}")
+        _say(f"  {key}: gpu {gi[key]!r} oracle {getattr(info, key)!r} permuted oracle {getattr(info_p, key)!r}")
     assert ig == io == k + 1
-    assert _rel(rg, ro) <= 1e-12
+    assert _rel(rg, ro) <= bar
     for key in ("normr", "normAr", "normA", "normx"):
-        assert abs(gi[key] - getattr(info, key)) <= 1e-12 * abs(getattr(info, key)), key
+        want = getattr(info, key)
+        sp = abs(getattr(info_p, key) - want)
+        kbar = 1e-12 * abs(want) if k <= 12 else max(1e-12 * abs(want), 10 * sp)
+        assert abs(gi[key] - want) <= kbar, (key, gi[key], want, sp)
 
 
 def test_matvec_skewed(gpu):
