@@ -23,6 +23,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fixedeffects.jl_amd"))
 sys.path.insert(0, ROOT)
@@ -41,49 +43,69 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (scaled by a plain device copy of known size), by (workload, precision):
-# profiles/worker_firm_1e7_fp64_k1.md
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE (scaled by a plain device copy of known size), by
+# (workload, precision, K, form).  Read from the profile named in
+# "traffic_source"; a form without a measured profile reports traffic null.
 PMC_TRAFFIC = {
-    ("worker_firm_1e7", "fp64"): {"k_fused": 324.1e6, "k_dtx_csr": 311.1e6, "k_update": 88.1e6},
+    ("worker_firm_1e7", "fp64", 1, "csr"): ({"k_fused": 324.1e6, "k_dtx_csr": 311.1e6, "k_update": 88.1e6},
+                                            "profiles/worker_firm_1e7_fp64_k1.md"),
 }
 
 
-def kernel_bytes(N, G, n, kinds, K, s, factored, weighted):
+def kernel_bytes(N, n, kinds, K, s, factored, weighted, plan):
     """Algorithmic HBM bytes per launch of each kernel of one LSMR iteration
     (each array element counted once; gathers counted as their distinct
-    elements).  factored: no FE is interacted, so the fused pass reads one
-    sqrt(w) stream (none without weights) instead of every FE's cache."""
+    elements), for the plan the library built (fehip_plan_info: few-level FEs
+    in the fused pass or not, the piece form or the CSR pass).  factored: no
+    FE is interacted, so the fused pass reads one sqrt(w) stream (none
+    without weights) instead of every FE's cache."""
+    G = len(n)
     p = sum(n)
     gs = kinds.index("sorted")
-    ndense_lev = sum(k for k, kd in zip(n, kinds) if kd == "dense")
-    fused_dense = 0 < ndense_lev <= 32
+    fused_dense = bool(plan["fused_dense"])
+    npc = plan["pieces"]
+    ndense = kinds.count("dense")
     out = {}
-    # fused pass: refs of every FE but s (whose level comes from its head bits
-    # and their word prefixes; with G = 3 and one few-level FE the other two
-    # share one packed stream), caches (or sqrt(w)), u read + write, each
-    # coefficient once, raw FE s sums (double) out
-    nref = G - 2 if (G == 3 and kinds.count("dense") == 1 and fused_dense) else G - 1
-    per_obs = N * G * s + N * nref * 4 if not factored else N * nref * 4 + (N * s if weighted else 0)
-    out["k_fused"] = per_obs + K * N * 2 * s + K * p * s + N // 8 + N // 16 + 8 * K * n[gs]
+    # fused pass: refs of every FE but s (whose level comes from head bits and
+    # their word prefixes; with G = 3 and the few-level FE in the pass, the
+    # other two share one packed stream), caches (or sqrt(w)), u read + write,
+    # each gathered coefficient once, the FE s (or piece) level map, the raw
+    # sums out (FE s levels in doubles, or the pieces in T)
+    nref = G - 2 if (G == 3 and ndense == 1 and fused_dense) else G - 1
+    per_obs = N * nref * 4 + ((N * s if weighted else 0) if factored else N * G * s)
+    if npc:
+        out["k_fused"] = per_obs + K * N * 2 * s + K * p * s + N // 8 + N // 16 + 4 * npc + s * K * npc
+    else:
+        out["k_fused"] = per_obs + K * N * 2 * s + K * p * s + N // 8 + N // 16 + 4 * n[gs] + 8 * K * n[gs]
     csr = 0
     dense = 0
+    scaled_u = factored and K < 8  # the CSR pass then reads the stored sqrt(w) .* u, no cache
     for g in range(G):
-        if kinds[g] == "csr":
-            csr += 4 * N + N * s + K * N * s + 2 * K * n[g] * s + N // 8 + 4 * n[g]
+        if kinds[g] == "csr" and not npc:
+            csr += 4 * N + (0 if scaled_u else N * s) + K * N * s + 2 * K * n[g] * s + N // 8 + 4 * n[g]
         elif kinds[g] == "dense" and not fused_dense:
             dense += N * (4 + s) + K * N * s
-    out["k_dtx_csr"] = csr
-    out["k_dtx_dense"] = dense
-    # FE s levels: raw sum (double) + v read + write, one flag byte (+ scale).
-    # With CSR FEs these items run as the first blocks of the CSR launch
-    # (k_dtx_seg_fix), whose slot then carries their bytes
-    fix = K * n[gs] * (8 + 2 * s) + n[gs] + (n[gs] * s if factored else 0)
-    if csr:
-        out["k_dtx_csr"] += fix
+    if npc:
+        # the piece launch: for FE s (identity order) and FE t (through the
+        # pieces' permutation by FE t level): order, the piece sums, head bits,
+        # level map and scale, and each level's v read and written
+        gt = [g for g in range(G) if kinds[g] == "csr"][0]
+        for g in (gs, gt):
+            csr += 4 * npc + K * npc * s + npc // 8 + 4 * n[g] + n[g] * s + 2 * K * n[g] * s
+        out["k_dtx_csr"] = csr
         out["k_dtx_fixup"] = 0
     else:
-        out["k_dtx_fixup"] = fix
+        # FE s levels: raw sum (double) + v read + write, one flag byte (+ scale).
+        # With CSR FEs these items run as the first blocks of the CSR launch
+        fix = K * n[gs] * (8 + 2 * s) + n[gs] + (n[gs] * s if factored else 0)
+        if csr:
+            out["k_dtx_csr"] = csr + fix
+            out["k_dtx_fixup"] = 0
+        else:
+            out["k_dtx_csr"] = 0
+            out["k_dtx_fixup"] = fix
+    out["k_dtx_dense"] = dense
     # v, h, hbar, x read + write (+ scale read, scale .* v written)
     out["k_update"] = 8 * K * p * s + ((K + 1) * p * s if factored else 0)
     # the pair formula of the survey: D*v + D'u over all G FEs with perm+cache each
@@ -91,39 +113,84 @@ def kernel_bytes(N, G, n, kinds, K, s, factored, weighted):
     return out
 
 
-def cpu_baseline(panel, iters_gpu, budget_s, nthreads):
-    """The oracle (C restatement of the reference's CPU path) timed on a
-    bounded sample of the same workload."""
-    import numpy as np
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(panel, column, tol, maxiter, nthreads):
+    """The oracle (C restatement of the reference's CPU path) solving one
+    column of the workload to convergence: set-up, then solve_residuals! at
+    the bench's tol.  Returns the baseline entry and the oracle's residual."""
     import oracle
 
-    y = np.ascontiguousarray(panel.Y[:, 0])
+    y = np.ascontiguousarray(panel.Y[:, column])
     t0 = time.perf_counter()
     s = oracle.OracleSolver(panel.refs, panel.n, panel.interactions, panel.weights, True, nthreads)
     setup = time.perf_counter() - t0
-    b = y * (np.sqrt(panel.weights) if panel.weights is not None else 1.0)
-    x0 = s.mul_At(b)
     t0 = time.perf_counter()
-    s.lsmr(x0, b, atol=0.0, btol=0.0, conlim=0.0, maxiter=2)
-    per_iter = (time.perf_counter() - t0) / 2
-    k = int(max(2, min(200, (budget_s - setup) / max(per_iter, 1e-6))))
-    t0 = time.perf_counter()
-    s.lsmr(x0, b, atol=0.0, btol=0.0, conlim=0.0, maxiter=k)
-    per_iter = (time.perf_counter() - t0) / k
-    col_s = (iters_gpu + 1) * per_iter
-    return {
-        "value": 1.0 / col_s,
+    r, iters, conv = s.solve_residuals(y.copy(), tol=tol, maxiter=maxiter)
+    solve = time.perf_counter() - t0
+    entry = {
+        "value": 1.0 / solve,
         "unit": "columns/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": (f"oracle (C restatement of src/SolverCPU.jl + src/utils/lsmr.jl; D'x serial as the reference, "
-                   f"D*x and cache threaded in 100k-row chunks) on the same panel: set-up {setup:.2f} s, "
-                   f"{k} LSMR iterations timed at {per_iter * 1e3:.1f} ms each; columns/s = 1 / "
-                   f"((iters + 1) x per-iteration time) with iters = {iters_gpu} from the GPU run"),
+        "cpu_model": cpu_model(),
+        "sample": (f"oracle (C restatement of src/SolverCPU.jl + src/utils/lsmr.jl + "
+                   f"src/AbstractFixedEffectSolver.jl; D'x serial as the reference, D*x and cache threaded in "
+                   f"100k-row chunks) solving column {column} of the same panel to tol {tol:g}: set-up "
+                   f"{setup:.2f} s (not in value, as for the GPU), solve {solve:.2f} s, {iters - 1} LSMR iterations"),
         "setup_s": setup,
-        "ms_per_lsmr_iter": per_iter * 1e3,
+        "solve_s": solve,
+        "lsmr_iters": int(iters) - 1,
+        "converged": bool(conv),
     }
+    return entry, r
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(float(np.linalg.norm(b)), 1e-300))
+
+
+KERNEL_NAMES = ["k_fused", "k_fin_db", "k_dtx_csr", "k_dtx_dense", "k_dtx_fixup", "k_fin_dtx", "k_update",
+                "k_fin_update"]
+
+
+def kernel_report(feM, panel, K, precision, name, form_key=None):
+    """Per-kernel HIP-event times of one iteration (fehip_time_kernels on the
+    handle's stream), the dominant kernel's roofline entry, and the survey's
+    pair-formula bytes over the iteration's D*x + D'u launches."""
+    import ctypes as C
+
+    from fixedeffects_rocm._lib import check, lib
+
+    ms = np.zeros(8)
+    check(lib.fehip_time_kernels(feM._h, K, 20, ms.ctypes.data_as(C.POINTER(C.c_double))))
+    s_max = int(np.argmax(panel.n))
+    kinds = ["sorted" if g == s_max else ("dense" if k <= 64 else "csr") for g, k in enumerate(panel.n)]
+    factored = all(z is None for z in panel.interactions)
+    # the FP32 streams of the mixed mode's first phase are what the timing hook runs
+    fp32 = precision != "fp64"
+    plan = feM.plan_info(K, fp32_storage=fp32)
+    s = 4 if fp32 else 8
+    kb = kernel_bytes(panel.N, list(panel.n), kinds, K, s, factored, panel.weights is not None, plan)
+    kt = {nm: float(m) for nm, m in zip(KERNEL_NAMES, ms)}
+    dom = max(["k_fused", "k_dtx_csr", "k_dtx_dense", "k_dtx_fixup", "k_update"], key=lambda q: kt[q])
+    achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
+    pair_ms = sum(kt[q] for q in KERNEL_NAMES[:6])
+    form = "pieces" if plan["pieces"] else "csr"
+    traffic, src = PMC_TRAFFIC.get((name, precision, K, form), ({}, None))
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get(dom), "traffic_source": src if traffic else None,
+            "algorithmic_bytes": int(kb[dom])}
+    return {"kernel_ms": kt, "roofline": roof, "hbm_gbs_pair": kb["pair"] / (pair_ms * 1e-3) / 1e9,
+            "plan": plan, "form": form}
 
 
 def run_panels(args, precision):
@@ -201,14 +268,76 @@ def run_panels(args, precision):
         s.close()
 
 
+def measure(feM, pristine, work, K, N, steps, warmup, tol, maxiter, label):
+    """Warm-up, then `steps` timed solves of K device-resident columns (each
+    step restores the columns first, inside the timed region, as a copy in
+    HBM)."""
+    from fixedeffects_rocm import device
+
+    def step():
+        work.copy_from(pristine)
+        device.synchronize()
+        return feM.solve_residuals_device(work.address, K, N, tol=tol, maxiter=maxiter)
+
+    for w in range(warmup):
+        its, cvs = step()
+        log(f"[bench] {label} warmup {w}: iters {its[:4]}{'...' if K > 4 else ''} converged {all(cvs)}")
+    device.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        its, cvs = step()
+    device.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    log(f"[bench] {label}: {steps} steps, {ms:.2f} ms/step, iters {its[:4]}")
+    return ms, [int(i) - 1 for i in its], [bool(c) for c in cvs]
+
+
+def batched_leg(args, precision, plain_pair_gbs, plain_value):
+    """The 32-column workload (the 1e7 panel with 31 regressors) in one
+    batched solve, in FP64 or the mixed mode, beside the K = 1 value."""
+    from fixedeffects_rocm import api, device, synth
+
+    name = "worker_firm_1e7_k32"
+    panel = _panel_cache.get(name)
+    if panel is None:
+        t0 = time.perf_counter()
+        panel = synth.config(name)
+        _panel_cache[name] = panel
+        log(f"[bench] panel {name}: N={panel.N} K={panel.K} ({time.perf_counter() - t0:.1f} s)")
+    K, N = panel.K, panel.N
+    fes = [api.FixedEffect.from_refs(r, k, z) for r, k, z in zip(panel.refs, panel.n, panel.interactions)]
+    feM = api.FixedEffectSolverROCm(fes, panel.weights, precision)
+    sched = None
+    if not args.no_autotune:
+        sched = feM.autotune(K=K, reps=3)
+    pristine = device.DeviceBuffer(N * K)
+    pristine.upload(np.asfortranarray(panel.Y).ravel(order="F"))
+    work = device.DeviceBuffer(N * K)
+    ms, its, cvs = measure(feM, pristine, work, K, N, args.leg_steps, 1, args.tol, args.maxiter,
+                           f"K=32 {precision}")
+    r = work.download(N * K)
+    rep = kernel_report(feM, panel, K, precision, name)
+    feM.close()
+    del pristine, work
+    leg = {"workload": name, "precision": precision, "K": K, "value": K / (ms / 1e3), "ms_per_step": ms,
+           "vs_value": K / (ms / 1e3) / plain_value,
+           "lsmr_iters": {"min": min(its), "max": max(its), "mean": float(np.mean(its))},
+           "converged": all(cvs), "schedule": sched, "kernel_ms": rep["kernel_ms"], "roofline": rep["roofline"],
+           "hbm_gbs_pair": rep["hbm_gbs_pair"], "hbm_gbs_vs_plain": rep["hbm_gbs_pair"] / plain_pair_gbs,
+           "form": rep["form"]}
+    return leg, r, panel
+
+
+_panel_cache = {}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="worker_firm_1e7", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle's solves (and the parity fields)")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--maxiter", type=int, default=10000)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
@@ -216,6 +345,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-sized default schedule")
     ap.add_argument("--panels", type=int, default=64, help="panels of the uneven_panels workload")
     ap.add_argument("--alt-steps", type=int, default=3, help="timed steps of the projected LSMR beside `value`")
+    ap.add_argument("--leg-steps", type=int, default=2, help="timed steps of each 32-column leg (0: no legs)")
     ap.add_argument("--algorithm", default="lsmr", choices=["lsmr", "projected"],
                     help="lsmr: the reference's LSMR; projected: LSMR with the FE of most levels eliminated")
     args = ap.parse_args()
@@ -224,12 +354,7 @@ def main():
                                    "(bench.py --gpus 1)"}))
         sys.exit(2)
 
-    import ctypes as C
-
-    import numpy as np
-
     from fixedeffects_rocm import api, device, synth
-    from fixedeffects_rocm._lib import check, lib
 
     name, K = WORKLOADS[args.workload]
     if name == "uneven_panels":
@@ -254,24 +379,10 @@ def main():
     pristine = device.DeviceBuffer(N * K)   # column c at offset c*N
     pristine.upload(np.asfortranarray(panel.Y).ravel(order="F"))
     work = device.DeviceBuffer(N * K)
-
-    def step():
-        work.copy_from(pristine)
-        device.synchronize()
-        return feM.solve_residuals_device(work.address, K, N, tol=args.tol, maxiter=args.maxiter)
-
-    for w in range(args.warmup):
-        its, cvs = step()
-        log(f"[bench] warmup {w}: iters {its[:4]}{'...' if K > 4 else ''} converged {all(cvs)}")
-    device.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        its, cvs = step()
-    device.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
+    ms_per_step, its, cvs = measure(feM, pristine, work, K, N, args.steps, args.warmup, args.tol, args.maxiter,
+                                    "value")
     value = K / (ms_per_step / 1e3)
-    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, iters {its[:4]}")
+    r = work.download(N)
 
     # the same solve through the host-buffer entry point (PCIe transfers of
     # the columns included); reported beside `value`, never as it
@@ -283,10 +394,10 @@ def main():
         feM.solve_residuals_many(cols, tol=args.tol, maxiter=args.maxiter)
         pcie_s += time.perf_counter() - t1
     pcie_ms = pcie_s / max(args.pcie_steps, 1) * 1e3
-    log(f"[bench] host-buffer path: {pcie_ms:.2f} ms/step")
+    if args.pcie_steps:
+        log(f"[bench] host-buffer path: {pcie_ms:.2f} ms/step")
 
     # normal equations of the result: ||A'(sqrt(w) r)||_inf / ||sqrt(w) y||_2 (size-independent check)
-    r = work.download(N)
     sw = np.sqrt(panel.weights) if panel.weights is not None else 1.0
     g = feM.mul_At(sw * r)
     neq = float(np.abs(g).max() / np.linalg.norm(sw * panel.Y[:, 0]))
@@ -294,46 +405,46 @@ def main():
     # the same workload with the projected LSMR (the FE of most levels
     # eliminated): same least-squares residual, fewer iterations; reported
     # beside `value`, which stays the reference algorithm's
-    alt = None
+    alt, rp = None, None
     if args.algorithm == "lsmr" and args.alt_steps > 0:
         feM.set_algorithm("projected")
-        step()
-        device.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.alt_steps):
-            its_p, cvs_p = step()
-        device.synchronize()
-        alt_ms = (time.perf_counter() - t1) / args.alt_steps * 1e3
+        alt_ms, its_p, cvs_p = measure(feM, pristine, work, K, N, args.alt_steps, 1, args.tol, args.maxiter,
+                                       "projected")
         rp = work.download(N)
         feM.set_algorithm("lsmr")
         alt = {"algorithm": "projected", "value": K / (alt_ms / 1e3), "ms_per_step": alt_ms,
-               "lsmr_iters": [int(i) - 1 for i in its_p], "converged": bool(all(cvs_p)),
-               "rel_diff_vs_value_run": float(np.linalg.norm(rp - r) / np.linalg.norm(r))}
-        log(f"[bench] projected: {alt_ms:.2f} ms/step, iters {its_p[:4]}")
+               "lsmr_iters": its_p, "converged": all(cvs_p), "rel_diff_vs_value_run": rel(rp, r)}
 
-    # per-kernel durations (HIP events on the solver's stream)
-    ms = np.zeros(8)
-    check(lib.fehip_time_kernels(feM._h, K, 20, ms.ctypes.data_as(C.POINTER(C.c_double))))
-    names = ["k_fused", "k_fin_db", "k_dtx_csr", "k_dtx_dense", "k_dtx_fixup", "k_fin_dtx", "k_update", "k_fin_update"]
-    kinds = []
-    s_max = int(np.argmax(panel.n))
-    for gi, k in enumerate(panel.n):
-        kinds.append("sorted" if gi == s_max else ("dense" if k <= 64 else "csr"))
-    factored = all(z is None for z in panel.interactions)
-    kb = kernel_bytes(N, len(panel.n), panel.n, kinds, K, 8 if args.precision == "fp64" else 4, factored,
-                      panel.weights is not None)
-    kt = {nm: float(m) for nm, m in zip(names, ms)}
-    dom = max(["k_fused", "k_dtx_csr", "k_dtx_dense", "k_dtx_fixup", "k_update"], key=lambda q: kt[q])
-    achieved = kb[dom] / (kt[dom] * 1e-3) / 1e9
-    pair_ms = sum(kt[q] for q in names[:6])
-    pair_gbs = kb["pair"] / (pair_ms * 1e-3) / 1e9
-    iters = int(np.max(its)) - 1  # LSMR iterations (solve_residuals! reports lsmr + 1)
+    rep = kernel_report(feM, panel, K, args.precision, name)
+    feM.close()
+    del pristine, work
+    iters = int(np.max(its))
 
-    cpu = None
+    # CPU oracle: a converged solve of column 0, and the parity of the GPU's
+    # residuals against it (same tol)
+    cpu, parity = None, None
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if not args.no_cpu:
-        nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = cpu_baseline(panel, iters, args.cpu_seconds, nthreads)
-        log(f"[bench] cpu baseline {cpu['value']:.4g} columns/s ({cpu['ms_per_lsmr_iter']:.1f} ms/iter)")
+        cpu, ro = cpu_baseline(panel, 0, args.tol, args.maxiter, nthreads)
+        log(f"[bench] cpu baseline {cpu['value']:.4g} columns/s ({cpu['solve_s']:.2f} s, {cpu['lsmr_iters']} iters)")
+        parity = {"rel_err_vs_oracle": rel(r, ro), "iters_gpu": int(its[0]), "iters_oracle": cpu["lsmr_iters"],
+                  "note": "column 0 of `value`'s last step against the oracle's solve at the same tol"}
+        if rp is not None:
+            parity["projected_rel_err_vs_oracle"] = rel(rp, ro)
+
+    legs = {}
+    if args.leg_steps > 0 and name == "worker_firm_1e7" and args.precision == "fp64" and args.algorithm == "lsmr":
+        leg64, r64, p32 = batched_leg(args, "fp64", rep["hbm_gbs_pair"], value)
+        legmx, rmx, _ = batched_leg(args, "mixed", rep["hbm_gbs_pair"], value)
+        Nb = p32.N
+        legmx["rel_diff_vs_fp64_leg"] = [rel(rmx[c * Nb:(c + 1) * Nb], r64[c * Nb:(c + 1) * Nb])
+                                         for c in range(p32.K)]
+        if not args.no_cpu:
+            o32, ro32 = cpu_baseline(p32, 0, args.tol, args.maxiter, nthreads)
+            leg64["col0_rel_err_vs_oracle"] = rel(r64[:Nb], ro32)
+            legmx["col0_rel_err_vs_oracle"] = rel(rmx[:Nb], ro32)
+            leg64["col0_iters_oracle"] = legmx["col0_iters_oracle"] = o32["lsmr_iters"]
+        legs = {"batched_k32": leg64, "mixed_k32": legmx}
 
     out = {
         "metric": "residualized columns/s @1e-8, N=1e7 3 FEs; LSMR iters; HBM GB/s vs plain",
@@ -354,25 +465,24 @@ def main():
                             else "FP32-stream LSMR phase, then FP64 LSMR iterations from its solution")
                    + ("" if args.algorithm == "lsmr" else "; projected LSMR (FE of most levels eliminated)"),
                    "algorithm": args.algorithm},
-        "lsmr_iters": [int(i) - 1 for i in its],
+        "lsmr_iters": its,
+        "converged": all(cvs),
         "normal_eq_note": "max_level |A'(sqrt(w) r)| / ||sqrt(w) y|| of column 0",
-        "converged": bool(all(cvs)),
         "normal_eq_rel": neq,
+        "parity": parity,
         "setup_s": setup_s,
         "schedule": sched,
-        "kernel_ms": kt,
-        "hbm_gbs_pair": pair_gbs,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": PMC_TRAFFIC.get((name, args.precision), {}).get(dom),
-                     "algorithmic_bytes": int(kb[dom])},
+        "form": rep["form"],
+        "kernel_ms": rep["kernel_ms"],
+        "hbm_gbs_pair": rep["hbm_gbs_pair"],
+        "roofline": rep["roofline"],
         "cpu_baseline": cpu,
         "projected_lsmr": alt,
         "pcie_inclusive": {"value": K / (pcie_ms / 1e3) if args.pcie_steps else None, "ms_per_step": pcie_ms,
                            "note": "fehip_solve_residuals on host columns: H2D + solve + D2H (not `value`)"},
     }
+    out.update(legs)
     print(json.dumps(out))
-    feM.close()
 
 
 if __name__ == "__main__":

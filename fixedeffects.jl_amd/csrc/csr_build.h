@@ -226,6 +226,51 @@ __global__ void k_seg_level(const int32_t *__restrict__ skeys, int64_t n,
     seg_level[ord] = skeys[j];
 }
 
+// ---- pieces: runs of one (FE s level, FE t level) pair inside one tile ----
+//
+// The internal order sorts observations by FE s, then FE t, so each (s, t)
+// cell is one run of positions.  A piece is such a run cut at the starts of
+// the fused pass's tiles (every R positions), so no piece crosses a tile.
+// Bit j of the piece head words is set when position j starts a piece (or
+// lies at or beyond n, as sentinel); wcount counts the heads inside [0, n).
+__global__ __launch_bounds__(kBlock) void k_piece_heads(const int32_t *__restrict__ ref_s,
+                                                        const int32_t *__restrict__ ref_t, int64_t n, int64_t R,
+                                                        uint64_t *__restrict__ heads, int64_t nwords,
+                                                        uint32_t *__restrict__ wcount) {
+    const int64_t w = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= nwords) return;
+    const int64_t j = w * 64 + lane;
+    bool h;
+    if (j >= n) h = true;
+    else if (j == 0 || j % R == 0) h = true;
+    else h = ref_s[j] != ref_s[j - 1] || ref_t[j] != ref_t[j - 1];
+    const uint64_t word = __ballot(h);
+    if (lane == 0) {
+        heads[w] = word;
+        uint64_t real = word;
+        const int64_t lo = w * 64;
+        if (lo >= n) real = 0;
+        else if (n - lo < 64) real &= (~0ull >> (64 - (n - lo)));
+        wcount[w] = (uint32_t)__popcll(real);
+    }
+}
+
+// piece_s[ordinal], piece_t[ordinal]: the FE s and FE t levels of each piece
+// (ordinal = piece heads before its first position)
+__global__ void k_piece_levels(const int32_t *__restrict__ ref_s, const int32_t *__restrict__ ref_t, int64_t n,
+                               const uint64_t *__restrict__ heads, const uint32_t *__restrict__ wprefix,
+                               int32_t *__restrict__ piece_s, int32_t *__restrict__ piece_t) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t word = heads[j >> 6];
+    const int b = (int)(j & 63);
+    if (!((word >> b) & 1)) return;
+    const uint32_t ord = wprefix[j >> 6] + (uint32_t)__popcll(word & lanes_le(b)) - 1u;
+    piece_s[ord] = ref_s[j];
+    piece_t[ord] = ref_t[j];
+}
+
 // ---- reference refs -> device int32 (0-based) with range check -----------
 
 template <typename R>

@@ -247,15 +247,19 @@ static void free_sort_scratch(fehip_solver *h) {
     h->sort_cap = 0;
 }
 
-static int exclusive_scan_u32(fehip_solver *h, const uint32_t *in, int64_t n, uint32_t *out, bool write_total,
+// sums: (n + kSortTile - 1) / kSortTile + 1 entries
+static int exclusive_scan_u32(hipStream_t st, const uint32_t *in, int64_t n, uint32_t *out, bool write_total,
                               uint32_t *sums) {
     const unsigned nb = grid_for(n, kSortTile);
-    hipLaunchKernelGGL(k_scan_chunk_sums, dim3(nb), dim3(kBlock), 0, h->stream, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums_single, dim3(1), dim3(kBlock), 0, h->stream, sums, (int64_t)nb);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, h->stream, in, n, sums, out,
-                       write_total ? 1 : 0);
+    hipLaunchKernelGGL(k_scan_chunk_sums, dim3(nb), dim3(kBlock), 0, st, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums_single, dim3(1), dim3(kBlock), 0, st, sums, (int64_t)nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, st, in, n, sums, out, write_total ? 1 : 0);
     HIPCHK(hipGetLastError());
     return 0;
+}
+static int exclusive_scan_u32(fehip_solver *h, const uint32_t *in, int64_t n, uint32_t *out, bool write_total,
+                              uint32_t *sums) {
+    return exclusive_scan_u32(h->stream, in, n, out, write_total, sums);
 }
 
 // keys (0..nlev-1) -> sorted keys + the stably permuted values (vals_in, or
@@ -790,6 +794,52 @@ static int64_t tile_size(int64_t N, int64_t ntiles) {
     return std::max<int64_t>(256, (t + 255) / 256 * 256);
 }
 
+// Tile starts over positions [0, n) of a segmented FE with segment offsets
+// off (nlev + 1 entries), ending with n: each tile is cut at the last segment
+// head at or before its start + tp, unless that segment began at or before
+// the start (a segment longer than a tile): then at start + tp.
+static void cut_at_heads(const int32_t *off, int64_t nlev, int64_t n, int64_t tp, std::vector<int64_t> &tsh) {
+    for (int64_t q = 0; q < n;) {
+        tsh.push_back(q);
+        const int64_t tgt = q + tp;
+        if (tgt >= n) break;
+        const int64_t head = *(std::upper_bound(off, off + nlev + 1, tgt) - 1);
+        q = head > q ? head : tgt;
+    }
+    tsh.push_back(n);
+}
+
+// The piece form of the plain iteration (factored handles whose only CSR FE
+// is the secondary sort key t): the fused pass sums sqrt(w) .* u over pieces
+// -- runs of one (FE s level, FE t level) pair inside one fused tile -- and
+// one launch then forms both FEs' level sums from the piece sums, FE s over
+// its contiguous pieces and FE t through the pieces' CSR by FE t level.
+// This replaces the CSR pass's gather of u over all N observations by a
+// gather of the pieces' sums.
+struct PieceForm {
+    int64_t np = 0;                  // pieces (0: the form is off for this plan)
+    uint64_t *heads = nullptr;       // piece heads over the internal order (+ sentinel bits)
+    uint32_t *wprefix = nullptr;     // heads before each word
+    int32_t *piece_s = nullptr;      // piece -> FE s level
+    int32_t *pt_perm = nullptr;      // pieces ordered stably by FE t level
+    FeInfo fs, ft;                   // head bits, word prefixes and level maps of the pieces by FE s / FE t level
+    int64_t *tstart = nullptr;       // tiles over pieces: FE s's, then FE t's, each ending with np
+    int64_t ts0_t = 0;               // FE t's first entry in tstart
+    int32_t tile0_t = 0, total_tiles = 0;
+    int32_t nsegs_s = 0, nsegs_t = 0;  // observed levels
+    unsigned blocks = 0;             // workgroups of the piece launch's tiles
+    unsigned fix1 = 1, fix2 = 1;     // workgroups of its fix-up parts (sized by their items)
+    double *tileA = nullptr, *tileB = nullptr;
+    // norm partials of one iteration's D'u, one span: part 1 [0, fix1), the
+    // tiles [fix1, fix1 + blocks), part 2 after them (when it runs)
+    double *part = nullptr;
+    // part 1 (the piece launch's first blocks): few-level FEs' levels, levels
+    // without observations; part 2 (after it): levels whose pieces cross tiles
+    FixItem *items1 = nullptr, *items2 = nullptr;
+    int32_t n1 = 0, n2 = 0;
+    void *psum = nullptr;            // np x K, T
+};
+
 struct Plan {
     int K = 0;
     int64_t waves_f = 0, waves_c = 0;  // resident waves of the fused pass / the CSR launch
@@ -834,7 +884,11 @@ struct Plan {
     int32_t nitems_sd = 0, nitems_c = 0, nitems_d = 0;  // items_d: the fused few-level FEs' (projected part 1)
     void *zs = nullptr;
     ColOps *ops_one = nullptr, *ops_neg = nullptr, *ops_sub = nullptr;
+    PieceForm pc;
 };
+
+static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<FixItem> &dense_items,
+                        int64_t tiles);
 
 // waves_f / waves_c: how many waves of the fused pass / of the CSR D'u launch
 // the device holds at once; each gets one tile
@@ -882,17 +936,10 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
         const size_t b0 = tsh.size();
         if (g == h->s) {
             for (int64_t q = 0; q < N; q += tp) tsh.push_back(q);
+            tsh.push_back(N);
         } else {
-            const int32_t *off = f.h_ioff.data();
-            for (int64_t q = 0; q < N;) {
-                tsh.push_back(q);
-                const int64_t tgt = q + tp;
-                if (tgt >= N) break;
-                const int64_t head = *(std::upper_bound(off, off + f.n + 1, tgt) - 1);
-                q = head > q ? head : tgt;
-            }
+            cut_at_heads(f.h_ioff.data(), f.n, N, tp, tsh);
         }
-        tsh.push_back(N);
         const int64_t *tb = tsh.data() + b0;
         const int64_t ntl = (int64_t)(tsh.size() - b0) - 1;
         auto tile_of = [&](int64_t pos) -> int64_t { return (int64_t)(std::upper_bound(tb, tb + ntl + 1, pos) - tb) - 1; };
@@ -1076,10 +1123,164 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     CHK(dalloc_bytes(h, &P->hh, ts * h->p * K));
     CHK(dalloc_bytes(h, &P->hbar, ts * h->p * K));
     if (h->factored) CHK(dalloc_bytes(h, &P->vt, ts * h->p * K));
+    {
+        std::vector<FixItem> dense_items;
+        for (const FixItem &fi : items)
+            if (fi.kind == 1) dense_items.push_back(fi);
+        const int64_t tiles_c =
+            h->sched_tiles_c > 0 ? h->sched_tiles_c : env_int("FEHIP_TILES_C", waves_c, 1, 1 << 20);
+        CHK(build_pieces(h, P, ts, dense_items, tiles_c));
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     h->plans[key] = pl.release();
     *out = P;
     return 0;
+}
+
+
+// Builds the piece form of plan P (see PieceForm) when the handle's shape
+// allows it: factored (no interactions), the secondary sort key t is the only
+// CSR FE, at most three FEs (compile-time FE positions in the fused pass), and
+// pieces are at most half the observations (else the CSR pass over the
+// observations reads less).  FEHIP_PIECES=0 keeps the CSR pass.
+// dense_items: the plan's few-level FE items; tiles: target tile count of the
+// piece launch (both FEs together).
+static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<FixItem> &dense_items,
+                        int64_t tiles) {
+    PieceForm &pc = P->pc;
+    const int s = h->s, t = h->t_sec;
+    const int64_t N = h->N;
+    if (!h->factored || t < 0 || h->fe[t].kind != KIND_CSR || h->G > 3 || env_int("FEHIP_PIECES", 1, 0, 1) == 0)
+        return 0;
+    for (int g = 0; g < h->G; g++)
+        if (g != t && h->fe[g].kind == KIND_CSR) return 0;
+    const int64_t nwords = N / 64 + 2;
+    uint64_t *heads = nullptr;
+    uint32_t *cnt = nullptr, *wp = nullptr, *sums = nullptr;
+    CHK(dalloc(h, &heads, nwords));
+    CHK(dalloc(h, &cnt, nwords + 1));
+    CHK(dalloc(h, &wp, nwords + 1));
+    CHK(dalloc(h, &sums, (nwords + kSortTile - 1) / kSortTile + 2));
+    hipLaunchKernelGGL(k_piece_heads, dim3(grid_for(nwords, kWavesPerBlock)), dim3(kBlock), 0, h->stream,
+                       (const int32_t *)h->fe[s].ref_int, (const int32_t *)h->fe[t].ref_int, N, P->R, heads, nwords,
+                       cnt);
+    HIPCHK(hipGetLastError());
+    CHK(exclusive_scan_u32(h, cnt, nwords, wp, true, sums));
+    uint32_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, wp + nwords, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, cnt);
+    dfree(h, sums);
+    const int64_t np = (int64_t)total;
+    // psum is addressed with 32-bit byte offsets in the fused pass
+    if (np < 1 || 2 * np > N || (double)np * P->K * ts >= 4294967296.0) {
+        dfree(h, heads);
+        dfree(h, wp);
+        return 0;
+    }
+    pc.np = np;
+    pc.heads = heads;
+    pc.wprefix = wp;
+    int32_t *lev_t = nullptr, *skeys = nullptr, *off_s = nullptr, *off_t = nullptr;
+    CHK(dalloc(h, &pc.piece_s, np));
+    CHK(dalloc(h, &lev_t, np));
+    hipLaunchKernelGGL(k_piece_levels, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
+                       (const int32_t *)h->fe[s].ref_int, (const int32_t *)h->fe[t].ref_int, N,
+                       (const uint64_t *)heads, (const uint32_t *)wp, pc.piece_s, lev_t);
+    HIPCHK(hipGetLastError());
+    // by FE s level: the pieces are already in that order
+    const int64_t ns = h->n[s], nt = h->n[t];
+    CHK(dalloc(h, &off_s, ns + 1));
+    CHK(csr_products(h, pc.piece_s, np, ns, off_s, &pc.fs));
+    // by FE t level: a stable sort of the pieces
+    CHK(dalloc(h, &skeys, np));
+    CHK(dalloc(h, &pc.pt_perm, np));
+    CHK(radix_sort(h, lev_t, np, nt, skeys, pc.pt_perm));
+    CHK(dalloc(h, &off_t, nt + 1));
+    CHK(csr_products(h, skeys, np, nt, off_t, &pc.ft));
+    std::vector<int32_t> hs(ns + 1), ht(nt + 1);
+    HIPCHK(hipMemcpyAsync(hs.data(), off_s, sizeof(int32_t) * (ns + 1), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(ht.data(), off_t, sizeof(int32_t) * (nt + 1), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, lev_t);
+    dfree(h, skeys);
+    dfree(h, off_s);
+    dfree(h, off_t);
+    free_sort_scratch(h);
+    // tiles over the pieces, cut at level heads (a quarter of the CSR pass's
+    // target tile count per FE: the piece launch is short, and longer tiles
+    // keep each wave's loads in flight; the autotuner's candidates then span
+    // about 256 to 1800 pieces per tile on the benchmark panel); fix-up items
+    const int64_t tp = tile_size(np, std::max<int64_t>(1, tiles / 8));
+    std::vector<int64_t> tsh;
+    std::vector<FixItem> it1(dense_items), it2;
+    int32_t t0 = 0;
+    for (int q = 0; q < 2; q++) {
+        const int g = q == 0 ? s : t;
+        const std::vector<int32_t> &off = q == 0 ? hs : ht;
+        const int64_t nlev = h->n[g];
+        const size_t b0 = tsh.size();
+        if (q == 1) {
+            pc.ts0_t = (int64_t)b0;
+            pc.tile0_t = t0;
+        }
+        cut_at_heads(off.data(), nlev, np, tp, tsh);
+        const int64_t *tb = tsh.data() + b0;
+        const int64_t ntl = (int64_t)(tsh.size() - b0) - 1;
+        auto tile_of = [&](int64_t pos) -> int64_t {
+            return (int64_t)(std::upper_bound(tb, tb + ntl + 1, pos) - tb) - 1;
+        };
+        int32_t nonempty = 0;
+        for (int64_t k = 0; k < nlev; k++) {
+            const int64_t st = off[k], en = off[k + 1];
+            FixItem fi{};
+            fi.fe = g;
+            fi.level = (int32_t)k;
+            fi.coef_off = h->off[g];
+            if (en <= st) {
+                fi.kind = 2;
+                it1.push_back(fi);
+                continue;
+            }
+            nonempty++;
+            const int64_t tf = tile_of(st), tl = tile_of(en - 1);
+            if (tf == tl) continue;
+            fi.kind = 0;
+            fi.first = t0 + tf;
+            fi.count = (int32_t)(tl - tf);
+            it2.push_back(fi);
+        }
+        (q == 0 ? pc.nsegs_s : pc.nsegs_t) = std::max<int32_t>(nonempty, 1);
+        t0 += (int32_t)ntl;
+    }
+    pc.total_tiles = t0;
+    pc.blocks = (unsigned)std::min<int64_t>(grid_for(t0, kWavesPerBlock), 2048);
+    pc.fix1 = (unsigned)std::min<int64_t>(grid_for((int64_t)it1.size(), kWavesPerBlock), 1024);
+    pc.fix2 = (unsigned)std::min<int64_t>(grid_for((int64_t)it2.size(), kWavesPerBlock), 1024);
+    CHK(dalloc(h, &pc.tstart, tsh.size()));
+    HIPCHK(hipMemcpyAsync(pc.tstart, tsh.data(), sizeof(int64_t) * tsh.size(), hipMemcpyHostToDevice, h->stream));
+    pc.n1 = (int32_t)it1.size();
+    pc.n2 = (int32_t)it2.size();
+    CHK(dalloc(h, &pc.items1, std::max<int32_t>(pc.n1, 1)));
+    CHK(dalloc(h, &pc.items2, std::max<int32_t>(pc.n2, 1)));
+    if (pc.n1)
+        HIPCHK(hipMemcpyAsync(pc.items1, it1.data(), sizeof(FixItem) * pc.n1, hipMemcpyHostToDevice, h->stream));
+    if (pc.n2)
+        HIPCHK(hipMemcpyAsync(pc.items2, it2.data(), sizeof(FixItem) * pc.n2, hipMemcpyHostToDevice, h->stream));
+    CHK(dalloc(h, &pc.tileA, (size_t)std::max<int32_t>(t0, 1) * P->K));
+    CHK(dalloc(h, &pc.tileB, (size_t)std::max<int32_t>(t0, 1) * P->K));
+    CHK(dalloc(h, &pc.part, (size_t)(pc.fix1 + pc.blocks + pc.fix2) * P->K));
+    CHK(dalloc_bytes(h, &pc.psum, ts * np * P->K));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static void destroy_pieces(fehip_solver *h, PieceForm &pc) {
+    void *ptrs[] = {pc.heads, pc.wprefix, pc.piece_s, pc.pt_perm, pc.fs.heads, pc.fs.wprefix, pc.fs.seg_level,
+                    pc.ft.heads, pc.ft.wprefix, pc.ft.seg_level, pc.tstart, pc.tileA, pc.tileB, pc.part,
+                    pc.items1, pc.items2, pc.psum};
+    for (void *q : ptrs) dfree(h, q);
+    pc = PieceForm{};
 }
 
 // LDS of the fused pass: the few-level FEs' accumulators when they fit 64 KB
@@ -1136,6 +1337,7 @@ static void destroy_plan(fehip_solver *h, Plan *P) {
                     P->v, P->hh, P->hbar, P->dY, P->tstart, P->items_s, P->items_o, P->items_sd, P->items_c,
                     P->zs, P->ops_one, P->ops_neg, P->ops_sub, P->items_d};
     for (void *q : ptrs) dfree(h, q);
+    destroy_pieces(h, P->pc);
     delete P;
 }
 
@@ -1485,12 +1687,19 @@ struct Engine {
 
     // u stored as sqrt(w) .* u in this engine's iteration (k_fused's rule)
     static bool ut(const fehip_solver *h) { return h->factored && stores_scaled_u<K, true>(); }
+    // the iteration runs in the piece form (plain LSMR on a plan that has it)
+    static bool use_pieces(const fehip_solver *h, const Plan *P) { return P->pc.np > 0 && !h->fwl_now; }
+    // ... and stores u scaled (never in the piece form)
+    static bool ut_now(const fehip_solver *h, const Plan *P) { return ut(h) && !use_pieces(h, P); }
 
     template <int PM>
     static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         const dim3 gr(P->fused_blocks), bl(kBlock);
         const size_t lds = PM == 1 ? 0 : P->fused_lds;  // the sums-only pass has no few-level sums
-        if (h->factored) {
+        if constexpr (PM == 3) {  // factored, two or three FEs (build_pieces)
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, 3>), gr, bl, lds, h->stream, p);
+            else hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, 3>), gr, bl, lds, h->stream, p);
+        } else if (h->factored) {
             if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, PM>), gr, bl, lds, h->stream, p);
             else if (h->G == 2)
                 hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, PM>), gr, bl, lds, h->stream, p);
@@ -1568,6 +1777,14 @@ struct Engine {
             p.zsw = (T *)P->zs;
             p.gsrc[0] = (const T *)(h->factored ? h->scale2_T[pidx<T>()] : h->fe[h->s].scale_T[pidx<T>()]);
             launch_fused<1>(h, P, p);
+        } else if (pm == 3) {
+            const PieceForm &pc = P->pc;
+            p.heads = pc.heads;
+            p.wprefix = pc.wprefix;
+            p.seg_level = pc.piece_s;
+            p.nsegs_s = (int32_t)pc.np;
+            p.psum = (T *)pc.psum;
+            launch_fused<3>(h, P, p);
         } else if (pm == 2) {
             p.gsrc[0] = (const T *)P->zs;
             launch_fused<2>(h, P, p);
@@ -1771,6 +1988,101 @@ struct Engine {
     // the iteration takes the merged launch (FEs beside FE s in CSR form)
     static bool use_seg_fix(const fehip_solver *h, const Plan *P) { return h->seg_fix && P->csr_blocks; }
 
+    // Piece form: D'u of FE s (its contiguous pieces, identity order) and of
+    // FE t (its pieces through pt_perm) as segmented sums of the piece sums,
+    // each level's sum taking its FE's scale (RAW)
+    static DtxSegParams<T> piece_seg_params(fehip_solver *h, Plan *P, const ColOps *ops, T *coef,
+                                            const int32_t *early_exit) {
+        const PieceForm &pc = P->pc;
+        DtxSegParams<T> p{};
+        p.N = pc.np;
+        p.nseg = 2;
+        p.first_tile = 0;
+        p.total_tiles = pc.total_tiles;
+        for (int q = 0; q < 2; q++) {
+            const int g = q == 0 ? h->s : h->t_sec;
+            const FeInfo &fi = q == 0 ? pc.fs : pc.ft;
+            SegFE<T> &e = p.fe[q];
+            e.perm = q == 0 ? h->ident : pc.pt_perm;
+            e.cache = nullptr;  // RAW: not read
+            e.heads = fi.heads;
+            e.wprefix = fi.wprefix;
+            e.seg_level = fi.seg_level;
+            e.nsegs = q == 0 ? pc.nsegs_s : pc.nsegs_t;
+            e.lscale = (const T *)h->fe[g].scale_T[pidx<T>()];
+            e.coef_off = h->off[g];
+            e.tstart = pc.tstart + (q == 0 ? 0 : pc.ts0_t);
+            e.tile0 = q == 0 ? 0 : pc.tile0_t;
+            e.ntiles = q == 0 ? pc.tile0_t : pc.total_tiles - pc.tile0_t;
+        }
+        p.u = (const T *)pc.psum;
+        p.v = coef;
+        p.tileA = pc.tileA;
+        p.tileB = pc.tileB;
+        p.ops = ops;
+        p.partials = pc.part + (size_t)pc.fix1 * K;
+        p.all_done = early_exit;
+        p.mode = MODE_ITER;
+        return p;
+    }
+
+    // part 1: few-level FEs' levels and levels without observations (first
+    // blocks of the piece launch); part 2: levels whose pieces cross tiles
+    static FixParams<T> piece_fix_params(fehip_solver *h, Plan *P, const ColOps *ops, T *coef,
+                                         const int32_t *early_exit, int part) {
+        const PieceForm &pc = P->pc;
+        FixParams<T> f{};
+        f.nitems = part == 1 ? pc.n1 : pc.n2;
+        f.items = part == 1 ? pc.items1 : pc.items2;
+        f.tileA = pc.tileA;
+        f.tileB = pc.tileB;
+        f.dpart = P->dpart;
+        f.v = coef;
+        f.ops = ops;
+        f.partials = pc.part + (part == 2 ? (size_t)(pc.fix1 + pc.blocks) * K : 0);
+        f.all_done = early_exit;
+        f.mode = MODE_ITER;
+        // raw sums of sqrt(w) .* u take their FE's scale here: the pieces' FEs
+        // and the few-level FEs summed in the fused pass
+        for (int g = 0; g < h->G; g++) {
+            const bool raw = g == h->s || g == h->t_sec || (h->fe[g].kind == KIND_DENSE && P->fused_dense);
+            if (raw) f.fscale[g] = (const T *)h->fe[g].scale_T[pidx<T>()];
+        }
+        return f;
+    }
+
+    static void piece_dtx(fehip_solver *h, Plan *P, const ColOps *ops, T *coef, const int32_t *early_exit) {
+        const DtxSegParams<T> p = piece_seg_params(h, P, ops, coef, early_exit);
+        const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 1);
+        const int nfix = (int)P->pc.fix1;
+        hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true>), dim3(P->pc.fix1 + P->pc.blocks), dim3(kBlock), 0,
+                           h->stream, p, f, nfix);
+    }
+
+    static void piece_fix2(fehip_solver *h, Plan *P, const ColOps *ops, T *coef, const int32_t *early_exit) {
+        const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 2);
+        hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->pc.fix2), dim3(kBlock), 0, h->stream, f);
+    }
+
+    static void piece_fin_dtx(fehip_solver *h, Plan *P) {
+        const PieceForm &pc = P->pc;
+        hipLaunchKernelGGL((k_fin_dtx<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream, (const double *)pc.part,
+                           (int64_t)(pc.fix1 + pc.blocks + (pc.n2 ? pc.fix2 : 0)), (const double *)nullptr,
+                           (int64_t)0, MODE_ITER, P->state, P->ops_db, P->all_done, (const int32_t *)P->all_done);
+    }
+
+    // one LSMR iteration in the piece form
+    static void piece_iteration(fehip_solver *h, Plan *P) {
+        fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u, 3);
+        fin_db(h, P, MODE_ITER, P->all_done, true);
+        if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, false);
+        piece_dtx(h, P, P->ops_dtx, (T *)P->v, P->all_done);
+        if (P->pc.n2) piece_fix2(h, P, P->ops_dtx, (T *)P->v, P->all_done);
+        piece_fin_dtx(h, P);
+        update(h, P);
+        fin_update(h, P);
+    }
+
     static void dtx(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
                     const int32_t *early_exit) {
         dtx_seg(h, P, mode, ops, y, coef, early_exit);
@@ -1802,7 +2114,7 @@ struct Engine {
         hipLaunchKernelGGL((k_init_vectors<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, pK, (T *)P->v,
                            (T *)P->hh, (T *)P->hbar, (const void *)P->state, (const T *)h->scale_all[pidx<T>()],
                            (T *)P->vt);
-        if (ut(h) && h->sqrtw_T[pidx<T>()])
+        if (ut_now(h, P) && h->sqrtw_T[pidx<T>()])
             hipLaunchKernelGGL(k_scale_rows<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, (T *)P->u,
                                (const T *)h->sqrtw_T[pidx<T>()], h->N, K);
         HIPCHK(hipGetLastError());
@@ -1816,6 +2128,29 @@ struct Engine {
             auto mark = [&](int q) {
                 if (rec) (void)hipEventRecord(ev[q], h->stream);
             };
+            if (use_pieces(h, P)) {
+                // slots as below: the piece launch counts as the CSR slot,
+                // its part 2 as the fix-up
+                mark(0);
+                fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u, 3);
+                mark(1);
+                fin_db(h, P, MODE_ITER, P->all_done, true);
+                mark(2);
+                mark(3);
+                if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, false);
+                mark(4);
+                piece_dtx(h, P, P->ops_dtx, (T *)P->v, P->all_done);
+                mark(5);
+                if (P->pc.n2) piece_fix2(h, P, P->ops_dtx, (T *)P->v, P->all_done);
+                mark(6);
+                piece_fin_dtx(h, P);
+                mark(7);
+                update(h, P);
+                mark(8);
+                fin_update(h, P);
+                mark(9);
+                return;
+            }
             mark(0);
             fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
             mark(1);
@@ -1858,7 +2193,7 @@ struct Engine {
         HIPCHK(hipMemcpy(&stopped, P->all_done, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (stopped) return fail(FEHIP_EHIP, "kernel timing: the filler LSMR stopped early");
         for (int q = 0; q < 8; q++) out_ms[q] = acc[q] / std::max(reps, 1);
-        if (!P->csr_blocks) out_ms[2] = 0.0;
+        if (!P->csr_blocks && !use_pieces(h, P)) out_ms[2] = 0.0;
         if (P->fused_dense || !P->ndense) out_ms[3] = 0.0;
         return 0;
     }
@@ -1962,6 +2297,10 @@ struct Engine {
             }
             update(h, P);
             fin_update(h, P);
+            return;
+        }
+        if (use_pieces(h, P)) {
+            piece_iteration(h, P);
             return;
         }
         fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u);
@@ -2078,8 +2417,9 @@ struct Engine {
         hipLaunchKernelGGL((k_init_vectors<T, Tr, K>), dim3(P->upd_blocks), dim3(kBlock), 0, h->stream, h->p * K,
                            (T *)P->v, (T *)P->hh, (T *)P->hbar, (const void *)P->state,
                            (const T *)h->scale_all[pidx<T>()], (T *)P->vt);
-        // the factored iteration with K < 8 keeps u as sqrt(w) .* u
-        if (ut(h) && h->sqrtw_T[pidx<T>()])
+        // the factored iteration with K < 8 keeps u as sqrt(w) .* u (not
+        // the piece form)
+        if (ut_now(h, P) && h->sqrtw_T[pidx<T>()])
             hipLaunchKernelGGL(k_scale_rows<T>, dim3(grid_for(h->N * K, kBlock)), dim3(kBlock), 0, h->stream,
                                (T *)P->u, (const T *)h->sqrtw_T[pidx<T>()], h->N, K);
         HIPCHK(hipGetLastError());
@@ -2722,14 +3062,54 @@ static int full_dev(fehip_solver *h, const double *dc, int g, double *out) {
 // Number the distinct values of u (n unsigned keys < 2^bits) densely in
 // ascending order into dense (n int64); returns the count in *ngroups.
 // kb/vb: ping-pong key/value buffers of n entries; hist/sums/flag/ex: scratch.
+// Device buffers of one grouping call of N observations (group_keys).
 struct GroupScratch {
-    uint64_t *kb[2] = {nullptr, nullptr};
-    int32_t *vb[2] = {nullptr, nullptr};
-    uint32_t *hist = nullptr, *sums = nullptr, *flag = nullptr, *ex = nullptr, *sums2 = nullptr;
+    uint64_t *kb[2] = {nullptr, nullptr};  // N each: radix sort key buffers
+    int32_t *vb[2] = {nullptr, nullptr};   // N each: radix sort index buffers
+    uint32_t *hist = nullptr;              // 256 * ntiles + 1
+    uint32_t *sums = nullptr;              // (256 * ntiles + kSortTile - 1) / kSortTile + 2
+    uint32_t *flag = nullptr;              // N
+    uint32_t *ex = nullptr;                // N + 1
+    uint32_t *sums2 = nullptr;             // (N + kSortTile - 1) / kSortTile + 2
+    int64_t *dense = nullptr, *dcol = nullptr;  // N each: numberings
+    uint64_t *u = nullptr;                 // N: keys as unsigned offsets
+    long long *mm = nullptr;               // 2: min, max
     int64_t ntiles = 0;
 };
 
-static int number_keys(fehip_solver *h, GroupScratch &gs, const uint64_t *u, int64_t n, int bits, int64_t *dense,
+// Carves the scratch of an N-observation grouping call out of one block of
+// memory, each piece 256-byte aligned; returns the bytes used (base may be
+// nullptr to only measure).
+static size_t carve_group_scratch(unsigned char *base, int64_t N, GroupScratch &gs) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> unsigned char * {
+        unsigned char *q = base ? base + off : nullptr;
+        off += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+        return q;
+    };
+    gs.ntiles = (N + kSortTile - 1) / kSortTile;
+    for (int b = 0; b < 2; b++) gs.kb[b] = (uint64_t *)take(sizeof(uint64_t) * N);
+    for (int b = 0; b < 2; b++) gs.vb[b] = (int32_t *)take(sizeof(int32_t) * N);
+    gs.hist = (uint32_t *)take(sizeof(uint32_t) * (256 * gs.ntiles + 1));
+    gs.sums = (uint32_t *)take(sizeof(uint32_t) * ((256 * gs.ntiles + kSortTile - 1) / kSortTile + 2));
+    gs.flag = (uint32_t *)take(sizeof(uint32_t) * N);
+    gs.ex = (uint32_t *)take(sizeof(uint32_t) * (N + 1));
+    gs.sums2 = (uint32_t *)take(sizeof(uint32_t) * ((N + kSortTile - 1) / kSortTile + 2));
+    gs.dense = (int64_t *)take(sizeof(int64_t) * N);
+    gs.dcol = (int64_t *)take(sizeof(int64_t) * N);
+    gs.u = (uint64_t *)take(sizeof(uint64_t) * N);
+    gs.mm = (long long *)take(sizeof(long long) * 2);
+    return off;
+}
+
+__global__ void k_minmax_init(long long *mm) {
+    if (threadIdx.x == 0) {
+        mm[0] = INT64_MAX;
+        mm[1] = INT64_MIN;
+    }
+}
+
+static int number_keys(hipStream_t st, GroupScratch &gs, const uint64_t *u, int64_t n, int bits, int64_t *dense,
                        int64_t *ngroups) {
     const int passes = std::max(1, (bits + 7) / 8);
     const uint64_t *kin = u;
@@ -2738,24 +3118,24 @@ static int number_keys(fehip_solver *h, GroupScratch &gs, const uint64_t *u, int
     for (int ps = 0; ps < passes; ps++) {
         uint64_t *kout = gs.kb[buf];
         int32_t *vout = gs.vb[buf];
-        hipLaunchKernelGGL(k_radix_hist<uint64_t>, dim3((unsigned)gs.ntiles), dim3(kBlock), 0, h->stream, kin, n,
-                           8 * ps, gs.hist, gs.ntiles);
-        CHK(exclusive_scan_u32(h, gs.hist, 256 * gs.ntiles, gs.hist, false, gs.sums));
-        hipLaunchKernelGGL(k_radix_scatter<uint64_t>, dim3((unsigned)gs.ntiles), dim3(kBlock), 0, h->stream, kin,
-                           vin, kout, vout, n, 8 * ps, gs.hist, gs.ntiles);
+        hipLaunchKernelGGL(k_radix_hist<uint64_t>, dim3((unsigned)gs.ntiles), dim3(kBlock), 0, st, kin, n, 8 * ps,
+                           gs.hist, gs.ntiles);
+        CHK(exclusive_scan_u32(st, gs.hist, 256 * gs.ntiles, gs.hist, false, gs.sums));
+        hipLaunchKernelGGL(k_radix_scatter<uint64_t>, dim3((unsigned)gs.ntiles), dim3(kBlock), 0, st, kin, vin, kout,
+                           vout, n, 8 * ps, gs.hist, gs.ntiles);
         HIPCHK(hipGetLastError());
         kin = kout;
         vin = vout;
         buf ^= 1;
     }
-    hipLaunchKernelGGL(k_group_heads, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, h->stream, kin, n, gs.flag);
-    CHK(exclusive_scan_u32(h, gs.flag, n, gs.ex, true, gs.sums2));
-    hipLaunchKernelGGL(k_group_assign, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, h->stream,
-                       (const uint32_t *)gs.ex, (const uint32_t *)gs.flag, vin, n, dense);
+    hipLaunchKernelGGL(k_group_heads, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, kin, n, gs.flag);
+    CHK(exclusive_scan_u32(st, gs.flag, n, gs.ex, true, gs.sums2));
+    hipLaunchKernelGGL(k_group_assign, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, (const uint32_t *)gs.ex,
+                       (const uint32_t *)gs.flag, vin, n, dense);
     HIPCHK(hipGetLastError());
     uint32_t total = 0;
-    HIPCHK(hipMemcpyAsync(&total, gs.ex + n, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpyAsync(&total, gs.ex + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     *ngroups = total;
     return 0;
 }
@@ -2766,56 +3146,59 @@ static int bits_of(uint64_t range) {
     return b;
 }
 
-static int group_keys(fehip_solver *h, int64_t N, int32_t ncols, const int64_t *const *keys, int64_t *refs,
-                      int64_t *ngroups) {
-    GroupScratch gs;
-    gs.ntiles = (N + kSortTile - 1) / kSortTile;
-    for (int b = 0; b < 2; b++) {
-        CHK(dalloc(h, &gs.kb[b], N));
-        CHK(dalloc(h, &gs.vb[b], N));
-    }
-    CHK(dalloc(h, &gs.hist, 256 * gs.ntiles + 1));
-    CHK(dalloc(h, &gs.sums, (256 * gs.ntiles + kSortTile - 1) / kSortTile + 2));
-    CHK(dalloc(h, &gs.flag, N));
-    CHK(dalloc(h, &gs.ex, N + 1));
-    CHK(dalloc(h, &gs.sums2, (N + kSortTile - 1) / kSortTile + 2));
-    int64_t *dkey = nullptr, *dense = nullptr, *dcol = nullptr;
-    uint64_t *u = nullptr;
-    long long *mm = nullptr;
-    CHK(dalloc(h, &dkey, N));
-    CHK(dalloc(h, &dense, N));
-    CHK(dalloc(h, &dcol, N));
-    CHK(dalloc(h, &u, N));
-    CHK(dalloc(h, &mm, 2));
+// The grouping on device-resident keys (dkeys[c]: N int64 each) into drefs
+// (N int64, 1-based), with every buffer taken from gs; on stream st.
+static int group_keys_device(hipStream_t st, GroupScratch &gs, int64_t N, int32_t ncols,
+                             const int64_t *const *dkeys, int64_t *drefs, int64_t *ngroups) {
     int64_t n_prev = 0;
     for (int c = 0; c < ncols; c++) {
-        HIPCHK(hipMemcpyAsync(dkey, keys[c], sizeof(int64_t) * N, hipMemcpyHostToDevice, h->stream));
-        const long long init[2] = {INT64_MAX, INT64_MIN};
-        HIPCHK(hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, gs.mm);
         hipLaunchKernelGGL(k_minmax_i64, dim3((unsigned)std::min<int64_t>(grid_for(N, kBlock), 4096)), dim3(kBlock),
-                           0, h->stream, (const int64_t *)dkey, N, mm);
+                           0, st, dkeys[c], N, gs.mm);
         long long hm[2];
-        HIPCHK(hipMemcpyAsync(hm, mm, sizeof(hm), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        hipLaunchKernelGGL(k_key_offset, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
-                           (const int64_t *)dkey, N, (int64_t)hm[0], u);
+        HIPCHK(hipMemcpyAsync(hm, gs.mm, sizeof(hm), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        hipLaunchKernelGGL(k_key_offset, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, st, dkeys[c], N, (int64_t)hm[0],
+                           gs.u);
         int64_t nc = 0;
-        CHK(number_keys(h, gs, u, N, bits_of((uint64_t)hm[1] - (uint64_t)hm[0]), c == 0 ? dense : dcol, &nc));
+        CHK(number_keys(st, gs, gs.u, N, bits_of((uint64_t)hm[1] - (uint64_t)hm[0]), c == 0 ? gs.dense : gs.dcol,
+                        &nc));
         if (c == 0) {
             n_prev = nc;
             continue;
         }
         // combinations of distinct values: n_prev * nc <= N * N < 2^63
-        hipLaunchKernelGGL(k_group_combine, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
-                           (const int64_t *)dense, (const int64_t *)dcol, N, nc, u);
-        CHK(number_keys(h, gs, u, N, bits_of((uint64_t)(n_prev * nc - 1)), dense, &n_prev));
+        hipLaunchKernelGGL(k_group_combine, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, st,
+                           (const int64_t *)gs.dense, (const int64_t *)gs.dcol, N, nc, gs.u);
+        CHK(number_keys(st, gs, gs.u, N, bits_of((uint64_t)(n_prev * nc - 1)), gs.dense, &n_prev));
     }
-    hipLaunchKernelGGL(k_plus_one, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, (const int64_t *)dense, N,
-                       dkey);
+    hipLaunchKernelGGL(k_plus_one, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, st, (const int64_t *)gs.dense, N,
+                       drefs);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(refs, dkey, sizeof(int64_t) * N, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipStreamSynchronize(st));
     *ngroups = n_prev;
+    return 0;
+}
+
+// fehip_group: host keys in, host refs out, device buffers allocated per call
+static int group_keys(fehip_solver *h, int64_t N, int32_t ncols, const int64_t *const *keys, int64_t *refs,
+                      int64_t *ngroups) {
+    GroupScratch gs;
+    const size_t bytes = carve_group_scratch(nullptr, N, gs);
+    unsigned char *block = nullptr;
+    int64_t *dkeys = nullptr, *drefs = nullptr;
+    CHK(dalloc(h, &block, bytes));
+    carve_group_scratch(block, N, gs);
+    CHK(dalloc(h, &dkeys, (size_t)N * ncols));
+    CHK(dalloc(h, &drefs, N));
+    std::vector<const int64_t *> dk(ncols);
+    for (int c = 0; c < ncols; c++) {
+        dk[c] = dkeys + (size_t)c * N;
+        HIPCHK(hipMemcpyAsync(dkeys + (size_t)c * N, keys[c], sizeof(int64_t) * N, hipMemcpyHostToDevice, h->stream));
+    }
+    CHK(group_keys_device(h->stream, gs, N, ncols, dk.data(), drefs, ngroups));
+    HIPCHK(hipMemcpyAsync(refs, drefs, sizeof(int64_t) * N, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
@@ -2928,6 +3311,47 @@ int fehip_solve_residuals_panels(fehip_t *const *hs, int32_t npanels, double *co
     }
     if (rc != 0)
         for (int32_t i = 0; i < npanels; i++) (void)hipStreamSynchronize(hs[i]->stream);
+    return rc;
+}
+
+int fehip_solve_residuals_panels_host(fehip_t *const *hs, int32_t npanels, double *const *const *cols,
+                                      const int32_t *K, double tol, int64_t maxiter, int64_t *const *iters,
+                                      uint8_t *const *converged) {
+    g_err.clear();
+    if (!hs || !cols || !K || npanels < 0) return fail(FEHIP_EARG, "bad arguments");
+    for (int32_t i = 0; i < npanels; i++) {
+        if (!hs[i] || !cols[i]) return fail(FEHIP_EARG, "panel " + std::to_string(i) + ": NULL handle or columns");
+        if (K[i] < 1 || K[i] > kMaxK) return fail(FEHIP_EARG, "panel " + std::to_string(i) + ": K must be in 1..32");
+        for (int32_t c = 0; c < K[i]; c++)
+            if (!cols[i][c]) return fail(FEHIP_EARG, "panel " + std::to_string(i) + ": NULL column");
+    }
+    // each panel's columns are staged on its own handle (and stream)
+    std::vector<double *> dY(npanels, nullptr);
+    std::vector<int64_t> ld(npanels);
+    int rc = 0;
+    for (int32_t i = 0; i < npanels && rc == 0; i++) {
+        fehip_solver *h = hs[i];
+        ld[i] = h->N;
+        rc = dalloc(h, &dY[i], (size_t)h->N * K[i]);
+        for (int32_t c = 0; c < K[i] && rc == 0; c++) {
+            const hipError_t e = hipMemcpyAsync(dY[i] + (int64_t)c * h->N, cols[i][c], sizeof(double) * h->N,
+                                                hipMemcpyHostToDevice, h->stream);
+            if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+        }
+    }
+    if (rc == 0) rc = fehip_solve_residuals_panels(hs, npanels, dY.data(), K, ld.data(), tol, maxiter, iters, converged);
+    for (int32_t i = 0; i < npanels && rc == 0; i++) {
+        fehip_solver *h = hs[i];
+        for (int32_t c = 0; c < K[i] && rc == 0; c++) {
+            const hipError_t e = hipMemcpyAsync(cols[i][c], dY[i] + (int64_t)c * h->N, sizeof(double) * h->N,
+                                                hipMemcpyDeviceToHost, h->stream);
+            if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+        }
+    }
+    for (int32_t i = 0; i < npanels; i++) {
+        if (hs[i]) (void)hipStreamSynchronize(hs[i]->stream);
+        if (dY[i]) dfree(hs[i], dY[i]);
+    }
     return rc;
 }
 
@@ -3291,7 +3715,7 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out) {
         out[3] = P->total_tiles;
         out[4] = P->nitems;
         out[5] = P->nslot;
-        out[6] = 0;
+        out[6] = P->pc.np;
         out[7] = P->csr_blocks;
         return 0;
     };

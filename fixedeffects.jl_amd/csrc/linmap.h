@@ -855,6 +855,10 @@ struct FusedParams {
     // segment inside a tile goes straight to zsw (levels of FE s x K);
     // gsrc[0] is then the per-level scale_s^2 (read at the level, not x K)
     T *zsw;
+    // PM = 3: heads/wprefix/seg_level describe pieces (runs of one (FE s,
+    // FE t) level pair inside a tile; seg_level maps a piece to its FE s
+    // level), and each piece's sum of sqrt(w) .* u goes to psum[piece]
+    T *psum;
     // FACT: cache_g = sqrt(w) * scale_g[ref] for every FE (no interactions),
     // so D*v = sqrt(w) * sum_g vt_g[ref_g] with vt = scale .* v (kept by the
     // vector update) and the raw D'u sums are of sqrt(w)*u (scaled in the
@@ -906,6 +910,9 @@ __host__ __device__ constexpr bool stores_scaled_u() {
 // PM (projected iteration, FE s eliminated): 0 = plain pass; 1 = sums only:
 // the FE s segment sums of cache_s * (D*v over the other FEs), no u load,
 // store or norm, no dense sums; 2 = the u pass without FE s segment sums.
+// PM = 3: the plain pass in the piece form (factored, GT > 0): the segments
+// are pieces, every tile starts one, and each piece's raw sum goes to psum
+// (its ordinal in position order); u is stored as it is.
 template <typename T, typename Acc, int K, int GT, bool FACT, int PM = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
@@ -931,7 +938,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // column (nslot x K per wave) that the P lanes of a column add to in P
     // ordered phases with LDS adds -- a quarter to a half of the LDS
     constexpr bool PHASED = dense_phased<K, Acc>();
-    constexpr bool UT = stores_scaled_u<K, FACT>();
+    constexpr bool UT = stores_scaled_u<K, FACT>() && PM != 3;
+    static_assert(PM != 3 || (GT > 0 && FACT), "the piece form is a factored pass with compile-time FE positions");
     using DA = DenseAcc<K, Acc>;
     DA *acc = reinterpret_cast<DA *>(fused_smem) + (size_t)wave * p.nslot * (PHASED ? K : kWave);
     if constexpr (PHASED) {
@@ -991,12 +999,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         // segment (heads up to it, minus one), clamped for positions past N,
         // through seg_level when some level has no observation.  Replaces a
         // load of ref_s: positions are sorted by FE s.
-        auto level_s = [&](uint32_t lb, int q, const uint64_t (&hw)[NW], const uint32_t (&wp)[NW]) -> int32_t {
+        auto level_s = [&](uint32_t lb, int q, const uint64_t (&hw)[NW], const uint32_t (&wp)[NW],
+                           int32_t &ordv) -> int32_t {
             const uint32_t ob = lb + (uint32_t)(q * P);
             const int jw = (q * P) / kWave;  // constant after unrolling
             const int bit = (int)((ob + (uint32_t)psub) & 63u);
             uint32_t ord = wp[jw] + (uint32_t)__popcll(hw[jw] & lanes_le(bit)) - 1u;
             ord = ord < (uint32_t)p.nsegs_s ? ord : (uint32_t)p.nsegs_s - 1u;
+            ordv = (int32_t)ord;
             return p.seg_level[ord];
         };
         auto compute = [&](const Step &st, uint32_t lb) {
@@ -1027,6 +1037,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                 cur, &p.tileA[(int64_t)t * K + c])) {
                     if constexpr (PM == 1)
                         sto(p.zsw, (uint32_t)st.rs[q] * K + c, (T)(-(FACT ? (double)st.g0[q] : 1.0) * acc_to_double(x)));
+                    else if constexpr (PM == 3)
+                        sto(p.psum, (uint32_t)st.rs[q] * K + c, acc_to_T<T, Acc>(x));
                     else
                         sto(p.wsum, (uint32_t)st.rs[q] * K + c, acc_to_double(x));
                 }
@@ -1068,6 +1080,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             //   compute step 0
             struct Refs {
                 int32_t r[GA][U];
+                int32_t o[U];  // PM = 3: the piece ordinal of each position
                 uint64_t hw[NW];
                 uint32_t wp[NW];
             };
@@ -1082,7 +1095,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             };
             auto issue_level = [&](uint32_t lb, Refs &rr) {
 #pragma unroll
-                for (int q = 0; q < U; q++) rr.r[0][q] = level_s(lb, q, rr.hw, rr.wp);
+                for (int q = 0; q < U; q++) rr.r[0][q] = level_s(lb, q, rr.hw, rr.wp, rr.o[q]);
                 if constexpr (GT == 3) {
                     // packed pair: branch-free unpack (the same loads either way)
                     const uint32_t mask = p.packed ? 0x3FFFFFFu : 0xFFFFFFFFu;
@@ -1123,7 +1136,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    d.rs[q] = rr.r[0][q];
+                    d.rs[q] = PM == 3 ? rr.o[q] : rr.r[0][q];  // where a segment ending here is written
                     d.rd[q] = rr.r[GT - 1][q];
                 }
 #pragma unroll
@@ -1250,7 +1263,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 compute(st, lb);
             }
         }
-        if (PM != 2 && cur.open && on && psub == 0) {
+        // (PM = 3: every piece ends inside its tile)
+        if (PM != 2 && PM != 3 && cur.open && on && psub == 0) {
             if (cur.cont) p.tileA[(int64_t)t * K + c] = acc_to_double(cur.carry);
             else p.tileB[(int64_t)t * K + c] = acc_to_double(cur.carry);
         }

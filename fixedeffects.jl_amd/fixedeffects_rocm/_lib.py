@@ -42,6 +42,7 @@ EXPORTS = (
     "fehip_solve_residuals",
     "fehip_solve_residuals_device",
     "fehip_solve_residuals_panels",
+    "fehip_solve_residuals_panels_host",
     "fehip_solve_coefficients",
     "fehip_solve_coefficients_full",
     "fehip_normalize",
@@ -93,6 +94,9 @@ def _load() -> C.CDLL:
     L.fehip_solve_residuals_panels.argtypes = [C.POINTER(_vp), C.c_int32, C.POINTER(_vp), C.POINTER(C.c_int32),
                                                C.POINTER(C.c_int64), C.c_double, C.c_int64, C.POINTER(_i64p),
                                                C.POINTER(_u8p)]
+    L.fehip_solve_residuals_panels_host.argtypes = [C.POINTER(_vp), C.c_int32, C.POINTER(C.POINTER(_dp)),
+                                                    C.POINTER(C.c_int32), C.c_double, C.c_int64, C.POINTER(_i64p),
+                                                    C.POINTER(_u8p)]
     L.fehip_solve_coefficients.argtypes = [_vp, _dp, C.c_double, C.c_int64, C.POINTER(_dp), _i64p, _u8p]
     L.fehip_solve_coefficients_full.argtypes = [_vp, _dp, C.c_double, C.c_int64, C.POINTER(_dp), C.POINTER(_dp),
                                                 _i64p, _u8p]

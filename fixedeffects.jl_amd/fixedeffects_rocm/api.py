@@ -461,6 +461,25 @@ def solve_residuals_panels(solvers, dptrs, Ks, lds, tol: float = 1e-8, maxiter: 
     return [[int(v) for v in a] for a in its], [[bool(v) for v in a] for a in cvs]
 
 
+def solve_residuals_panels_host(solvers, columns, tol: float = 1e-8, maxiter: int = 10_000):
+    """Residualize many panels from host columns in one call
+    (fehip_solve_residuals_panels_host): columns[i] is a list of contiguous
+    float64 vectors of panel i, overwritten in place; the library stages them
+    in device memory.  Returns (iterations, converged) per panel."""
+    P = len(solvers)
+    cols = [[_as_inplace_vector(c, s.N) for c in cs] for s, cs in zip(solvers, columns)]
+    hs = (C.c_void_p * P)(*[s._h for s in solvers])
+    per = [(_dp * len(cs))(*[_ptr(c, C.c_double) for c in cs]) for cs in cols]
+    arr = (C.POINTER(_dp) * P)(*[C.cast(a, C.POINTER(_dp)) for a in per])
+    ks = (C.c_int32 * P)(*[len(cs) for cs in cols])
+    its = [np.zeros(len(cs), np.int64) for cs in cols]
+    cvs = [np.zeros(len(cs), np.uint8) for cs in cols]
+    ip = (C.POINTER(C.c_int64) * P)(*[_ptr(a, C.c_int64) for a in its])
+    cp = (C.POINTER(C.c_uint8) * P)(*[_ptr(a, C.c_uint8) for a in cvs])
+    check(lib.fehip_solve_residuals_panels_host(hs, P, arr, ks, float(tol), int(maxiter), ip, cp))
+    return [[int(v) for v in a] for a in its], [[bool(v) for v in a] for a in cvs]
+
+
 def solve_residuals(y, fes, w=None, *, method: str = "rocm", double_precision: Optional[bool] = None,
                     tol: Optional[float] = None, maxiter: int = 10000, precision: Optional[str] = None,
                     progress_bar: bool = True, algorithm: str = "lsmr"):

@@ -170,52 +170,31 @@ end
 
 # Many panels at once (multi-panel form of the multi-column solve_residuals!):
 # xss[i] holds the columns of panel i (host vectors), residualized in place;
-# the columns are staged in device memory (HIP runtime, same library libfehip
-# links) and all panels' LSMR iterations run concurrently on the GPU.
-const libhip = "libamdhip64.so.7"
-
+# the library stages them in device memory and runs all panels' LSMR
+# iterations concurrently on the GPU (fehip_solve_residuals_panels_host).
 function solve_residuals_panels!(xss::Vector, feMs::Vector{<:FixedEffectSolverROCm};
                                  tol::Real = 1e-8, maxiter::Integer = 10_000)
     length(xss) == length(feMs) || error("one column set per solver")
     P = length(feMs)
     Ks = Int32[length(xs) for xs in xss]
-    lds = Int64[feM.N for feM in feMs]
-    dbufs = [Ptr{Cvoid}(C_NULL) for _ in 1:P]
+    cols = [[convert(Vector{Float64}, x) for x in xs] for xs in xss]
     iters = [zeros(Int64, k) for k in Ks]
     convs = [zeros(UInt8, k) for k in Ks]
-    try
-        for i in 1:P
-            bytes = sizeof(Float64) * lds[i] * Ks[i]
-            ref = Ref{Ptr{Cvoid}}(C_NULL)
-            ccall((:hipMalloc, libhip), Cint, (Ref{Ptr{Cvoid}}, Csize_t), ref, bytes) == 0 || error("hipMalloc failed")
-            dbufs[i] = ref[]
-            for (c, x) in enumerate(xss[i])
-                col = convert(Vector{Float64}, x)
-                ccall((:hipMemcpy, libhip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Csize_t, Cint),
-                      dbufs[i] + sizeof(Float64) * lds[i] * (c - 1), col, sizeof(Float64) * lds[i], 1) == 0 ||
-                    error("hipMemcpy failed")
-            end
-        end
-        handles = [feM.handle for feM in feMs]
-        GC.@preserve iters convs begin
+    handles = [feM.handle for feM in feMs]
+    GC.@preserve cols iters convs begin
+        colptrs = [[pointer(c) for c in cs] for cs in cols]
+        GC.@preserve colptrs begin
+            cp_ = [pointer(v) for v in colptrs]
             ip = [pointer(a) for a in iters]
-            cp = [pointer(a) for a in convs]
-            check(ccall((:fehip_solve_residuals_panels, libfehip), Cint,
-                        (Ptr{Ptr{Cvoid}}, Int32, Ptr{Ptr{Cvoid}}, Ptr{Int32}, Ptr{Int64}, Float64, Int64,
+            vp = [pointer(a) for a in convs]
+            check(ccall((:fehip_solve_residuals_panels_host, libfehip), Cint,
+                        (Ptr{Ptr{Cvoid}}, Int32, Ptr{Ptr{Ptr{Float64}}}, Ptr{Int32}, Float64, Int64,
                          Ptr{Ptr{Int64}}, Ptr{Ptr{UInt8}}),
-                        handles, Int32(P), dbufs, Ks, lds, tol, maxiter, ip, cp))
+                        handles, Int32(P), cp_, Ks, tol, maxiter, ip, vp))
         end
-        for i in 1:P, (c, x) in enumerate(xss[i])
-            col = Vector{Float64}(undef, lds[i])
-            ccall((:hipMemcpy, libhip), Cint, (Ptr{Float64}, Ptr{Cvoid}, Csize_t, Cint),
-                  col, dbufs[i] + sizeof(Float64) * lds[i] * (c - 1), sizeof(Float64) * lds[i], 2) == 0 ||
-                error("hipMemcpy failed")
-            copyto!(x, col)
-        end
-    finally
-        for d in dbufs
-            d == C_NULL || ccall((:hipFree, libhip), Cint, (Ptr{Cvoid},), d)
-        end
+    end
+    for i in 1:P, (c, x) in enumerate(xss[i])
+        copyto!(x, cols[i][c])
     end
     return xss, iters, [Bool.(c) for c in convs]
 end

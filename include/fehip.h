@@ -111,6 +111,15 @@ int fehip_solve_residuals_panels(fehip_t *const *hs, int32_t npanels, double *co
                                  const int64_t *ldY, double tol, int64_t maxiter, int64_t *const *iters,
                                  uint8_t *const *converged);
 
+/* fehip_solve_residuals_panels on host columns: cols[i][c] is column c (N_i
+ * doubles, overwritten with its residuals) of panel i; the library stages
+ * each panel's columns in device memory of its own, one H2D and one D2H per
+ * column.  Same multi-column solve_residuals! over several fixed-effect
+ * structures (src/AbstractFixedEffectSolver.jl:74-97); K[i] <= 32. */
+int fehip_solve_residuals_panels_host(fehip_t *const *hs, int32_t npanels, double *const *const *cols,
+                                      const int32_t *K, double tol, int64_t maxiter, int64_t *const *iters,
+                                      uint8_t *const *converged);
+
 /* Replaces the inner solve_coefficients!(r, feM; tol, maxiter)
  * (src/AbstractFixedEffectSolver.jl:147-160) up to the collect at :158:
  * LSMR from x0 = 0 with maxiter, then x .*= scale.  coef[g] receives the

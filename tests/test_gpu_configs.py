@@ -214,3 +214,29 @@ def test_fp32_few_level_fe_against_float32_oracle(gpu, K, nlev):
         assert cvs[c] == co
         assert errs[c] <= 1e-4, (c, errs[c])
         assert abs(its[c] - io) <= max(3, io // 10), (c, its[c], io)
+
+
+def test_panels_from_host_columns(gpu):
+    # fehip_solve_residuals_panels_host stages host columns itself: bitwise
+    # the residuals of the device-column form
+    _say("case: panels from host columns")
+    panels = [synth.uneven_panel(920_000 + i, 1e4, 5e4) for i in range(4)]
+    solvers = [api.FixedEffectSolverROCm(_fes(p), p.weights, "fp64") for p in panels]
+    try:
+        bufs = []
+        for p in panels:
+            b = device.DeviceBuffer(p.N * p.K)
+            b.upload(np.asfortranarray(p.Y).ravel(order="F"))
+            bufs.append(b)
+        its_d, _ = api.solve_residuals_panels(solvers, [b.address for b in bufs], [p.K for p in panels],
+                                              [p.N for p in panels], tol=1e-8, maxiter=10_000)
+        cols = [[np.ascontiguousarray(p.Y[:, c]).copy() for c in range(p.K)] for p in panels]
+        its_h, cvs_h = api.solve_residuals_panels_host(solvers, cols, tol=1e-8, maxiter=10_000)
+        assert its_h == its_d and all(all(c) for c in cvs_h)
+        for p, b, cs in zip(panels, bufs, cols):
+            flat = b.download(p.N * p.K)
+            for c in range(p.K):
+                np.testing.assert_array_equal(cs[c], flat[c * p.N:(c + 1) * p.N])
+    finally:
+        for s in solvers:
+            s.close()
