@@ -23,6 +23,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3202,6 +3203,33 @@ static int group_keys(fehip_solver *h, int64_t N, int32_t ncols, const int64_t *
     return 0;
 }
 
+// The grouping without allocation (fehip_group_device_noalloc): its scratch
+// is one file-scope device array, each GPU holding its own copy for the life
+// of the process.  Host code never passes the array's name to a kernel (on
+// the host it is only a stand-in); the device address comes from
+// hipGetSymbolAddress, once per device with that device current.
+constexpr int64_t kGroupNoAllocMaxN = int64_t(1) << 20;
+constexpr size_t kGroupScratchBytes = size_t(60) << 20;  // carve_group_scratch(2^20) is 56.3 MiB
+static __device__ __align__(256) unsigned char scratch[kGroupScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this GPU's scratch array
+static int scratch_on_device(unsigned char **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(FEHIP_EARG, "device index beyond " + std::to_string(kMaxDevices));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        unsigned char *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
@@ -3470,6 +3498,29 @@ int fehip_group(int64_t N, int32_t ncols, const int64_t *const *keys, int64_t *r
     int rc = e == hipSuccess ? group_keys(h, N, ncols, keys, refs, ngroups) : fail(FEHIP_EHIP, hipGetErrorString(e));
     fehip_destroy(h);
     return rc;
+}
+
+int fehip_group_device_noalloc(int64_t N, int32_t ncols, const int64_t *const *dkeys, int64_t *drefs,
+                               int64_t *ngroups) {
+    g_err.clear();
+    if (N < 0 || ncols < 1 || !dkeys || !drefs || !ngroups) return fail(FEHIP_EARG, "bad arguments");
+    for (int c = 0; c < ncols; c++)
+        if (!dkeys[c]) return fail(FEHIP_EARG, "dkeys[c] is required for every column");
+    if (N > kGroupNoAllocMaxN)
+        return fail(FEHIP_EDIM, "N = " + std::to_string(N) + " exceeds the no-allocation grouping's maximum of " +
+                                    std::to_string(kGroupNoAllocMaxN) +
+                                    " observations (its device scratch has a fixed size); use fehip_group");
+    if (N == 0) {
+        *ngroups = 0;
+        return 0;
+    }
+    unsigned char *base = nullptr;
+    CHK(scratch_on_device(&base));
+    GroupScratch gs;
+    if (carve_group_scratch(nullptr, N, gs) > kGroupScratchBytes)
+        return fail(FEHIP_EHIP, "grouping scratch layout exceeds the device array");
+    carve_group_scratch(base, N, gs);
+    return group_keys_device(nullptr, gs, N, ncols, dkeys, drefs, ngroups);
 }
 
 int fehip_normalize(fehip_t *h, double *const *coef) {

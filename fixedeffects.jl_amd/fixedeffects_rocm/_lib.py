@@ -47,6 +47,7 @@ EXPORTS = (
     "fehip_solve_coefficients_full",
     "fehip_normalize",
     "fehip_group",
+    "fehip_group_device_noalloc",
     "fehip_set_algorithm",
     "fehip_plan_info",
     "fehip_get_csr",
@@ -102,6 +103,7 @@ def _load() -> C.CDLL:
                                                 _i64p, _u8p]
     L.fehip_normalize.argtypes = [_vp, C.POINTER(_dp)]
     L.fehip_group.argtypes = [C.c_int64, C.c_int32, C.POINTER(_i64p), _i64p, _i64p]
+    L.fehip_group_device_noalloc.argtypes = [C.c_int64, C.c_int32, C.POINTER(_vp), _vp, _i64p]
     L.fehip_set_algorithm.argtypes = [_vp, C.c_int32]
     L.fehip_plan_info.argtypes = [_vp, C.c_int32, C.c_int32, _i64p]
     L.fehip_get_csr.argtypes = [_vp, C.c_int32, _i32p, _i32p]

@@ -173,6 +173,20 @@ int fehip_set_algorithm(fehip_t *h, int32_t algorithm);
 int fehip_group(int64_t N, int32_t ncols, const int64_t *const *keys,
                 int64_t *refs, int64_t *ngroups);
 
+/* No counterpart in the reference: fehip_group for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same numbering as fehip_group, on
+ * device-resident data and without any allocation: its scratch is a fixed
+ * device array of the library (one per GPU, held until the process ends).
+ *   dkeys[c] : N device int64 keys of column c (host array of device pointers)
+ *   drefs    : N device int64, out: 1-based refs
+ *   ngroups  : host int64, out
+ * N <= 2^20 (FEHIP_EDIM beyond).  Runs on the null stream and returns when
+ * drefs is written.  Calls on one GPU share the scratch: calls that overlap
+ * on one GPU (other threads) race. */
+int fehip_group_device_noalloc(int64_t N, int32_t ncols, const int64_t *const *dkeys, int64_t *drefs,
+                               int64_t *ngroups);
+
 /* normalize!(fecoefs, fes) (src/FixedEffect.jl:106-111) of G host arrays in
  * place, for the handle's fixed effects. */
 int fehip_normalize(fehip_t *h, double *const *coef);
