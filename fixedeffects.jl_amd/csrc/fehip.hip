@@ -823,6 +823,7 @@ struct PieceForm {
     uint32_t *wprefix = nullptr;     // heads before each word
     int32_t *piece_s = nullptr;      // piece -> FE s level
     int32_t *pt_perm = nullptr;      // pieces ordered stably by FE t level
+    int32_t *pt_lev = nullptr;       // the FE t level of each piece in that order
     FeInfo fs, ft;                   // head bits, word prefixes and level maps of the pieces by FE s / FE t level
     int64_t *tstart = nullptr;       // tiles over pieces: FE s's, then FE t's, each ending with np
     int64_t ts0_t = 0;               // FE t's first entry in tstart
@@ -1182,7 +1183,7 @@ static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<F
     pc.np = np;
     pc.heads = heads;
     pc.wprefix = wp;
-    int32_t *lev_t = nullptr, *skeys = nullptr, *off_s = nullptr, *off_t = nullptr;
+    int32_t *lev_t = nullptr, *off_s = nullptr, *off_t = nullptr;
     CHK(dalloc(h, &pc.piece_s, np));
     CHK(dalloc(h, &lev_t, np));
     hipLaunchKernelGGL(k_piece_levels, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
@@ -1194,25 +1195,26 @@ static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<F
     CHK(dalloc(h, &off_s, ns + 1));
     CHK(csr_products(h, pc.piece_s, np, ns, off_s, &pc.fs));
     // by FE t level: a stable sort of the pieces
-    CHK(dalloc(h, &skeys, np));
+    CHK(dalloc(h, &pc.pt_lev, np));
     CHK(dalloc(h, &pc.pt_perm, np));
-    CHK(radix_sort(h, lev_t, np, nt, skeys, pc.pt_perm));
+    CHK(radix_sort(h, lev_t, np, nt, pc.pt_lev, pc.pt_perm));
     CHK(dalloc(h, &off_t, nt + 1));
-    CHK(csr_products(h, skeys, np, nt, off_t, &pc.ft));
+    CHK(csr_products(h, pc.pt_lev, np, nt, off_t, &pc.ft));
     std::vector<int32_t> hs(ns + 1), ht(nt + 1);
     HIPCHK(hipMemcpyAsync(hs.data(), off_s, sizeof(int32_t) * (ns + 1), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(ht.data(), off_t, sizeof(int32_t) * (nt + 1), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     dfree(h, lev_t);
-    dfree(h, skeys);
     dfree(h, off_s);
     dfree(h, off_t);
     free_sort_scratch(h);
-    // tiles over the pieces, cut at level heads (a quarter of the CSR pass's
-    // target tile count per FE: the piece launch is short, and longer tiles
-    // keep each wave's loads in flight; the autotuner's candidates then span
-    // about 256 to 1800 pieces per tile on the benchmark panel); fix-up items
-    const int64_t tp = tile_size(np, std::max<int64_t>(1, tiles / 8));
+    // tiles over the pieces, cut at level heads: half the CSR pass's target
+    // tile count per FE, multiples of 64 pieces (the launch is short and
+    // latency-bound: many short tiles keep more waves' loads in flight; the
+    // autotuner's candidates span 64 to about 450 pieces per tile on the
+    // benchmark panel); fix-up items
+    const int64_t tgt = std::max<int64_t>(1, tiles / 2);
+    const int64_t tp = std::max<int64_t>(64, ((np + tgt - 1) / tgt + 63) / 64 * 64);
     std::vector<int64_t> tsh;
     std::vector<FixItem> it1(dense_items), it2;
     int32_t t0 = 0;
@@ -1255,7 +1257,7 @@ static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<F
         t0 += (int32_t)ntl;
     }
     pc.total_tiles = t0;
-    pc.blocks = (unsigned)std::min<int64_t>(grid_for(t0, kWavesPerBlock), 2048);
+    pc.blocks = (unsigned)std::min<int64_t>(grid_for(t0, kWavesPerBlock), 4096);
     pc.fix1 = (unsigned)std::min<int64_t>(grid_for((int64_t)it1.size(), kWavesPerBlock), 1024);
     pc.fix2 = (unsigned)std::min<int64_t>(grid_for((int64_t)it2.size(), kWavesPerBlock), 1024);
     CHK(dalloc(h, &pc.tstart, tsh.size()));
@@ -1277,7 +1279,7 @@ static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<F
 }
 
 static void destroy_pieces(fehip_solver *h, PieceForm &pc) {
-    void *ptrs[] = {pc.heads, pc.wprefix, pc.piece_s, pc.pt_perm, pc.fs.heads, pc.fs.wprefix, pc.fs.seg_level,
+    void *ptrs[] = {pc.heads, pc.wprefix, pc.piece_s, pc.pt_perm, pc.pt_lev, pc.fs.heads, pc.fs.wprefix, pc.fs.seg_level,
                     pc.ft.heads, pc.ft.wprefix, pc.ft.seg_level, pc.tstart, pc.tileA, pc.tileB, pc.part,
                     pc.items1, pc.items2, pc.psum};
     for (void *q : ptrs) dfree(h, q);
@@ -2009,6 +2011,7 @@ struct Engine {
             e.heads = fi.heads;
             e.wprefix = fi.wprefix;
             e.seg_level = fi.seg_level;
+            e.plev = q == 0 ? pc.piece_s : pc.pt_lev;  // pieces by FE s level are in piece order
             e.nsegs = q == 0 ? pc.nsegs_s : pc.nsegs_t;
             e.lscale = (const T *)h->fe[g].scale_T[pidx<T>()];
             e.coef_off = h->off[g];
@@ -2056,7 +2059,7 @@ struct Engine {
         const DtxSegParams<T> p = piece_seg_params(h, P, ops, coef, early_exit);
         const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 1);
         const int nfix = (int)P->pc.fix1;
-        hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true>), dim3(P->pc.fix1 + P->pc.blocks), dim3(kBlock), 0,
+        hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true, true>), dim3(P->pc.fix1 + P->pc.blocks), dim3(kBlock), 0,
                            h->stream, p, f, nfix);
     }
 

@@ -442,6 +442,7 @@ struct SegFE {
     const uint64_t *heads;    // segment head bits, 1 beyond N
     const uint32_t *wprefix;  // heads before each 64-position word
     const int32_t *seg_level; // ordinal of non-empty segment -> level
+    const int32_t *plev;      // PLEV form: the level of each position (replaces the ordinal and seg_level)
     const T *lscale;          // RAW form: per-level scale applied to each level's sum (cache not read)
     const int64_t *tstart;    // tile i covers positions [tstart[i], tstart[i+1])
     int64_t coef_off;
@@ -540,9 +541,12 @@ __device__ __forceinline__ int32_t seg_level_of(uint32_t wp, const int32_t *seg_
 // the segment sums are of the stored vector as it is, and each level's sum
 // takes the FE's scale (lscale; crossing segments take it in the fix-up), so
 // the CSR-ordered cache is not read.
+// PLEV: each position's level is read from plev with its permutation entry
+// (one memory round trip less per step than ordinal -> seg_level; used by the
+// piece form, whose position counts are small).
 // the body of one launch block (bid of nblk): launched alone (k_dtx_seg) or
 // as the later blocks of k_dtx_seg_fix
-template <typename T, typename Acc, int K, bool RAW>
+template <typename T, typename Acc, int K, bool RAW, bool PLEV = false>
 __device__ __forceinline__ void dtx_seg_body(const DtxSegParams<T> &p, int bid, int nblk) {
     constexpr int P = kWave / K;
     constexpr int U = 4;  // wave steps whose loads are issued together
@@ -576,7 +580,7 @@ __device__ __forceinline__ void dtx_seg_body(const DtxSegParams<T> &p, int bid, 
         // per lane, one loop step ahead: the observation each position reads
         // and the head-bit words, so that a step's u gathers, its cache loads
         // and the v of the levels its positions belong to go out together
-        int32_t in[U];
+        int32_t in[U], lvn[U];
         uint64_t hwn[U], hnn[U];
         uint32_t wpn[U];
         auto meta = [&](int64_t b0) {
@@ -586,12 +590,13 @@ __device__ __forceinline__ void dtx_seg_body(const DtxSegParams<T> &p, int bid, 
                 const int64_t pos = base + psub;
                 const int64_t pc = (pos >= start && pos < end) ? pos : start;
                 in[q] = __builtin_nontemporal_load(fe.perm + pc);
+                if constexpr (PLEV) lvn[q] = fe.plev[pc];
                 hwn[q] = hnn[q] = 0;
                 wpn[q] = 0;
                 if (base < end) {  // wave-uniform
                     hwn[q] = fe.heads[base >> 6];
                     hnn[q] = fe.heads[(base >> 6) + 1];
-                    wpn[q] = fe.wprefix[base >> 6];
+                    if constexpr (!PLEV) wpn[q] = fe.wprefix[base >> 6];
                 }
             }
         };
@@ -605,12 +610,16 @@ __device__ __forceinline__ void dtx_seg_body(const DtxSegParams<T> &p, int bid, 
                 iq[q] = in[q];
                 hw[q] = hwn[q];
                 hn[q] = hnn[q];
-                // ordinal of the segment holding this lane's position,
-                // clamped for lanes past the tile's end
-                const int64_t pos = base0 + q * P + psub;
-                uint32_t ord = wpn[q] + (uint32_t)__popcll(hw[q] & lanes_le((int)(pos & 63))) - 1u;
-                ord = ord < (uint32_t)fe.nsegs ? ord : (uint32_t)fe.nsegs - 1u;
-                lev[q] = fe.seg_level[ord];
+                if constexpr (PLEV) {
+                    lev[q] = lvn[q];
+                } else {
+                    // ordinal of the segment holding this lane's position,
+                    // clamped for lanes past the tile's end
+                    const int64_t pos = base0 + q * P + psub;
+                    uint32_t ord = wpn[q] + (uint32_t)__popcll(hw[q] & lanes_le((int)(pos & 63))) - 1u;
+                    ord = ord < (uint32_t)fe.nsegs ? ord : (uint32_t)fe.nsegs - 1u;
+                    lev[q] = fe.seg_level[ord];
+                }
             }
             // unconditional loads (positions past the tile read its first
             // position): every load of the step is in flight together
@@ -1429,11 +1438,11 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
 // CSR tiles.  The fix-up's short dependent loads then overlap the CSR
 // pass's stream instead of running after it.  The two parts touch disjoint
 // levels of v and leave their norm partials in separate arrays.
-template <typename T, typename Acc, int K, bool RAW>
+template <typename T, typename Acc, int K, bool RAW, bool PLEV = false>
 __global__ __launch_bounds__(kBlock) void k_dtx_seg_fix(DtxSegParams<T> p, FixParams<T> f, int nfix) {
     if (p.all_done && *p.all_done) return;
     if ((int)blockIdx.x < nfix) dtx_fixup_body<T, K>(f, blockIdx.x, nfix);
-    else dtx_seg_body<T, Acc, K, RAW>(p, (int)blockIdx.x - nfix, (int)gridDim.x - nfix);
+    else dtx_seg_body<T, Acc, K, RAW, PLEV>(p, (int)blockIdx.x - nfix, (int)gridDim.x - nfix);
 }
 
 // ---------------------------------------------------------------------------

@@ -82,8 +82,12 @@ def test_interacted_k16_workload(gpu, prec):
     _say(f"  tol 1e-8 per-column rel err vs the FP64 oracle (tol 1e-8): {' '.join(f'{e:.1e}' for e in d_or)}")
     _say(f"  tol 1e-8 distance to converged: gpu {max(d_ex):.2e} oracle {max(d_oe):.2e}; "
          f"iterations gpu {its8} oracle {[c[1] for c in loose]}")
+    same = [c for c in range(p.K) if its8[c] == loose[c][1]]
+    _say(f"  columns stopping at the oracle's iteration: {len(same)} of {p.K}")
     for c in range(p.K):
         assert d_ex[c] <= 3 * d_oe[c] + 1e-12, (c, d_ex[c], d_oe[c])
+        if prec == "fp64" and c in same:
+            assert d_or[c] <= RES_RTOL, (c, d_or[c])
 
 
 # -- many uneven panels ---------------------------------------------------------------
@@ -105,13 +109,17 @@ def _panels():
     return panels, ref
 
 
-def _check_panel_columns(label, cols, conv, loose):
+def _check_panel_columns(label, cols, conv, loose, its=None):
+    """its: the GPU's iteration counts when its start is the reference's:
+    columns stopping at the oracle's iteration then agree with it to 1e-8."""
     worst_or, worst_ex = 0.0, 0.0
     for c, r in enumerate(cols):
         d_ex, d_oe = _rel(r, conv[c]), _rel(loose[c][0], conv[c])
         worst_or = max(worst_or, _rel(r, loose[c][0]))
         worst_ex = max(worst_ex, d_ex)
         assert d_ex <= 3 * d_oe + 1e-12, (label, c, d_ex, d_oe)
+        if its is not None and its[c] == loose[c][1]:
+            assert _rel(r, loose[c][0]) <= RES_RTOL, (label, c, _rel(r, loose[c][0]))
     return worst_or, worst_ex
 
 
@@ -134,7 +142,7 @@ def test_uneven_panels_against_oracle(gpu):
             assert all(cvs[i])
             flat = b.download(p.N * p.K)
             cols = [flat[c * p.N:(c + 1) * p.N] for c in range(p.K)]
-            w_or, w_ex = _check_panel_columns(f"panel {i}", cols, conv, loose)
+            w_or, w_ex = _check_panel_columns(f"panel {i}", cols, conv, loose, its[i])
             lrn = [np.ascontiguousarray(p.Y[:, c]).copy() for c in range(p.K)]
             it_l, cv_l = warmstart.solve_residuals_learned(s, lrn, net, tol=1e-8, maxiter=10_000)
             assert all(cv_l)
@@ -240,3 +248,32 @@ def test_panels_from_host_columns(gpu):
     finally:
         for s in solvers:
             s.close()
+
+
+@pytest.mark.parametrize("shape", ["worker_firm", "uniform"])
+def test_piece_form_taken_where_it_applies(gpu, shape):
+    # the worker/firm panel (runs of one worker at one firm) takes the piece
+    # form of D'u; uniform refs (almost every (s, t) pair distinct) keep the
+    # CSR pass; both converge to the oracle's residual
+    _say(f"case: piece form on the {shape} panel")
+    if shape == "worker_firm":
+        p = synth.config("worker_firm_1e7", scale=0.01)
+    else:
+        rng = np.random.default_rng(12)
+        N = 60_000
+        refs = [rng.integers(1, 6001, N).astype(np.int64), rng.integers(1, 2001, N).astype(np.int64),
+                rng.integers(1, 21, N).astype(np.int64)]
+        p = synth.Panel(refs, [6000, 2000, 20], [None] * 3, rng.lognormal(0, 0.5, N),
+                        np.asfortranarray(rng.standard_normal((N, 1))))
+    o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=NTHREADS)
+    ro = o.solve_residuals(p.Y[:, 0].copy(), tol=1e-12, maxiter=100_000)[0]
+    with api.FixedEffectSolverROCm(_fes(p), p.weights, "fp64") as s:
+        info = s.plan_info(1)
+        _say(f"  N={p.N} pieces {info['pieces']}")
+        if shape == "worker_firm":
+            assert 0 < info["pieces"] <= p.N // 2
+        else:
+            assert info["pieces"] == 0
+        r, it, cv = s.solve_residuals(np.ascontiguousarray(p.Y[:, 0]).copy(), tol=1e-12, maxiter=100_000)
+    _say(f"  iters {it} rel err {_rel(r, ro):.2e}")
+    assert cv and _rel(r, ro) <= RES_RTOL
