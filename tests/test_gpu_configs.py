@@ -17,8 +17,9 @@ The solve under test is the inner solve_residuals! (src/AbstractFixedEffectSolve
 (src/utils/lsmr.jl:43).  Bars (per column):
 * tol 1e-12: ||r_gpu - r_oracle|| / ||r_oracle|| <= 1e-8;
 * tol 1e-8: the GPU residual is at least as close to the converged residual
-  as the oracle's own tol-1e-8 residual, within a factor 3, and the relative
-  difference to the oracle's tol-1e-8 residual is printed.
+  as the oracle's own tol-1e-8 residual, within a factor 3; in FP64 from the
+  reference start it lies within a quarter of that tolerance-level error of
+  the oracle's residual; the relative difference is printed.
 """
 import functools
 import sys
@@ -86,8 +87,8 @@ def test_interacted_k16_workload(gpu, prec):
     _say(f"  columns stopping at the oracle's iteration: {len(same)} of {p.K}")
     for c in range(p.K):
         assert d_ex[c] <= 3 * d_oe[c] + 1e-12, (c, d_ex[c], d_oe[c])
-        if prec == "fp64" and c in same:
-            assert d_or[c] <= RES_RTOL, (c, d_or[c])
+        if prec == "fp64":
+            assert d_or[c] <= 0.25 * d_oe[c] + 1e-12, (c, d_or[c], d_oe[c])
 
 
 # -- many uneven panels ---------------------------------------------------------------
@@ -110,16 +111,17 @@ def _panels():
 
 
 def _check_panel_columns(label, cols, conv, loose, its=None):
-    """its: the GPU's iteration counts when its start is the reference's:
-    columns stopping at the oracle's iteration then agree with it to 1e-8."""
+    """its: given when the GPU's start is the reference's: every column then
+    lies within a quarter of the oracle's tolerance-level error (its distance
+    to the converged residual) of the oracle's own residual."""
     worst_or, worst_ex = 0.0, 0.0
     for c, r in enumerate(cols):
         d_ex, d_oe = _rel(r, conv[c]), _rel(loose[c][0], conv[c])
         worst_or = max(worst_or, _rel(r, loose[c][0]))
         worst_ex = max(worst_ex, d_ex)
         assert d_ex <= 3 * d_oe + 1e-12, (label, c, d_ex, d_oe)
-        if its is not None and its[c] == loose[c][1]:
-            assert _rel(r, loose[c][0]) <= RES_RTOL, (label, c, _rel(r, loose[c][0]))
+        if its is not None:
+            assert _rel(r, loose[c][0]) <= 0.25 * d_oe + 1e-12, (label, c, _rel(r, loose[c][0]), d_oe)
     return worst_or, worst_ex
 
 
@@ -154,6 +156,12 @@ def test_uneven_panels_against_oracle(gpu):
                  f"vs converged {w_ex:.1e} (learned {l_ex:.1e})")
         _say(f"  LSMR iterations in total: reference start {tot_ref}, learned start {tot_lrn} "
              f"({100.0 * (tot_ref - tot_lrn) / tot_ref:+.1f}% saved)")
+        # what the committed network gains on these held-out panels (seeds
+        # outside its training range): measured 2589 against 2600 iterations,
+        # 0.4% (0.9% on the 48-panel evaluation up to N = 2e6).  A start
+        # from per-level features cannot reach the graph-global error modes
+        # that set LSMR's iteration count, so the gain stays small.
+        assert tot_lrn <= tot_ref
     finally:
         for s in solvers:
             s.close()

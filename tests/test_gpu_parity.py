@@ -9,8 +9,9 @@ Bars:
   iterates agree to 1e-12; converged (tol = 1e-12) ||r_gpu - r_oracle|| /
   ||r_oracle|| <= 1e-8; at the reference's tol = 1e-8 the iteration counts
   agree within +-2, the GPU residual is at least as close to the exact
-  least-squares residual as the oracle's own (within a factor 3), and where
-  both stop at the same iteration the residuals agree to 1e-8.  LSMR in
+  least-squares residual as the oracle's own (within a factor 3), and in FP64
+  it lies within a quarter of the oracle's own tolerance-level error of the
+  oracle's residual (measured: at most a tenth).  LSMR in
   finite precision loses orthogonality, and a reordering of the oracle's own
   sums already moves its mid-run scalars by up to 1e-2 and its stopping
   iteration by one, so tol-level residual differences are not parity defects.
@@ -190,9 +191,11 @@ def _check_solve(refs, n, z, w, Y, prec="fp64", tol=1e-8, iter_slack=2):
         assert cg == co
         assert abs(ig - io) <= iter_slack, (ig, io)
         assert _rel(rg, rt) <= 3 * _rel(ro, rt) + 1e-12, (_rel(rg, rt), _rel(ro, rt))
-        if ig == io and prec == "fp64":
-            # stopped at the oracle's iteration: the same answer to 1e-8
-            assert _rel(rg, ro) <= RES_RTOL, (c, _rel(rg, ro))
+        if prec == "fp64":
+            # the same answer up to a fraction of the tolerance-level error:
+            # measured at most ~0.1 of the oracle's distance to the converged
+            # residual, also where the stop is one iteration apart
+            assert _rel(rg, ro) <= 0.25 * _rel(ro, rt) + 1e-12, (c, _rel(rg, ro), _rel(ro, rt))
     return worst, [g[1] for g in got], [w_[1] for w_ in want]
 
 
