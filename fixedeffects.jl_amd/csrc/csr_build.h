@@ -271,6 +271,59 @@ __global__ void k_piece_levels(const int32_t *__restrict__ ref_s, const int32_t 
     piece_t[ord] = ref_t[j];
 }
 
+// ---- internal numbering of the FE with most levels ------------------------
+//
+// Runs of one (s, t) pair in the order sorted by s, then t: run_s, run_t and
+// the first position of each (ordinal from the head words, as pieces).
+__global__ void k_run_starts(const int32_t *__restrict__ ss, const int32_t *__restrict__ tt, int64_t n,
+                             const uint64_t *__restrict__ heads, const uint32_t *__restrict__ wprefix,
+                             int32_t *__restrict__ run_s, int32_t *__restrict__ run_t,
+                             int32_t *__restrict__ run_start) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t word = heads[j >> 6];
+    const int b = (int)(j & 63);
+    if (!((word >> b) & 1)) return;
+    const uint32_t ord = wprefix[j >> 6] + (uint32_t)__popcll(word & lanes_le(b)) - 1u;
+    run_s[ord] = ss[j];
+    run_t[ord] = tt[j];
+    run_start[ord] = (int32_t)j;
+}
+
+__global__ void k_set_i32(int32_t *p, int32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// main[k] = the t level of the longest run of s level k (runs of a level are
+// in ascending t order, so ties go to the smaller t); nt when the level has
+// no observation
+__global__ void k_main_level(const int32_t *__restrict__ roff, const int32_t *__restrict__ run_start,
+                             const int32_t *__restrict__ run_t, int64_t ns, int32_t nt, int32_t *__restrict__ main) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    int32_t best = nt, blen = 0;
+    for (int32_t r = roff[k]; r < roff[k + 1]; r++) {
+        const int32_t len = run_start[r + 1] - run_start[r];
+        if (len > blen) {
+            blen = len;
+            best = run_t[r];
+        }
+    }
+    main[k] = best;
+}
+
+// inv[perm[j]] = j
+__global__ void k_invert_perm(const int32_t *__restrict__ perm, int64_t n, int32_t *__restrict__ inv) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) inv[perm[j]] = (int32_t)j;
+}
+
+// refs[i] = map[refs[i]] (in place)
+__global__ void k_relabel(int32_t *__restrict__ refs, int64_t n, const int32_t *__restrict__ map) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) refs[i] = map[refs[i]];
+}
+
 // ---- reference refs -> device int32 (0-based) with range check -----------
 
 template <typename R>

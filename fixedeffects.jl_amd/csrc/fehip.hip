@@ -102,6 +102,13 @@ struct fehip_solver {
     bool has64 = true, has32 = false;  // storage precisions present
     double mixed_tol = 1e-6;           // FEHIP_MIXED: tolerance of the FP32-stream phase
     int s = 0;  // FE defining the internal order
+    // FE s's levels in an internal numbering (renumber_s): every device array
+    // indexed by an FE s level uses it; coefficient blocks, scales and the
+    // CSR cross the C-ABI in the caller's numbering.  nullptr = identity.
+    int32_t *s_int_of = nullptr;  // caller level -> internal level
+    int32_t *s_ext_of = nullptr;  // internal level -> caller level
+    std::vector<int32_t> h_s_int_of;
+    double *s_tmp = nullptr;      // n_s doubles: a permuted FE s block
     std::vector<int64_t> n, off;
     hipStream_t stream = nullptr;
     std::vector<FeInfo> fe;
@@ -316,6 +323,97 @@ static int csr_products(fehip_solver *h, const int32_t *skeys, int64_t N, int64_
     HIPCHK(hipStreamSynchronize(h->stream));
     dfree(h, cnt);
     dfree(h, sums);
+    return 0;
+}
+
+
+// FE s's internal numbering: its levels ordered by their main level of FE t
+// (the t level holding the longest run of the s level's observations, ties
+// to the smaller t; levels without observations last), then by caller
+// number.  Workers sharing a main firm are then adjacent in the internal
+// order, so a firm's pieces lie close together in the piece order and the
+// piece launch's gathers of their sums share cache lines.  ref_s (the
+// caller's order, 0-based) is relabeled in place.
+static int renumber_s(fehip_solver *h, int32_t *ref_s, const int32_t *ref_t, int t) {
+    const int64_t N = h->N, ns = h->n[h->s], nt = h->n[t];
+    int32_t *sk = nullptr, *p2 = nullptr, *k1 = nullptr, *ss = nullptr, *perm = nullptr, *tt = nullptr;
+    CHK(dalloc(h, &sk, N));
+    CHK(dalloc(h, &p2, N));
+    CHK(dalloc(h, &k1, N));
+    CHK(dalloc(h, &ss, N));
+    CHK(dalloc(h, &perm, N));
+    CHK(dalloc(h, &tt, N));
+    // observations sorted by (s, t)
+    CHK(radix_sort(h, ref_t, N, nt, sk, p2));
+    hipLaunchKernelGGL(k_gather_perm<int32_t>, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, ref_s, p2, N,
+                       k1);
+    CHK(radix_sort(h, k1, N, ns, ss, perm, p2));
+    hipLaunchKernelGGL(k_gather_perm<int32_t>, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, ref_t, perm, N,
+                       tt);
+    HIPCHK(hipGetLastError());
+    dfree(h, sk);
+    dfree(h, p2);
+    dfree(h, k1);
+    dfree(h, perm);
+    // runs of one (s, t) pair
+    const int64_t nwords = N / 64 + 2;
+    uint64_t *heads = nullptr;
+    uint32_t *cnt = nullptr, *wp = nullptr, *sums = nullptr;
+    CHK(dalloc(h, &heads, nwords));
+    CHK(dalloc(h, &cnt, nwords + 1));
+    CHK(dalloc(h, &wp, nwords + 1));
+    CHK(dalloc(h, &sums, (nwords + kSortTile - 1) / kSortTile + 2));
+    hipLaunchKernelGGL(k_piece_heads, dim3(grid_for(nwords, kWavesPerBlock)), dim3(kBlock), 0, h->stream,
+                       (const int32_t *)ss, (const int32_t *)tt, N, N + 1, heads, nwords, cnt);
+    HIPCHK(hipGetLastError());
+    CHK(exclusive_scan_u32(h, cnt, nwords, wp, true, sums));
+    uint32_t nruns = 0;
+    HIPCHK(hipMemcpyAsync(&nruns, wp + nwords, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    int32_t *run_s = nullptr, *run_t = nullptr, *run_start = nullptr, *roff = nullptr, *mainl = nullptr,
+            *skeys = nullptr;
+    CHK(dalloc(h, &run_s, std::max<int64_t>(nruns, 1)));
+    CHK(dalloc(h, &run_t, std::max<int64_t>(nruns, 1)));
+    CHK(dalloc(h, &run_start, (int64_t)nruns + 1));
+    hipLaunchKernelGGL(k_run_starts, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, (const int32_t *)ss,
+                       (const int32_t *)tt, N, (const uint64_t *)heads, (const uint32_t *)wp, run_s, run_t,
+                       run_start);
+    hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(64), 0, h->stream, run_start + nruns, (int32_t)N);
+    HIPCHK(hipGetLastError());
+    dfree(h, ss);
+    dfree(h, tt);
+    dfree(h, heads);
+    dfree(h, cnt);
+    dfree(h, wp);
+    dfree(h, sums);
+    // each s level's main t level, then the levels stably sorted by it
+    CHK(dalloc(h, &roff, ns + 1));
+    CHK(dalloc(h, &mainl, ns));
+    CHK(dalloc(h, &skeys, ns));
+    CHK(dalloc(h, &h->s_ext_of, ns));
+    CHK(dalloc(h, &h->s_int_of, ns));
+    CHK(dalloc(h, &h->s_tmp, ns));
+    hipLaunchKernelGGL(k_offsets_from_sorted, dim3(grid_for((int64_t)nruns + 1, kBlock)), dim3(kBlock), 0, h->stream,
+                       (const int32_t *)run_s, (int64_t)nruns, (int32_t)ns, roff);
+    hipLaunchKernelGGL(k_main_level, dim3(grid_for(ns, kBlock)), dim3(kBlock), 0, h->stream, (const int32_t *)roff,
+                       (const int32_t *)run_start, (const int32_t *)run_t, ns, (int32_t)nt, mainl);
+    HIPCHK(hipGetLastError());
+    CHK(radix_sort(h, mainl, ns, nt + 1, skeys, h->s_ext_of));
+    hipLaunchKernelGGL(k_invert_perm, dim3(grid_for(ns, kBlock)), dim3(kBlock), 0, h->stream,
+                       (const int32_t *)h->s_ext_of, ns, h->s_int_of);
+    hipLaunchKernelGGL(k_relabel, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream, ref_s, N,
+                       (const int32_t *)h->s_int_of);
+    HIPCHK(hipGetLastError());
+    h->h_s_int_of.resize(ns);
+    HIPCHK(hipMemcpyAsync(h->h_s_int_of.data(), h->s_int_of, sizeof(int32_t) * ns, hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, run_s);
+    dfree(h, run_t);
+    dfree(h, run_start);
+    dfree(h, roff);
+    dfree(h, mainl);
+    dfree(h, skeys);
     return 0;
 }
 
@@ -658,6 +756,8 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
     for (int g = 0; g < G; g++)
         if (g != h->s && h->n[g] > dense_limit(precision) && (t_sec < 0 || h->n[g] > h->n[t_sec])) t_sec = g;
     h->t_sec = t_sec;
+    // FE s in its internal numbering from here on (FEHIP_RENUMBER=0 keeps the caller's)
+    if (t_sec >= 0 && env_int("FEHIP_RENUMBER", 1, 0, 1)) CHK(renumber_s(h, ref_orig[h->s], ref_orig[t_sec], t_sec));
     {
         FeInfo &f = h->fe[h->s];
         f.n = h->n[h->s];
@@ -1456,6 +1556,34 @@ __global__ void k_scale_coef(const T *__restrict__ x, const T *__restrict__ scal
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) out[e] = (double)(x[e] * scale[e]);
 }
+
+// dst[i] = src[map[i]]: a block between two numberings
+template <typename V>
+__global__ void k_gather_map(const V *__restrict__ src, const int32_t *__restrict__ map, int64_t n, V *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[map[i]];
+}
+// rows of W floats: dst[i*W + j] = src[map[i]*W + j]
+__global__ void k_gather_rows(const float *__restrict__ src, const int32_t *__restrict__ map, int64_t n, int W,
+                             float *__restrict__ dst) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n * W) dst[e] = src[(int64_t)map[e / W] * W + e % W];
+}
+
+// The FE s block of a device coefficient vector (p doubles), in place, from
+// the internal numbering to the caller's (to_caller) or back.
+static int s_block_permute(fehip_solver *h, double *dc, bool to_caller) {
+    if (!h->s_int_of) return 0;
+    double *blk = dc + h->off[h->s];
+    const int64_t ns = h->n[h->s];
+    hipLaunchKernelGGL(k_gather_map<double>, dim3(grid_for(ns, kBlock)), dim3(kBlock), 0, h->stream,
+                       (const double *)blk, (const int32_t *)(to_caller ? h->s_int_of : h->s_ext_of), ns, h->s_tmp);
+    HIPCHK(hipMemcpyAsync(blk, h->s_tmp, sizeof(double) * ns, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+static int s_block_to_caller(fehip_solver *h, double *dc) { return s_block_permute(h, dc, true); }
+static int s_block_to_internal(fehip_solver *h, double *dc) { return s_block_permute(h, dc, false); }
 
 // ---------------------------------------------------------------------------
 // warm-start support: per-level features and initial iterates
@@ -2719,6 +2847,7 @@ struct Engine {
                                    (const double *)h->ws_x0 + (size_t)c * h->p, h->p, (T *)P->x, K, c);
             } else if (coef0 && coef0[c]) {
                 HIPCHK(hipMemcpyAsync(h->ws_coef, coef0[c], sizeof(double) * h->p, hipMemcpyHostToDevice, h->stream));
+                CHK(s_block_to_internal(h, h->ws_coef));
                 hipLaunchKernelGGL((k_x0_from_coef<T>), dim3(gp), dim3(kBlock), 0, h->stream,
                                    (const double *)h->ws_coef, ws_params(h), h->p, (T *)P->x, K, c);
             }
@@ -2750,6 +2879,7 @@ struct Engine {
         if (coef0) {
             CHK(ensure_ws(h));
             HIPCHK(hipMemcpyAsync(h->ws_coef, coef0, sizeof(double) * h->p, hipMemcpyHostToDevice, h->stream));
+            CHK(s_block_to_internal(h, h->ws_coef));
             hipLaunchKernelGGL((k_x0_from_coef<T>), dim3(grid_for(h->p, kBlock)), dim3(kBlock), 0, h->stream,
                                (const double *)h->ws_coef, ws_params(h), h->p, (T *)P->x, K, 0);
         }
@@ -3437,6 +3567,7 @@ int fehip_solve_coefficients(fehip_t *h, const double *y, double tol, int64_t ma
         if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
     }
     if (rc == 0) rc = solve_coefficients_dev(h, dy, tol, maxiter, dc, iters, converged);
+    if (rc == 0) rc = s_block_to_caller(h, dc);
     for (int g = 0; g < h->G && rc == 0; g++) {
         hipError_t e = hipMemcpy(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
@@ -3462,12 +3593,6 @@ int fehip_solve_coefficients_full(fehip_t *h, const double *y, double tol, int64
     }
     if (rc == 0) rc = solve_coefficients_dev(h, dy, tol, maxiter, dc, iters, converged);
     if (rc == 0) rc = normalize_dev(h, dc);
-    for (int g = 0; coef && g < h->G && rc == 0; g++) {
-        if (!coef[g]) continue;
-        hipError_t e = hipMemcpyAsync(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost,
-                                      h->stream);
-        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
-    }
     // dy is free once the solve has read it: it holds one FE's expansion at a time
     for (int g = 0; obs && g < h->G && rc == 0; g++) {
         if (!obs[g]) continue;
@@ -3475,6 +3600,13 @@ int fehip_solve_coefficients_full(fehip_t *h, const double *y, double tol, int64
         if (rc) break;
         hipError_t e = hipMemcpyAsync(obs[g], dy, sizeof(double) * h->N, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
+    }
+    if (rc == 0 && coef) rc = s_block_to_caller(h, dc);
+    for (int g = 0; coef && g < h->G && rc == 0; g++) {
+        if (!coef[g]) continue;
+        hipError_t e = hipMemcpyAsync(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost,
+                                      h->stream);
         if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
     }
     if (rc == 0) {
@@ -3539,7 +3671,9 @@ int fehip_normalize(fehip_t *h, double *const *coef) {
                                       h->stream);
         if (e != hipSuccess) rc = fail(FEHIP_EHIP, hipGetErrorString(e));
     }
+    if (rc == 0) rc = s_block_to_internal(h, dc);
     if (rc == 0) rc = normalize_dev(h, dc);
+    if (rc == 0) rc = s_block_to_caller(h, dc);
     for (int g = 0; g < h->G && rc == 0; g++) {
         hipError_t e = hipMemcpyAsync(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost,
                                       h->stream);
@@ -3569,6 +3703,19 @@ int fehip_features(fehip_t *h, const double *y, float *feats, double *coef_ap, d
             CHK(E1::ensure_ws(h));
             return E1::features(h, dy, h->ws_feats, dca);
         });
+    if (rc == 0 && h->s_int_of) {
+        // FE s's rows and coefficients in the caller's numbering
+        const int64_t ns = h->n[h->s], o = h->off[h->s];
+        float *rows = nullptr;
+        CHK(dalloc(h, &rows, ns * 16));
+        hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(ns * 16, kBlock)), dim3(kBlock), 0, h->stream,
+                           (const float *)h->ws_feats + o * 16, (const int32_t *)h->s_int_of, ns, 16, rows);
+        HIPCHK(hipMemcpyAsync(h->ws_feats + o * 16, rows, sizeof(float) * ns * 16, hipMemcpyDeviceToDevice,
+                              h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        dfree(h, rows);
+        rc = s_block_to_caller(h, dca);
+    }
     if (rc == 0) {
         HIPCHK(hipMemcpyAsync(feats, h->ws_feats, sizeof(float) * h->p * 16, hipMemcpyDeviceToHost, h->stream));
         if (coef_ap) HIPCHK(hipMemcpyAsync(coef_ap, dca, sizeof(double) * h->p, hipMemcpyDeviceToHost, h->stream));
@@ -3654,6 +3801,7 @@ int fehip_solve_coefficients_x0(fehip_t *h, const double *y, double tol, int64_t
         rc = dispatch(prec, 1, [&](auto eng) {
             return decltype(eng)::solve_coefficients(h, dy, tol, maxiter, dc, iters, converged, coef0);
         });
+    if (rc == 0) rc = s_block_to_caller(h, dc);
     for (int g = 0; g < h->G && rc == 0; g++) {
         hipError_t e2 = hipMemcpyAsync(coef[g], dc + h->off[g], sizeof(double) * h->n[g], hipMemcpyDeviceToHost,
                                        h->stream);
@@ -3673,6 +3821,21 @@ int fehip_get_csr(fehip_t *h, int32_t g, int32_t *perm, int32_t *offsets) {
     if (!h) return fail(FEHIP_EARG, "handle is NULL");
     if (g < 0 || g >= h->G) return fail(FEHIP_EARG, "no such fixed effect");
     const FeInfo &f = h->fe[g];
+    if (g == h->s && h->s_int_of) {
+        // the blocks of the internal numbering, in caller level order
+        std::vector<int32_t> pi(h->N), oi(f.n + 1);
+        HIPCHK(hipMemcpy(pi.data(), f.operm, sizeof(int32_t) * h->N, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(oi.data(), f.ooff, sizeof(int32_t) * (f.n + 1), hipMemcpyDeviceToHost));
+        int32_t at = 0;
+        for (int64_t k = 0; k < f.n; k++) {
+            const int32_t j = h->h_s_int_of[k];
+            if (offsets) offsets[k] = at;
+            for (int32_t q = oi[j]; q < oi[j + 1]; q++, at++)
+                if (perm) perm[at] = pi[q];
+        }
+        if (offsets) offsets[f.n] = at;
+        return 0;
+    }
     if (perm) HIPCHK(hipMemcpy(perm, f.operm, sizeof(int32_t) * h->N, hipMemcpyDeviceToHost));
     if (offsets) HIPCHK(hipMemcpy(offsets, f.ooff, sizeof(int32_t) * (f.n + 1), hipMemcpyDeviceToHost));
     return 0;
@@ -3683,6 +3846,10 @@ int fehip_get_scale(fehip_t *h, int32_t g, double *scale) {
     if (!h) return fail(FEHIP_EARG, "handle is NULL");
     if (g < 0 || g >= h->G || !scale) return fail(FEHIP_EARG, "bad arguments");
     HIPCHK(hipMemcpy(scale, h->fe[g].scale_d, sizeof(double) * h->fe[g].n, hipMemcpyDeviceToHost));
+    if (g == h->s && h->s_int_of) {
+        const std::vector<double> si(scale, scale + h->fe[g].n);
+        for (int64_t k = 0; k < h->fe[g].n; k++) scale[k] = si[h->h_s_int_of[k]];
+    }
     return 0;
 }
 
@@ -3720,10 +3887,12 @@ static int mul_common(fehip_t *h, double *io, const double *other, double a, dou
     double *x_host = adjoint ? io : (double *)other;
     HIPCHK(hipMemcpy(dy, y_host, sizeof(double) * h->N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dx, x_host, sizeof(double) * h->p, hipMemcpyHostToDevice));
-    const int rc = dispatch(h->precision == FEHIP_MIXED ? FEHIP_FP64 : h->precision, 1, [&](auto e) {
+    CHK(s_block_to_internal(h, dx));
+    int rc = dispatch(h->precision == FEHIP_MIXED ? FEHIP_FP64 : h->precision, 1, [&](auto e) {
         using E = decltype(e);
         return adjoint ? E::mul_At(h, dx, dy, a, b) : E::mul_A(h, dy, dx, a, b);
     });
+    if (rc == 0 && adjoint) rc = s_block_to_caller(h, dx);
     if (rc == 0) {
         if (adjoint) HIPCHK(hipMemcpy(io, dx, sizeof(double) * h->p, hipMemcpyDeviceToHost));
         else HIPCHK(hipMemcpy(io, dy, sizeof(double) * h->N, hipMemcpyDeviceToHost));
