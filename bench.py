@@ -49,7 +49,9 @@ def log(*a):
 # "traffic_source"; a form without a measured profile reports traffic null.
 PMC_TRAFFIC = {
     ("worker_firm_1e7", "fp64", 1, "csr"): ({"k_fused": 324.1e6, "k_dtx_csr": 311.1e6, "k_update": 88.1e6},
-                                            "profiles/worker_firm_1e7_fp64_k1.md"),
+                                            "profiles/worker_firm_1e7_fp64_k1_csr.md"),
+    ("worker_firm_1e7", "fp64", 1, "pieces"): ({"k_fused": 333.8e6, "k_dtx_csr": 103.1e6, "k_update": 88.1e6},
+                                               "profiles/worker_firm_1e7_fp64_k1.md"),
 }
 
 
@@ -88,11 +90,12 @@ def kernel_bytes(N, n, kinds, K, s, factored, weighted, plan):
             dense += N * (4 + s) + K * N * s
     if npc:
         # the piece launch: for FE s (identity order) and FE t (through the
-        # pieces' permutation by FE t level): order, the piece sums, head bits,
-        # level map and scale, and each level's v read and written
+        # pieces' permutation by FE t level): the order and each position's
+        # level (4 + 4 bytes a piece), the piece sums, head bits, the level's
+        # scale, and each level's v read and written
         gt = [g for g in range(G) if kinds[g] == "csr"][0]
         for g in (gs, gt):
-            csr += 4 * npc + K * npc * s + npc // 8 + 4 * n[g] + n[g] * s + 2 * K * n[g] * s
+            csr += 8 * npc + K * npc * s + npc // 8 + n[g] * s + 2 * K * n[g] * s
         out["k_dtx_csr"] = csr
         out["k_dtx_fixup"] = 0
     else:

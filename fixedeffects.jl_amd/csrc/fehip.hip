@@ -138,6 +138,9 @@ struct fehip_solver {
     // the iteration's CSR pass and the fused-pass fix-up items in one launch
     // (FEHIP_SEG_FIX=0: separate launches)
     bool seg_fix = true;
+    // FP64 batches of 16 or more columns: the fused pass in the wide lane
+    // map (several adjacent columns per lane, FUSED_WIDE_C)
+    bool wide = true;
     // schedule overrides (fehip_set_schedule / fehip_autotune); 0 or -1 =
     // the default (environment, else sized by occupancy)
     int64_t sched_tiles_s = 0, sched_tiles_c = 0;
@@ -695,6 +698,7 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
     h->has32 = precision != FEHIP_FP64;
     h->factored = env_int("FEHIP_FACTORED", 1, 0, 1) != 0;
     h->seg_fix = env_int("FEHIP_SEG_FIX", 1, 0, 1) != 0;
+    h->wide = env_int("FEHIP_WIDE", 1, 0, 1) != 0;
     for (int g = 0; g < G; g++)
         if (interaction && interaction[g]) h->factored = false;
     if (const char *e = std::getenv("FEHIP_MIXED_TOL")) h->mixed_tol = std::atof(e);
@@ -1428,6 +1432,14 @@ static int get_plan_for(fehip_solver *h, Plan **out) {
                                  : (h->G == 3   ? (const void *)k_fused<T, Acc, K, 3, false>
                                     : h->G == 2 ? (const void *)k_fused<T, Acc, K, 2, false>
                                                 : (const void *)k_fused<T, Acc, K, 0, false>);
+    constexpr int WC = fused_wide_c<Acc, K>();
+    if constexpr (WC > 1) {
+        if (h->wide && h->G >= 2 && h->G <= 3)
+            kf = h->factored ? (h->G == 3 ? (const void *)k_fused<T, Acc, K, 3, true, 0, WC>
+                                          : (const void *)k_fused<T, Acc, K, 2, true, 0, WC>)
+                             : (h->G == 3 ? (const void *)k_fused<T, Acc, K, 3, false, 0, WC>
+                                          : (const void *)k_fused<T, Acc, K, 2, false, 0, WC>);
+    }
     const int64_t wf = resident_waves(kf, fused_lds_bytes(h, sizeof(T), K));
     const int64_t wc = resident_waves((const void *)k_dtx_seg<T, Acc, K>, 0);
     return get_plan(h, K, sizeof(T), out, wf, wc);
@@ -1823,24 +1835,37 @@ struct Engine {
     // ... and stores u scaled (never in the piece form)
     static bool ut_now(const fehip_solver *h, const Plan *P) { return ut(h) && !use_pieces(h, P); }
 
-    template <int PM>
-    static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
+    // columns per lane of the fused pass when the wide lane map applies
+    static constexpr int kWideC = fused_wide_c<Acc, K>();
+    static bool wide_now(const fehip_solver *h) { return kWideC > 1 && h->wide && h->G >= 2 && h->G <= 3; }
+
+    template <int PM, int C>
+    static void launch_fused_c(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         const dim3 gr(P->fused_blocks), bl(kBlock);
         const size_t lds = PM == 1 ? 0 : P->fused_lds;  // the sums-only pass has no few-level sums
         if constexpr (PM == 3) {  // factored, two or three FEs (build_pieces)
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, 3>), gr, bl, lds, h->stream, p);
-            else hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, 3>), gr, bl, lds, h->stream, p);
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, 3, C>), gr, bl, lds, h->stream, p);
+            else hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, 3, C>), gr, bl, lds, h->stream, p);
         } else if (h->factored) {
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, PM>), gr, bl, lds, h->stream, p);
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, PM, C>), gr, bl, lds, h->stream, p);
             else if (h->G == 2)
-                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, PM>), gr, bl, lds, h->stream, p);
-            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, true, PM>), gr, bl, lds, h->stream, p);
+                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, PM, C>), gr, bl, lds, h->stream, p);
+            else if constexpr (C == 1)
+                hipLaunchKernelGGL((k_fused<T, Acc, K, 0, true, PM>), gr, bl, lds, h->stream, p);
         } else {
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, false, PM>), gr, bl, lds, h->stream, p);
+            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, false, PM, C>), gr, bl, lds, h->stream, p);
             else if (h->G == 2)
-                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, false, PM>), gr, bl, lds, h->stream, p);
-            else hipLaunchKernelGGL((k_fused<T, Acc, K, 0, false, PM>), gr, bl, lds, h->stream, p);
+                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, false, PM, C>), gr, bl, lds, h->stream, p);
+            else if constexpr (C == 1)
+                hipLaunchKernelGGL((k_fused<T, Acc, K, 0, false, PM>), gr, bl, lds, h->stream, p);
         }
+    }
+    template <int PM>
+    static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
+        if constexpr (kWideC > 1) {
+            if (wide_now(h)) return launch_fused_c<PM, kWideC>(h, P, p);
+        }
+        launch_fused_c<PM, 1>(h, P, p);
     }
 
     // u <- b*u + a*A v with the raw D'u sums of FE s and of the dense FEs.
@@ -3931,7 +3956,7 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out) {
     if (!h || !out) return fail(FEHIP_EARG, "bad arguments");
     if (K < 1 || K > kMaxK) return fail(FEHIP_EARG, "K must be in 1..32");
     if (fp32_storage ? !h->has32 : !h->has64) return fail(FEHIP_EARG, "the handle has no such storage precision");
-    auto fill = [&](Plan *P) {
+    auto fill = [&](Plan *P, int wc) {
         out[0] = P->fused_dense ? 1 : 0;
         out[1] = P->R;
         out[2] = P->ntiles_s;
@@ -3940,6 +3965,7 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out) {
         out[5] = P->nslot;
         out[6] = P->pc.np;
         out[7] = P->csr_blocks;
+        out[8] = (wc > 1 && h->wide && h->G >= 2 && h->G <= 3) ? wc : 1;
         return 0;
     };
     const int Kp = padded_K(K);
@@ -3949,20 +3975,20 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out) {
                 using E = decltype(e);
                 Plan *P = nullptr;
                 CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
-                return fill(P);
+                return fill(P, fused_wide_c<typename E::AccT, E::kK>());
             });
         return for_K<float, F32x2, double>(Kp, [&](auto e) {
             using E = decltype(e);
             Plan *P = nullptr;
             CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
-            return fill(P);
+            return fill(P, fused_wide_c<typename E::AccT, E::kK>());
         });
     }
     return for_K<double, double, double>(Kp, [&](auto e) {
         using E = decltype(e);
         Plan *P = nullptr;
         CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
-        return fill(P);
+        return fill(P, fused_wide_c<typename E::AccT, E::kK>());
     });
 }
 

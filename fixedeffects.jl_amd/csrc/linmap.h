@@ -511,6 +511,81 @@ __device__ __forceinline__ bool seg_step(uint64_t w, uint64_t wnext, int sh, boo
     return em;
 }
 
+// Wide lane map of the fused pass: every lane holds C adjacent columns of
+// one position (lane = psub*KL + cg, columns cg*C .. cg*C+C-1, KL = K/C
+// lanes and P = 64/KL positions per wave step), so the per-position work
+// (refs, head words, level, addresses) is done once for C columns.  C = 1
+// is the plain map (lane = psub*K + c).
+template <typename T, int C>
+struct alignas(sizeof(T) * C) VecC {
+    T v[C];
+};
+// C adjacent elements at element offset i from a wave-uniform base (one
+// 8- to 32-byte access; the host keeps every row of K elements aligned)
+template <int C, typename T>
+__device__ __forceinline__ VecC<T, C> ldv(const T *base, uint32_t i) {
+    return *reinterpret_cast<const VecC<T, C> *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T));
+}
+template <int C, typename T>
+__device__ __forceinline__ void stv(T *base, uint32_t i, const VecC<T, C> &v) {
+    *reinterpret_cast<VecC<T, C> *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+
+template <typename Acc, int C>
+struct SegCursorC {
+    Acc carry[C];  // open runs carried into the next step, per column
+    bool cont;     // no segment head seen yet in this tile
+    bool open;     // a run is still open after the last step
+};
+
+// seg_step for C columns per lane (KL lanes per position): the head-bit
+// logic once per position, the carry, scan and hand-over per column.  `on`
+// is whether any of the lane's columns is on; values of columns that are
+// off are zero and their sums are never read.  tileA_tc points at the
+// lane's first column.
+template <int KL, int C, typename Acc>
+__device__ __forceinline__ bool seg_step_c(uint64_t w, uint64_t wnext, int sh, bool valid, int psub, int cg, bool on,
+                                           Acc (&x)[C], SegCursorC<Acc, C> &cur, double *tileA_tc) {
+    constexpr int P = kWave / KL;
+    const uint64_t hb = (P == 64) ? w : ((w >> sh) & ((1ull << (P & 63)) - 1ull));
+    const uint64_t nb = (sh + P < 64) ? ((w >> (sh + P)) & 1ull) : (wnext & 1ull);
+    const uint64_t hle = hb & lanes_le(psub);
+    const int rs = hle ? (63 - __clzll(hle)) : 0;
+    if (psub == 0 && !(hb & 1ull)) {
+#pragma unroll
+        for (int j = 0; j < C; j++) x[j] += cur.carry[j];
+    }
+#pragma unroll
+    for (int j = 0; j < C; j++) x[j] = seg_scan<KL>(x[j], psub, rs);
+    const bool isend = (psub == P - 1) ? (nb != 0) : (((hb >> (psub + 1)) & 1ull) != 0);
+    const bool mycont = cur.cont && (hle == 0);
+    bool em = false;
+    if (valid && on && isend) {
+        if (mycont) {
+#pragma unroll
+            for (int j = 0; j < C; j++) tileA_tc[j] = acc_to_double(x[j]);
+        } else {
+            em = true;
+        }
+    }
+    cur.open = (nb == 0);
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+        Acc last;
+        if constexpr (KL == 1) {
+            last = lane_value(x[j], kWave - 1);
+        } else if constexpr (KL == 32) {
+            const Acc other = half_swap(x[j]);
+            last = psub == 0 ? other : x[j];
+        } else {
+            last = __shfl(x[j], (P - 1) * KL + cg, kWave);
+        }
+        cur.carry[j] = cur.open ? last : Acc(0.f);
+    }
+    cur.cont = cur.cont && (hb == 0);
+    return em;
+}
+
 // acc[slot[q]*64 + lane] += x[q] for q = 0..U-1 in this order, with one LDS
 // round trip: all slots are read first, repeated slots are chained in
 // registers, and the writes go out in order (the last write to a repeated
@@ -716,6 +791,17 @@ __host__ __device__ constexpr bool dense_phased() {
     return K >= 16 && std::is_same<Acc, double>::value;
 }
 
+// Columns per lane of the fused pass's wide lane map (k_fused, C): FP64
+// batches of 16 or more columns (their few-level sums are phased); 1 = the
+// plain map.  The host uses it unless FEHIP_WIDE=0.
+#ifndef FUSED_WIDE_C
+#define FUSED_WIDE_C 4
+#endif
+template <typename Acc, int K>
+__host__ __device__ constexpr int fused_wide_c() {
+    return dense_phased<K, Acc>() ? FUSED_WIDE_C : 1;
+}
+
 // Lane-private few-level accumulators (the form that is not phased): each
 // sums the positions of one lane and level within a chunk, a small share of
 // the level, and the lanes' sums are combined in FP64.  FP32 modes keep them
@@ -749,6 +835,25 @@ __device__ __forceinline__ void phased_add(Acc *acc, const int32_t (&slot)[U], c
         if (psub == ph) {
 #pragma unroll
             for (int q = 0; q < U; q++) atomicAdd(acc + slot[q] * K + c, x[q]);
+        }
+    }
+}
+
+// phased_add for C columns per lane (KL = K/C lanes per position, P = 64/KL
+// phases): the lanes of a phase hold distinct column groups, so again no two
+// lanes of an instruction meet; within a phase, steps in order, then columns.
+template <int U, int K, int C, typename Acc>
+__device__ __forceinline__ void phased_add_c(Acc *acc, const int32_t (&slot)[U], const Acc (&x)[U][C], int psub,
+                                             int c0) {
+    static_assert(std::is_same<Acc, double>::value, "phased accumulators are FP64 only");
+    constexpr int P = kWave / (K / C);
+#pragma unroll 1
+    for (int ph = 0; ph < P; ph++) {
+        if (psub == ph) {
+#pragma unroll
+            for (int q = 0; q < U; q++)
+#pragma unroll
+                for (int j = 0; j < C; j++) atomicAdd(acc + slot[q] * K + c0 + j, x[q][j]);
         }
     }
 }
@@ -885,6 +990,12 @@ struct FusedParams {
 #ifndef FUSED_U64
 #define FUSED_U64 2
 #endif
+#ifndef FUSED_U_C2
+#define FUSED_U_C2 2
+#endif
+#ifndef FUSED_U_C4
+#define FUSED_U_C4 1
+#endif
 // GT > 0 (the FE count, compile-time): a three-stage software pipeline per
 // wave.  While loop step j is computed, the loads of step j+1 (u, caches,
 // head words and the coefficient gathers, whose refs arrived one step
@@ -922,26 +1033,41 @@ __host__ __device__ constexpr bool stores_scaled_u() {
 // PM = 3: the plain pass in the piece form (factored, GT > 0): the segments
 // are pieces, every tile starts one, and each piece's raw sum goes to psum
 // (its ordinal in position order); u is stored as it is.
-template <typename T, typename Acc, int K, int GT, bool FACT, int PM = 0>
+// C (columns per lane, wide lane map, seg_step_c): 1, or 2/4 for FP64
+// batches of 16 or more columns with compile-time FE positions, whose
+// few-level sums are phased.
+template <typename T, typename Acc, int K, int GT, bool FACT, int PM = 0, int C = 1>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
     extern __shared__ __align__(16) unsigned char fused_smem[];
-    constexpr int P = kWave / K;
+    static_assert(K % C == 0, "C divides K");
+    constexpr int KL = K / C;  // lanes per position
+    constexpr int P = kWave / KL;
     // wave steps per loop step (their loads are in flight together); FP64
     // and batched columns take fewer to stay within 128 VGPRs (4 waves per
     // SIMD) without spills (measured: U = 4 is slower for FP64 even without
-    // spills)
-    constexpr int U = (sizeof(T) == 8 || K > 1) ? FUSED_U64 : 4;
+    // spills); a wide lane map holds C columns' values per step
+    constexpr int U = C >= 4 ? FUSED_U_C4 : C == 2 ? FUSED_U_C2 : (sizeof(T) == 8 || K > 1) ? FUSED_U64 : 4;
     constexpr uint32_t UP = U * P;
     // head words one loop step touches: steps q use word (q*P)/64 and the next
     constexpr int NW = UP / kWave + 1 > 2 ? UP / kWave + 1 : 2;
     constexpr int GA = GT > 0 ? GT : 1;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
-    const int psub = lane / K, c = lane % K;
-    const bool on = p.ops->on[c] != 0;
-    const T a = (T)p.ops->a[c], b = (T)p.ops->b[c];
-    const bool keep = p.ops->b[c] != 0.0;
+    const int psub = lane / KL, cg = lane % KL;
+    const int c0 = cg * C;  // the lane's first column
+    bool on[C], keep[C];
+    T a[C], b[C];
+    bool anyon = false;
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+        const double bj = p.ops->b[c0 + j];
+        on[j] = p.ops->on[c0 + j] != 0;
+        a[j] = (T)p.ops->a[c0 + j];
+        b[j] = (T)bj;
+        keep[j] = bj != 0.0;
+        anyon = anyon || on[j];
+    }
     // few-level FE sums: lane-private accumulators (nslot x 64 per wave), or
     // for wide batches in FP64 (dense_phased) one accumulator per level and
     // column (nslot x K per wave) that the P lanes of a column add to in P
@@ -949,19 +1075,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     constexpr bool PHASED = dense_phased<K, Acc>();
     constexpr bool UT = stores_scaled_u<K, FACT>() && PM != 3;
     static_assert(PM != 3 || (GT > 0 && FACT), "the piece form is a factored pass with compile-time FE positions");
+    static_assert(C == 1 || (GT > 0 && PHASED && !UT), "the wide lane map: FP64, phased few-level sums");
     using DA = DenseAcc<K, Acc>;
+    using VT = VecC<T, C>;
     DA *acc = reinterpret_cast<DA *>(fused_smem) + (size_t)wave * p.nslot * (PHASED ? K : kWave);
     if constexpr (PHASED) {
         for (int e = lane; e < p.nslot * K; e += kWave) acc[e] = DA(0.f);
     } else {
         for (int k = 0; k < p.nslot; k++) acc[k * kWave + lane] = DA(0.f);
     }
-    auto dense_add = [&](const int32_t (&sl)[U], const DA (&x)[U]) {
-        if constexpr (PHASED) phased_add<U, K, DA>(acc, sl, x, psub, c);
-        else lds_accumulate<U, DA>(acc, sl, x, lane);
+    auto dense_add = [&](const int32_t (&sl)[U], const DA (&x)[U][C]) {
+        if constexpr (C > 1) {
+            phased_add_c<U, K, C, DA>(acc, sl, x, psub, c0);
+        } else {
+            DA x1[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) x1[q] = x[q][0];
+            if constexpr (PHASED) phased_add<U, K, DA>(acc, sl, x1, psub, c0);
+            else lds_accumulate<U, DA>(acc, sl, x1, lane);
+        }
     };
     const int t = blockIdx.x * kWavesPerBlock + wave;
-    double sq = 0.0;
+    double sq[C];
+#pragma unroll
+    for (int j = 0; j < C; j++) sq[j] = 0.0;
     if (t < p.ntiles) {  // wave-uniform
         const int64_t start = (int64_t)t * p.R;
         const int64_t end = start + p.R < p.N ? start + p.R : p.N;
@@ -969,11 +1106,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         // tile-relative bases; positions below are offsets from start
         T *ub = p.target + start * K;
         const T *swb = (FACT && p.sqrtw) ? p.sqrtw + start : nullptr;
-        SegCursor<Acc> cur{Acc(0.f), true, false};
+        SegCursorC<Acc, C> cur;
+#pragma unroll
+        for (int j = 0; j < C; j++) cur.carry[j] = Acc(0.f);
+        cur.cont = true;
+        cur.open = false;
         // one loop step, ready to compute: D*v sums, old u, FE s and first
         // dense FE cache/ref, head words
         struct Step {
-            T sum[U], uo[U], cs[U], cd[U], g0[U];
+            T sum[U][C], uo[U][C], cs[U], cd[U], g0[U];
             int32_t rs[U], rd[U];
             uint64_t hw[NW];
         };
@@ -1019,19 +1160,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             return p.seg_level[ord];
         };
         auto compute = [&](const Step &st, uint32_t lb) {
-            T un[U];
+            T un[U][C];
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const uint32_t o = lb + (uint32_t)(q * P + psub);
-                const bool ok = on && o < len;
-                T out = a * st.sum[q];
-                // UT: the stored vector is sqrt(w) .* u (st.cs = sqrt(w)),
-                // which the D'u passes then sum without caches
-                if constexpr (PM != 1) out = keep ? b * (UT ? st.uo[q] / st.cs[q] : st.uo[q]) + out : out;
-                un[q] = ok ? out : (T)0;
-                if constexpr (PM != 1) {
-                    if (ok) sto(ub, o * K + c, UT ? st.cs[q] * out : out);
-                    sq += (double)un[q] * (double)un[q];
+                VT w;
+#pragma unroll
+                for (int j = 0; j < C; j++) {
+                    const bool ok = on[j] && o < len;
+                    T out = a[j] * st.sum[q][j];
+                    // UT: the stored vector is sqrt(w) .* u (st.cs = sqrt(w)),
+                    // which the D'u passes then sum without caches
+                    if constexpr (PM != 1)
+                        out = keep[j] ? b[j] * (UT ? st.uo[q][j] / st.cs[q] : st.uo[q][j]) + out : out;
+                    un[q][j] = ok ? out : (T)0;
+                    const T sv = UT ? st.cs[q] * out : out;
+                    if constexpr (PM != 1) {
+                        if constexpr (C == 1) {
+                            if (ok) sto(ub, o * K + c0, sv);
+                        } else {
+                            w.v[j] = ok ? sv : st.uo[q][j];  // a column that is off keeps its old value
+                        }
+                        sq[j] += (double)un[q][j] * (double)un[q][j];
+                    }
+                }
+                if constexpr (PM != 1 && C > 1) {
+                    if (o < len) stv<C>(ub, o * K + c0, w);
                 }
             }
             // FE s: segmented sums of cache_s * u; a segment ending at a
@@ -1040,27 +1194,42 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             for (int q = 0; q < (PM == 2 ? 0 : U); q++) {
                 const uint32_t ob = lb + (uint32_t)(q * P);
                 if (ob >= len) break;  // wave-uniform
-                Acc x = mul_acc<Acc, T>(st.cs[q], un[q]);
+                Acc x[C];
+#pragma unroll
+                for (int j = 0; j < C; j++) x[j] = mul_acc<Acc, T>(st.cs[q], un[q][j]);
                 const int jw = (q * P) / kWave;  // constant after unrolling
-                if (seg_step<K>(st.hw[jw], st.hw[jw + 1], (int)(ob & 63u), ob + (uint32_t)psub < len, psub, c, on, x,
-                                cur, &p.tileA[(int64_t)t * K + c])) {
-                    if constexpr (PM == 1)
-                        sto(p.zsw, (uint32_t)st.rs[q] * K + c, (T)(-(FACT ? (double)st.g0[q] : 1.0) * acc_to_double(x)));
-                    else if constexpr (PM == 3)
-                        sto(p.psum, (uint32_t)st.rs[q] * K + c, acc_to_T<T, Acc>(x));
-                    else
-                        sto(p.wsum, (uint32_t)st.rs[q] * K + c, acc_to_double(x));
+                if (seg_step_c<KL, C>(st.hw[jw], st.hw[jw + 1], (int)(ob & 63u), ob + (uint32_t)psub < len, psub, cg,
+                                      anyon, x, cur, &p.tileA[(int64_t)t * K + c0])) {
+                    const uint32_t e = (uint32_t)st.rs[q] * K + c0;
+                    if constexpr (PM == 1) {
+                        VT z;
+#pragma unroll
+                        for (int j = 0; j < C; j++)
+                            z.v[j] = (T)(-(FACT ? (double)st.g0[q] : 1.0) * acc_to_double(x[j]));
+                        stv<C>(p.zsw, e, z);
+                    } else if constexpr (PM == 3) {
+                        VT z;
+#pragma unroll
+                        for (int j = 0; j < C; j++) z.v[j] = acc_to_T<T, Acc>(x[j]);
+                        stv<C>(p.psum, e, z);
+                    } else {
+                        VecC<double, C> z;
+#pragma unroll
+                        for (int j = 0; j < C; j++) z.v[j] = acc_to_double(x[j]);
+                        stv<C>(p.wsum, e, z);
+                    }
                 }
             }
             // dense FEs: lane-private LDS accumulators
             if (PM != 1 && p.ndense) {
                 const int s0 = p.dense_slot0[0];
                 int32_t sl[U];
-                DA x[U];
+                DA x[U][C];
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     sl[q] = s0 + st.rd[q];
-                    x[q] = mul_acc<DA, T>(st.cd[q], un[q]);
+#pragma unroll
+                    for (int j = 0; j < C; j++) x[q][j] = mul_acc<DA, T>(st.cd[q], un[q][j]);
                 }
                 dense_add(sl, x);
                 for (int d = 1; d < p.ndense; d++) {
@@ -1071,7 +1240,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     for (int q = 0; q < U; q++) {
                         const uint32_t i = loc(lb, q);
                         sl[q] = sd + ldo(rf, i);
-                        x[q] = mul_acc<DA, T>(FACT ? st.cs[q] : ldo(cf, i), un[q]);
+                        const T cv = FACT ? st.cs[q] : ldo(cf, i);
+#pragma unroll
+                        for (int j = 0; j < C; j++) x[q][j] = mul_acc<DA, T>(cv, un[q][j]);
                     }
                     dense_add(sl, x);
                 }
@@ -1119,7 +1290,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             // a step's loads in flight, with the refs and head words that
             // compute needs (copied, so the Refs can move on)
             struct InFlight {
-                T uo[U], cv[FACT ? 1 : GA][U], gv[GA][U], sw[U];
+                VT uo[U], gv[GA][U];
+                T cv[FACT ? 1 : GA][U], sw[U];
                 int32_t rs[U], rd[U];
                 uint64_t hw[NW];
             };
@@ -1127,7 +1299,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     const uint32_t i = loc(lb, q);
-                    d.uo[q] = PM == 1 ? (T)0 : ldo(ub, i * K + c);
+                    if constexpr (PM == 1) {
+#pragma unroll
+                        for (int j = 0; j < C; j++) d.uo[q].v[j] = (T)0;
+                    } else {
+                        d.uo[q] = ldv<C>(ub, i * K + c0);
+                    }
                     d.sw[q] = (T)1;
                     if constexpr (FACT) {
                         if (swb) d.sw[q] = ldo(swb, i);  // wave-uniform test
@@ -1138,8 +1315,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     const T *cf = p.gsrc[g];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        d.gv[g][q] = (PM == 1 && g == 0) ? ldo(cf, (uint32_t)rr.r[g][q])
-                                                         : ldo(cf, (uint32_t)rr.r[g][q] * K + c);
+                        if (PM == 1 && g == 0) d.gv[g][q].v[0] = ldo(cf, (uint32_t)rr.r[g][q]);
+                        else d.gv[g][q] = ldv<C>(cf, (uint32_t)rr.r[g][q] * K + c0);
                         if constexpr (!FACT) d.cv[g][q] = ldo(p.cache[g] + start, loc(lb, q));
                     }
                 }
@@ -1155,9 +1332,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             auto settle = [&](const InFlight &d, Step &st) {
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    st.sum[q] = (T)0;
-                    st.g0[q] = d.gv[0][q];
-                    st.uo[q] = d.uo[q];
+#pragma unroll
+                    for (int j = 0; j < C; j++) {
+                        st.sum[q][j] = (T)0;
+                        st.uo[q][j] = d.uo[q].v[j];
+                    }
+                    st.g0[q] = d.gv[0][q].v[0];
                     st.cs[q] = st.cd[q] = (T)0;
                     st.rs[q] = d.rs[q];
                     st.rd[q] = p.ndense ? d.rd[q] : 0;
@@ -1166,10 +1346,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                     for (int g = 0; g < GT; g++)
 #pragma unroll
-                        for (int q = 0; q < U; q++) st.sum[q] += (PM == 1 && g == 0) ? (T)0 : d.gv[g][q];
+                        for (int q = 0; q < U; q++)
+#pragma unroll
+                            for (int j = 0; j < C; j++) st.sum[q][j] += (PM == 1 && g == 0) ? (T)0 : d.gv[g][q].v[j];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        st.sum[q] = d.sw[q] * st.sum[q];
+#pragma unroll
+                        for (int j = 0; j < C; j++) st.sum[q][j] = d.sw[q] * st.sum[q][j];
                         st.cs[q] = st.cd[q] = d.sw[q];
                     }
                 } else {
@@ -1177,7 +1360,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     for (int g = 0; g < GT; g++)
 #pragma unroll
                         for (int q = 0; q < U; q++)
-                            st.sum[q] += (PM == 1 && g == 0) ? (T)0 : d.cv[g][q] * d.gv[g][q];
+#pragma unroll
+                            for (int j = 0; j < C; j++)
+                                st.sum[q][j] += (PM == 1 && g == 0) ? (T)0 : d.cv[g][q] * d.gv[g][q].v[j];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
                         st.cs[q] = d.cv[0][q];
@@ -1210,13 +1395,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 rb = ra;
             }
         } else {
+            // any FE count, one FE at a time (C = 1)
+            const int c = c0;
             for (uint32_t lb = 0; lb < len; lb += UP) {
                 Step st;
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    st.sum[q] = (T)0;
+                    st.sum[q][0] = (T)0;
                     st.g0[q] = (T)0;
-                    st.uo[q] = PM == 1 ? (T)0 : ldo(ub, loc(lb, q) * K + c);
+                    st.uo[q][0] = PM == 1 ? (T)0 : ldo(ub, loc(lb, q) * K + c);
                     st.cs[q] = st.cd[q] = (T)0;
                     st.rs[q] = st.rd[q] = 0;
                 }
@@ -1247,7 +1434,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                         for (int q = 0; q < U; q++) {
                             const T gv = ldo(cf, (uint32_t)r[q] * K + c);
-                            st.sum[q] += FACT ? gv : cv[q] * gv;
+                            st.sum[q][0] += FACT ? gv : cv[q] * gv;
                         }
                     }
                     if (g == p.s) {
@@ -1267,15 +1454,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                 }
                 if constexpr (FACT) {
 #pragma unroll
-                    for (int q = 0; q < U; q++) st.sum[q] = sw[q] * st.sum[q];
+                    for (int q = 0; q < U; q++) st.sum[q][0] = sw[q] * st.sum[q][0];
                 }
                 compute(st, lb);
             }
         }
         // (PM = 3: every piece ends inside its tile)
-        if (PM != 2 && PM != 3 && cur.open && on && psub == 0) {
-            if (cur.cont) p.tileA[(int64_t)t * K + c] = acc_to_double(cur.carry);
-            else p.tileB[(int64_t)t * K + c] = acc_to_double(cur.carry);
+        if (PM != 2 && PM != 3 && cur.open && anyon && psub == 0) {
+#pragma unroll
+            for (int j = 0; j < C; j++) {
+                if (cur.cont) p.tileA[(int64_t)t * K + c0 + j] = acc_to_double(cur.carry[j]);
+                else p.tileB[(int64_t)t * K + c0 + j] = acc_to_double(cur.carry[j]);
+            }
         }
     }
     if (PM != 1 && p.ndense) {
@@ -1286,7 +1476,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             for (int k = 0; k < p.nslot; k++) {
                 double x = acc_to_double(acc[k * kWave + lane]);
                 x = column_allreduce<K>(x);
-                if (psub == 0) ws[k * K + c] = x;
+                if (psub == 0) ws[k * K + c0] = x;
             }
         }
         __syncthreads();
@@ -1303,8 +1493,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
     }
     __shared__ double red[kBlock];
-    const double s = block_column_sum<K>(sq, red);
-    if (threadIdx.x < K) p.partials[(int64_t)blockIdx.x * K + threadIdx.x] = s;
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+        // column c0 + j: the lanes of column group cg, then the waves
+        const double s = block_column_sum<KL>(sq[j], red);
+        if (threadIdx.x < KL) p.partials[(int64_t)blockIdx.x * K + threadIdx.x * C + j] = s;
+    }
 }
 
 // Fix-up items: kind 0 = a segment crossing tiles (total = tileB[first] +

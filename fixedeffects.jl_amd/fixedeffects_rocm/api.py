@@ -347,10 +347,10 @@ class FixedEffectSolverROCm:
 
     def plan_info(self, K: int = 1, fp32_storage: bool = False) -> dict:
         """Launch plan of K columns (test hook fehip_plan_info)."""
-        out = np.zeros(8, np.int64)
+        out = np.zeros(9, np.int64)
         check(lib.fehip_plan_info(self._h, int(K), 1 if fp32_storage else 0, _ptr(out, C.c_int64)))
         keys = ("fused_dense", "tile_positions", "fused_tiles", "dtx_tiles", "fixup_items", "dense_slots",
-                "pieces", "csr_blocks")
+                "pieces", "csr_blocks", "fused_cols_per_lane")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def lsmr_info(self, col: int = 0) -> dict:

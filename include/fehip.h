@@ -286,8 +286,9 @@ int fehip_lsmr_info(fehip_t *h, int32_t col, double *out);
  *   out[0] 1 if the few-level FEs are summed inside the fused pass
  *   out[1] positions per fused-pass tile      out[2] fused-pass tiles
  *   out[3] tiles of the D'u passes (all FEs)  out[4] fix-up items
- *   out[5] few-level accumulator slots        out[6] reserved (0)
- *   out[7] workgroups of the CSR D'u launch */
+ *   out[5] few-level accumulator slots        out[6] pieces of the piece form (0: CSR form)
+ *   out[7] workgroups of the CSR D'u launch   out[8] columns per lane of the fused pass
+ * out holds 9 elements. */
 int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out);
 
 /* Measurement hook (no reference counterpart): average device time in ms of

@@ -10,7 +10,9 @@ cover, at sizes the CPU oracle finishes in seconds, through the C-ABI:
 * bitwise repeatability of wide FP64 batches with a few-level FE (the phased
   LDS accumulators);
 * the reference Float32 mode with a 33-64-level FE inside the fused pass,
-  against the oracle's Float32 restatement.
+  against the oracle's Float32 restatement;
+* the fused pass's wide lane map (4 columns per lane) against the plain map
+  on the same panels, in the piece, cache and projected forms.
 
 The solve under test is the inner solve_residuals! (src/AbstractFixedEffectSolver.jl
 :49-68), its multi-column loop (:74-97), and lsmr!'s start hook
@@ -285,3 +287,53 @@ def test_piece_form_taken_where_it_applies(gpu, shape):
         r, it, cv = s.solve_residuals(np.ascontiguousarray(p.Y[:, 0]).copy(), tol=1e-12, maxiter=100_000)
     _say(f"  iters {it} rel err {_rel(r, ro):.2e}")
     assert cv and _rel(r, ro) <= RES_RTOL
+
+
+# -- the fused pass's wide lane map (several columns per lane) -------------------------
+
+def _solve_wide(p, K, wide, algorithm="lsmr", tol=1e-10):
+    import os
+    old = os.environ.get("FEHIP_WIDE")
+    os.environ["FEHIP_WIDE"] = "1" if wide else "0"
+    try:
+        with api.FixedEffectSolverROCm(_fes(p), p.weights, "fp64", algorithm=algorithm) as s:
+            info = s.plan_info(K)
+            cols = [np.ascontiguousarray(p.Y[:, c % p.Y.shape[1]]).copy() for c in range(K)]
+            _, its, cvs = s.solve_residuals_many(cols, tol=tol, maxiter=100_000)
+    finally:
+        if old is None:
+            del os.environ["FEHIP_WIDE"]
+        else:
+            os.environ["FEHIP_WIDE"] = old
+    return info, cols, its, cvs
+
+
+@pytest.mark.parametrize("K", [16, 32])
+@pytest.mark.parametrize("form", ["pieces", "cache", "projected"])
+def test_wide_lane_map_matches_plain(gpu, K, form):
+    # FP64 batches of 16 or 32 columns: the fused pass with 4 columns per lane
+    # against the plain map (one column per lane) on the same panel, and both
+    # against the oracle's converged residual of the first columns.  Forms:
+    # the factored piece form, the cache form (an interacted FE), and the
+    # projected iteration (sums-only and corrected passes)
+    _say(f"case: wide lane map K={K} form={form}")
+    if form == "cache":
+        p = synth.config("worker_firm_1e8_k16", scale=0.001)
+    else:
+        p = synth.config("worker_firm_1e7_k32", scale=0.01)
+    algorithm = "projected" if form == "projected" else "lsmr"
+    iw, rw, itw, cw = _solve_wide(p, K, True, algorithm)
+    ip, rp, itp, cp = _solve_wide(p, K, False, algorithm)
+    _say(f"  N={p.N} lanes {iw['fused_cols_per_lane']} / {ip['fused_cols_per_lane']} pieces {iw['pieces']} "
+         f"iters {itw[:4]} / {itp[:4]}")
+    assert iw["fused_cols_per_lane"] == 4 and ip["fused_cols_per_lane"] == 1
+    if form != "projected":  # (the projected iteration keeps the CSR pass whatever the plan holds)
+        assert (iw["pieces"] > 0) == (form == "pieces")
+    assert all(cw) and all(cp)
+    worst = max(_rel(a, b) for a, b in zip(rw, rp))
+    _say(f"  max rel diff wide vs plain {worst:.2e}")
+    assert worst <= 1e-9
+    o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=NTHREADS)
+    for c in range(min(2, p.Y.shape[1])):
+        ro = o.solve_residuals(p.Y[:, c].copy(), tol=1e-12, maxiter=100_000)[0]
+        assert _rel(rw[c], ro) <= RES_RTOL, (c, _rel(rw[c], ro))
