@@ -894,6 +894,13 @@ static bool fuse_dense_on(const fehip_solver *h) {
     return env_int("FEHIP_FUSE_DENSE", 1, 0, 1) != 0;
 }
 
+// columns per lane of the fused pass (the wide lane map, fused_wide_c in
+// linmap.h) for storage of ts bytes and K columns on this handle
+static int fused_cols(const fehip_solver *h, size_t ts, int K) {
+    if (!h->wide || h->G < 2 || h->G > 3 || K < 16) return 1;
+    return ts == 8 ? FUSED_WIDE_C : FUSED_WIDE_C32;
+}
+
 static int64_t tile_size(int64_t N, int64_t ntiles) {
     const int64_t t = (N + ntiles - 1) / ntiles;
     return std::max<int64_t>(256, (t + 255) / 256 * 256);
@@ -1091,7 +1098,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     // 64 KB of LDS.
     const int Pq = kWave / K;
     // accumulator bytes: FP64 double; FP32 modes a float per lane and level
-    // (DenseAcc in linmap.h: phasing is FP64-only)
+    // (DenseAcc in linmap.h: k_dtx_dense phases FP64 only)
     const size_t accsz = ts == 8 ? sizeof(double) : sizeof(float);
     int maxlev = 1;
     for (int g = 0; g < h->G; g++)
@@ -1101,8 +1108,11 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
             P->nslot += (int32_t)h->fe[g].n;
         }
     // accumulators per slot and wave: one per lane, or one per column
-    // (dense_phased: FP64, K >= 16), in k_fused and k_dtx_dense alike
+    // (dense_phased: FP64, K >= 16), in k_fused and k_dtx_dense alike; the
+    // fused pass in the wide lane map keeps one FP64 accumulator per column
+    // in every precision
     const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * accsz : (size_t)kWave * accsz;
+    const size_t per_slot_fused = fused_cols(h, ts, K) > 1 ? (size_t)K * sizeof(double) : per_slot;
     P->dense_lds_wave = (int32_t)(per_slot * maxlev);
     // (k_dtx_dense then runs on the fused partition, 4 waves per workgroup:
     // its accumulators must fit 64 KB as well)
@@ -1112,7 +1122,7 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     if (P->fused_dense) {
         chunk = P->R;
         P->dense_wpb = kWavesPerBlock;
-        P->fused_lds = (size_t)P->nslot * per_slot * kWavesPerBlock;
+        P->fused_lds = (size_t)P->nslot * per_slot_fused * kWavesPerBlock;
     } else {
         chunk = (N + 4095) / 4096;
         chunk = std::max<int64_t>(chunk, 16 * Pq);
@@ -1395,6 +1405,7 @@ static void destroy_pieces(fehip_solver *h, PieceForm &pc) {
 static size_t fused_lds_bytes(const fehip_solver *h, size_t ts, int K) {
     const size_t accsz = ts == 8 ? sizeof(double) : sizeof(float);
     const size_t per_slot = ts == 8 && K >= 16 ? (size_t)K * accsz : (size_t)kWave * accsz;
+    const size_t per_slot_fused = fused_cols(h, ts, K) > 1 ? (size_t)K * sizeof(double) : per_slot;
     int64_t nslot = 0;
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) nslot += h->fe[g].n;
@@ -1402,7 +1413,9 @@ static size_t fused_lds_bytes(const fehip_solver *h, size_t ts, int K) {
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) maxlev = std::max<int64_t>(maxlev, h->fe[g].n);
     const size_t b = (size_t)nslot * per_slot * kWavesPerBlock;
-    return (b <= 65536 && (size_t)maxlev * per_slot * kWavesPerBlock <= 65536 && fuse_dense_on(h)) ? b : 0;
+    return (b <= 65536 && (size_t)maxlev * per_slot * kWavesPerBlock <= 65536 && fuse_dense_on(h))
+               ? (size_t)nslot * per_slot_fused * kWavesPerBlock
+               : 0;
 }
 
 // resident waves of a kernel on this device (all CUs, kBlock threads)
@@ -1434,7 +1447,7 @@ static int get_plan_for(fehip_solver *h, Plan **out) {
                                                 : (const void *)k_fused<T, Acc, K, 0, false>);
     constexpr int WC = fused_wide_c<Acc, K>();
     if constexpr (WC > 1) {
-        if (h->wide && h->G >= 2 && h->G <= 3)
+        if (fused_cols(h, sizeof(T), K) == WC)
             kf = h->factored ? (h->G == 3 ? (const void *)k_fused<T, Acc, K, 3, true, 0, WC>
                                           : (const void *)k_fused<T, Acc, K, 2, true, 0, WC>)
                              : (h->G == 3 ? (const void *)k_fused<T, Acc, K, 3, false, 0, WC>
@@ -1837,7 +1850,7 @@ struct Engine {
 
     // columns per lane of the fused pass when the wide lane map applies
     static constexpr int kWideC = fused_wide_c<Acc, K>();
-    static bool wide_now(const fehip_solver *h) { return kWideC > 1 && h->wide && h->G >= 2 && h->G <= 3; }
+    static bool wide_now(const fehip_solver *h) { return kWideC > 1 && fused_cols(h, sizeof(T), K) == kWideC; }
 
     template <int PM, int C>
     static void launch_fused_c(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
@@ -3965,7 +3978,7 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out) {
         out[5] = P->nslot;
         out[6] = P->pc.np;
         out[7] = P->csr_blocks;
-        out[8] = (wc > 1 && h->wide && h->G >= 2 && h->G <= 3) ? wc : 1;
+        out[8] = (wc > 1 && fused_cols(h, fp32_storage ? 4 : 8, wc > 1 ? (int)P->K : 1) == wc) ? wc : 1;
         return 0;
     };
     const int Kp = padded_K(K);

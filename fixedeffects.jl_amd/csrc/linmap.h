@@ -69,6 +69,11 @@ __device__ __forceinline__ F32x2 mul_acc<F32x2, float>(float a, float b) {
     const float p = a * b;
     return F32x2(p, __builtin_fmaf(a, b, -p));
 }
+// exact product of two floats in a double (24 + 24 significant bits)
+template <>
+__device__ __forceinline__ double mul_acc<double, float>(float a, float b) {
+    return (double)a * (double)b;
+}
 
 template <typename Acc>
 __device__ __forceinline__ double acc_to_double(Acc a) {
@@ -791,15 +796,20 @@ __host__ __device__ constexpr bool dense_phased() {
     return K >= 16 && std::is_same<Acc, double>::value;
 }
 
-// Columns per lane of the fused pass's wide lane map (k_fused, C): FP64
-// batches of 16 or more columns (their few-level sums are phased); 1 = the
-// plain map.  The host uses it unless FEHIP_WIDE=0.
+// Columns per lane of the fused pass's wide lane map (k_fused, C): batches
+// of 16 or more columns (FUSED_WIDE_C in FP64, FUSED_WIDE_C32 in the FP32
+// modes); 1 = the plain map.  The fused pass then sums few-level FEs in
+// phased per-column FP64 accumulators whatever the precision.  The host uses
+// it unless FEHIP_WIDE=0 (fused_cols in fehip.hip applies the same rule).
 #ifndef FUSED_WIDE_C
 #define FUSED_WIDE_C 4
 #endif
+#ifndef FUSED_WIDE_C32
+#define FUSED_WIDE_C32 4
+#endif
 template <typename Acc, int K>
 __host__ __device__ constexpr int fused_wide_c() {
-    return dense_phased<K, Acc>() ? FUSED_WIDE_C : 1;
+    return K < 16 ? 1 : std::is_same<Acc, double>::value ? FUSED_WIDE_C : FUSED_WIDE_C32;
 }
 
 // Lane-private few-level accumulators (the form that is not phased): each
@@ -842,10 +852,11 @@ __device__ __forceinline__ void phased_add(Acc *acc, const int32_t (&slot)[U], c
 // phased_add for C columns per lane (KL = K/C lanes per position, P = 64/KL
 // phases): the lanes of a phase hold distinct column groups, so again no two
 // lanes of an instruction meet; within a phase, steps in order, then columns.
+// FP64 accumulators in every precision (k_fused).
 template <int U, int K, int C, typename Acc>
 __device__ __forceinline__ void phased_add_c(Acc *acc, const int32_t (&slot)[U], const Acc (&x)[U][C], int psub,
                                              int c0) {
-    static_assert(std::is_same<Acc, double>::value, "phased accumulators are FP64 only");
+    static_assert(std::is_same<Acc, double>::value, "FP64 LDS adds");
     constexpr int P = kWave / (K / C);
 #pragma unroll 1
     for (int ph = 0; ph < P; ph++) {
@@ -996,6 +1007,9 @@ struct FusedParams {
 #ifndef FUSED_U_C4
 #define FUSED_U_C4 1
 #endif
+#ifndef FUSED_U_C4F
+#define FUSED_U_C4F 1
+#endif
 // GT > 0 (the FE count, compile-time): a three-stage software pipeline per
 // wave.  While loop step j is computed, the loads of step j+1 (u, caches,
 // head words and the coefficient gathers, whose refs arrived one step
@@ -1047,7 +1061,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // and batched columns take fewer to stay within 128 VGPRs (4 waves per
     // SIMD) without spills (measured: U = 4 is slower for FP64 even without
     // spills); a wide lane map holds C columns' values per step
-    constexpr int U = C >= 4 ? FUSED_U_C4 : C == 2 ? FUSED_U_C2 : (sizeof(T) == 8 || K > 1) ? FUSED_U64 : 4;
+    constexpr int U = C >= 4 ? (sizeof(T) == 8 ? FUSED_U_C4 : FUSED_U_C4F)
+                      : C == 2 ? FUSED_U_C2 : (sizeof(T) == 8 || K > 1) ? FUSED_U64 : 4;
     constexpr uint32_t UP = U * P;
     // head words one loop step touches: steps q use word (q*P)/64 and the next
     constexpr int NW = UP / kWave + 1 > 2 ? UP / kWave + 1 : 2;
@@ -1069,15 +1084,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         anyon = anyon || on[j];
     }
     // few-level FE sums: lane-private accumulators (nslot x 64 per wave), or
-    // for wide batches in FP64 (dense_phased) one accumulator per level and
-    // column (nslot x K per wave) that the P lanes of a column add to in P
-    // ordered phases with LDS adds -- a quarter to a half of the LDS
-    constexpr bool PHASED = dense_phased<K, Acc>();
+    // for wide batches in FP64 (dense_phased) and in the wide lane map one
+    // accumulator per level and column (nslot x K per wave) that the P lanes
+    // of a column add to in P ordered phases with LDS adds -- a quarter to a
+    // half of the LDS
+    constexpr bool PHASED = dense_phased<K, Acc>() || C > 1;
     constexpr bool UT = stores_scaled_u<K, FACT>() && PM != 3;
     static_assert(PM != 3 || (GT > 0 && FACT), "the piece form is a factored pass with compile-time FE positions");
-    static_assert(C == 1 || (GT > 0 && PHASED && !UT), "the wide lane map: FP64, phased few-level sums");
-    using DA = DenseAcc<K, Acc>;
+    static_assert(C == 1 || (GT > 0 && !UT), "the wide lane map: compile-time FE positions, u stored as it is");
+    // the wide map's phased accumulators are FP64 in every precision (the
+    // exact products of floats summed in doubles): measured on MI355X, LDS
+    // float atomic adds (ds_add_f32) cost 1.4 ms per K = 32 pass where the
+    // same adds in FP64 (ds_add_f64) cost about 30 us
+    using DA = typename std::conditional<(C > 1), double, DenseAcc<K, Acc>>::type;
     using VT = VecC<T, C>;
+    // segment (FE s level or piece) sums: the mixed mode's FP32 phase adds
+    // the exact products of its floats in FP64 (full-rate FP64 arithmetic,
+    // fewer instructions than a compensated pair and at least as accurate)
+    using SA = typename std::conditional<std::is_same<Acc, F32x2>::value, double, Acc>::type;
     DA *acc = reinterpret_cast<DA *>(fused_smem) + (size_t)wave * p.nslot * (PHASED ? K : kWave);
     if constexpr (PHASED) {
         for (int e = lane; e < p.nslot * K; e += kWave) acc[e] = DA(0.f);
@@ -1106,9 +1130,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         // tile-relative bases; positions below are offsets from start
         T *ub = p.target + start * K;
         const T *swb = (FACT && p.sqrtw) ? p.sqrtw + start : nullptr;
-        SegCursorC<Acc, C> cur;
+        SegCursorC<SA, C> cur;
 #pragma unroll
-        for (int j = 0; j < C; j++) cur.carry[j] = Acc(0.f);
+        for (int j = 0; j < C; j++) cur.carry[j] = SA(0.f);
         cur.cont = true;
         cur.open = false;
         // one loop step, ready to compute: D*v sums, old u, FE s and first
@@ -1194,9 +1218,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             for (int q = 0; q < (PM == 2 ? 0 : U); q++) {
                 const uint32_t ob = lb + (uint32_t)(q * P);
                 if (ob >= len) break;  // wave-uniform
-                Acc x[C];
+                SA x[C];
 #pragma unroll
-                for (int j = 0; j < C; j++) x[j] = mul_acc<Acc, T>(st.cs[q], un[q][j]);
+                for (int j = 0; j < C; j++) x[j] = mul_acc<SA, T>(st.cs[q], un[q][j]);
                 const int jw = (q * P) / kWave;  // constant after unrolling
                 if (seg_step_c<KL, C>(st.hw[jw], st.hw[jw + 1], (int)(ob & 63u), ob + (uint32_t)psub < len, psub, cg,
                                       anyon, x, cur, &p.tileA[(int64_t)t * K + c0])) {
@@ -1210,7 +1234,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                     } else if constexpr (PM == 3) {
                         VT z;
 #pragma unroll
-                        for (int j = 0; j < C; j++) z.v[j] = acc_to_T<T, Acc>(x[j]);
+                        for (int j = 0; j < C; j++) z.v[j] = acc_to_T<T, SA>(x[j]);
                         stv<C>(p.psum, e, z);
                     } else {
                         VecC<double, C> z;
@@ -1480,11 +1504,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
         }
         __syncthreads();
-        const size_t region = (size_t)p.nslot * (PHASED ? K : kWave) * sizeof(DA) / sizeof(double);
+        // phased: the waves' accumulators as they are (DA); lane-private: the
+        // per-wave doubles written in place above
+        const size_t region = PHASED ? (size_t)p.nslot * K : (size_t)p.nslot * kWave * sizeof(DA) / sizeof(double);
+        const DA *base_acc = reinterpret_cast<const DA *>(fused_smem);
         const double *base_ws = reinterpret_cast<const double *>(fused_smem);
         for (int e = threadIdx.x; e < p.nslot * K; e += blockDim.x) {
             double s = 0.0;
-            for (int w = 0; w < kWavesPerBlock; w++) s += base_ws[w * region + e];
+            for (int w = 0; w < kWavesPerBlock; w++)
+                s += PHASED ? acc_to_double(base_acc[w * region + e]) : base_ws[w * region + e];
             const int slot = e / K, cc = e % K;
             int d = 0;
             while (d + 1 < p.ndense && slot >= p.dense_slot0[d + 1]) d++;

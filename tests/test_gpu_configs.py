@@ -291,13 +291,13 @@ def test_piece_form_taken_where_it_applies(gpu, shape):
 
 # -- the fused pass's wide lane map (several columns per lane) -------------------------
 
-def _solve_wide(p, K, wide, algorithm="lsmr", tol=1e-10):
+def _solve_wide(p, K, wide, algorithm="lsmr", tol=1e-10, prec="fp64"):
     import os
     old = os.environ.get("FEHIP_WIDE")
     os.environ["FEHIP_WIDE"] = "1" if wide else "0"
     try:
-        with api.FixedEffectSolverROCm(_fes(p), p.weights, "fp64", algorithm=algorithm) as s:
-            info = s.plan_info(K)
+        with api.FixedEffectSolverROCm(_fes(p), p.weights, prec, algorithm=algorithm) as s:
+            info = s.plan_info(K, fp32_storage=prec != "fp64")
             cols = [np.ascontiguousarray(p.Y[:, c % p.Y.shape[1]]).copy() for c in range(K)]
             _, its, cvs = s.solve_residuals_many(cols, tol=tol, maxiter=100_000)
     finally:
@@ -309,21 +309,25 @@ def _solve_wide(p, K, wide, algorithm="lsmr", tol=1e-10):
 
 
 @pytest.mark.parametrize("K", [16, 32])
-@pytest.mark.parametrize("form", ["pieces", "cache", "projected"])
-def test_wide_lane_map_matches_plain(gpu, K, form):
-    # FP64 batches of 16 or 32 columns: the fused pass with 4 columns per lane
-    # against the plain map (one column per lane) on the same panel, and both
-    # against the oracle's converged residual of the first columns.  Forms:
-    # the factored piece form, the cache form (an interacted FE), and the
-    # projected iteration (sums-only and corrected passes)
-    _say(f"case: wide lane map K={K} form={form}")
+@pytest.mark.parametrize("prec,form", [("fp64", "pieces"), ("fp64", "cache"), ("fp64", "projected"),
+                                       ("mixed", "pieces"), ("mixed", "cache"), ("fp32", "pieces")])
+def test_wide_lane_map_matches_plain(gpu, K, prec, form):
+    # batches of 16 or 32 columns: the fused pass with 4 columns per lane
+    # against the plain map (one column per lane) on the same panel, and the
+    # FP64 and mixed solves against the oracle's converged residual of the
+    # first columns.  Forms: the factored piece form, the cache form (an
+    # interacted FE), the projected iteration (sums-only and corrected
+    # passes); precisions: FP64, the mixed mode (both phases wide) and the
+    # reference Float32 mode
+    _say(f"case: wide lane map K={K} prec={prec} form={form}")
     if form == "cache":
         p = synth.config("worker_firm_1e8_k16", scale=0.001)
     else:
         p = synth.config("worker_firm_1e7_k32", scale=0.01)
     algorithm = "projected" if form == "projected" else "lsmr"
-    iw, rw, itw, cw = _solve_wide(p, K, True, algorithm)
-    ip, rp, itp, cp = _solve_wide(p, K, False, algorithm)
+    tol = 1e-6 if prec == "fp32" else 1e-10
+    iw, rw, itw, cw = _solve_wide(p, K, True, algorithm, tol, prec)
+    ip, rp, itp, cp = _solve_wide(p, K, False, algorithm, tol, prec)
     _say(f"  N={p.N} lanes {iw['fused_cols_per_lane']} / {ip['fused_cols_per_lane']} pieces {iw['pieces']} "
          f"iters {itw[:4]} / {itp[:4]}")
     assert iw["fused_cols_per_lane"] == 4 and ip["fused_cols_per_lane"] == 1
@@ -332,7 +336,15 @@ def test_wide_lane_map_matches_plain(gpu, K, form):
     assert all(cw) and all(cp)
     worst = max(_rel(a, b) for a, b in zip(rw, rp))
     _say(f"  max rel diff wide vs plain {worst:.2e}")
-    assert worst <= 1e-9
+    if prec == "fp32":
+        # Float32 streams and scalars: both stop near tol 1e-6 in Float32
+        assert worst <= 1e-3
+        assert max(abs(a - b) for a, b in zip(itw, itp)) <= max(3, 0.1 * max(itp))
+        return
+    # FP64: the two maps differ only in the order of the norm partials' sums;
+    # mixed: the FP32 phase's end point differs in float rounding, and the FP64
+    # phase stops at tol 1e-10 from there (the converged bar)
+    assert worst <= (1e-9 if prec == "fp64" else RES_RTOL)
     o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=NTHREADS)
     for c in range(min(2, p.Y.shape[1])):
         ro = o.solve_residuals(p.Y[:, c].copy(), tol=1e-12, maxiter=100_000)[0]
