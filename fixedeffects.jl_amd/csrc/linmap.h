@@ -801,8 +801,11 @@ __host__ __device__ constexpr bool dense_phased() {
 // modes); 1 = the plain map.  The fused pass then sums few-level FEs in
 // phased per-column FP64 accumulators whatever the precision.  The host uses
 // it unless FEHIP_WIDE=0 (fused_cols in fehip.hip applies the same rule).
+// measured at K = 32 (fused pass, N = 1e7): FP64 C = 2 with two wave steps
+// in flight 1.44 ms, C = 4 with one 1.59 ms; FP32 C = 4 1.07 ms, C = 2 1.30
+// ms; C = 8, or C = 4 with two steps, spill registers (2.8-6.4 ms)
 #ifndef FUSED_WIDE_C
-#define FUSED_WIDE_C 4
+#define FUSED_WIDE_C 2
 #endif
 #ifndef FUSED_WIDE_C32
 #define FUSED_WIDE_C32 4
