@@ -113,6 +113,7 @@ struct fehip_solver {
     hipStream_t stream = nullptr;
     std::vector<FeInfo> fe;
     int32_t *perm_s = nullptr;  // internal position -> caller's observation
+    int32_t *pos_s = nullptr;   // caller's observation -> internal position (perm_s inverted)
     int32_t *ident = nullptr;   // 0..N-1: the "permutation" of the sorted FE (kernels load it unconditionally)
     // the fused pass's last two FEs in one ref stream (G = 3): the first
     // few-level FE (pack_d, <= 64 levels) in the top 6 bits over the other
@@ -787,6 +788,10 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
         } else {
             CHK(radix_sort(h, ref_orig[h->s], N, f.n, f.ref_int, h->perm_s));
         }
+        CHK(dalloc(h, &h->pos_s, N));
+        hipLaunchKernelGGL(k_invert_perm, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, h->stream,
+                           (const int32_t *)h->perm_s, N, h->pos_s);
+        HIPCHK(hipGetLastError());
         CHK(csr_products(h, f.ref_int, N, f.n, nullptr, &f));
         // the reference's bucketing over the caller's order (scale!, get_csr)
         CHK(dalloc(h, &f.operm, N));
@@ -898,7 +903,7 @@ static bool fuse_dense_on(const fehip_solver *h) {
 // linmap.h) for storage of ts bytes and K columns on this handle
 static int fused_cols(const fehip_solver *h, size_t ts, int K) {
     if (!h->wide || h->G < 2 || h->G > 3 || K < 16) return 1;
-    return ts == 8 ? FUSED_WIDE_C : FUSED_WIDE_C32;
+    return ts != 8 ? FUSED_WIDE_C32 : h->factored ? FUSED_WIDE_C : FUSED_WIDE_C_CACHE;
 }
 
 static int64_t tile_size(int64_t N, int64_t ntiles) {
@@ -1445,13 +1450,16 @@ static int get_plan_for(fehip_solver *h, Plan **out) {
                                  : (h->G == 3   ? (const void *)k_fused<T, Acc, K, 3, false>
                                     : h->G == 2 ? (const void *)k_fused<T, Acc, K, 2, false>
                                                 : (const void *)k_fused<T, Acc, K, 0, false>);
-    constexpr int WC = fused_wide_c<Acc, K>();
-    if constexpr (WC > 1) {
-        if (fused_cols(h, sizeof(T), K) == WC)
-            kf = h->factored ? (h->G == 3 ? (const void *)k_fused<T, Acc, K, 3, true, 0, WC>
-                                          : (const void *)k_fused<T, Acc, K, 2, true, 0, WC>)
-                             : (h->G == 3 ? (const void *)k_fused<T, Acc, K, 3, false, 0, WC>
-                                          : (const void *)k_fused<T, Acc, K, 2, false, 0, WC>);
+    constexpr int WF = fused_wide_c<Acc, K, true>(), WN = fused_wide_c<Acc, K, false>();
+    if constexpr (WF > 1) {
+        if (h->factored && fused_cols(h, sizeof(T), K) == WF)
+            kf = h->G == 3 ? (const void *)k_fused<T, Acc, K, 3, true, 0, WF>
+                           : (const void *)k_fused<T, Acc, K, 2, true, 0, WF>;
+    }
+    if constexpr (WN > 1) {
+        if (!h->factored && fused_cols(h, sizeof(T), K) == WN)
+            kf = h->G == 3 ? (const void *)k_fused<T, Acc, K, 3, false, 0, WN>
+                           : (const void *)k_fused<T, Acc, K, 2, false, 0, WN>;
     }
     const int64_t wf = resident_waves(kf, fused_lds_bytes(h, sizeof(T), K));
     const int64_t wc = resident_waves((const void *)k_dtx_seg<T, Acc, K>, 0);
@@ -1538,6 +1546,65 @@ __global__ void k_store_cols(double *__restrict__ Y, int64_t ldY, int K, int Kre
     T val = r[e];
     if (sqrtw) val = (T)((double)val / sqrtw[j]);
     Y[(int64_t)c * ldY + perm_s[j]] = (double)val;
+}
+
+// Tiled forms for K >= 8 (internal rows of 64 bytes or more): a workgroup
+// takes 64 consecutive caller rows, reads each column's 64 values as one
+// coalesced run into LDS, and writes each row's K values as one contiguous
+// run at its internal position (pos_s); the store is the reverse.  The
+// untiled forms gather every element of the column-major block through
+// perm_s, one 8-byte read per fetched line at K = 32.  Same arithmetic.
+constexpr int kTileRows = 64;
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_load_cols_t(const double *__restrict__ Y, int64_t ldY, int K, int Kreal,
+                                                        const int32_t *__restrict__ pos_s,
+                                                        const double *__restrict__ sqrtw, int64_t N,
+                                                        T *__restrict__ r, T *__restrict__ u) {
+#pragma clang fp contract(off)
+    __shared__ double tile[kMaxK * (kTileRows + 1)];  // [column][row], padded against bank conflicts
+    const int64_t j0 = (int64_t)blockIdx.x * kTileRows;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int64_t jj = j0 + lane;
+    for (int c = wave; c < Kreal; c += kWavesPerBlock) tile[c * (kTileRows + 1) + lane] = jj < N ? Y[c * ldY + jj] : 0.0;
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTileRows * K; e += kBlock) {
+        const int l = e / K, c = e % K;
+        const int64_t j = j0 + l;
+        if (j >= N) break;  // (rows in order)
+        const int64_t i = pos_s[j];
+        T val = (T)0;
+        if (c < Kreal) {
+            val = (T)tile[c * (kTileRows + 1) + l];
+            if (sqrtw) val = (T)((double)val * sqrtw[i]);
+        }
+        r[i * K + c] = val;
+        if (u) u[i * K + c] = val;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_store_cols_t(double *__restrict__ Y, int64_t ldY, int K, int Kreal,
+                                                         const int32_t *__restrict__ pos_s,
+                                                         const double *__restrict__ sqrtw, int64_t N,
+                                                         const T *__restrict__ r) {
+#pragma clang fp contract(off)
+    __shared__ double tile[kMaxK * (kTileRows + 1)];
+    const int64_t j0 = (int64_t)blockIdx.x * kTileRows;
+    for (int e = threadIdx.x; e < kTileRows * K; e += kBlock) {
+        const int l = e / K, c = e % K;
+        const int64_t j = j0 + l;
+        if (j >= N) break;
+        if (c >= Kreal) continue;
+        const int64_t i = pos_s[j];
+        T val = r[i * K + c];
+        if (sqrtw) val = (T)((double)val / sqrtw[i]);
+        tile[c * (kTileRows + 1) + l] = (double)val;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int64_t jj = j0 + lane;
+    if (jj < N)
+        for (int c = wave; c < Kreal; c += kWavesPerBlock) Y[c * ldY + jj] = tile[c * (kTileRows + 1) + lane];
 }
 
 template <typename T>
@@ -1848,37 +1915,29 @@ struct Engine {
     // ... and stores u scaled (never in the piece form)
     static bool ut_now(const fehip_solver *h, const Plan *P) { return ut(h) && !use_pieces(h, P); }
 
-    // columns per lane of the fused pass when the wide lane map applies
-    static constexpr int kWideC = fused_wide_c<Acc, K>();
-    static bool wide_now(const fehip_solver *h) { return kWideC > 1 && fused_cols(h, sizeof(T), K) == kWideC; }
-
-    template <int PM, int C>
-    static void launch_fused_c(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
-        const dim3 gr(P->fused_blocks), bl(kBlock);
-        const size_t lds = PM == 1 ? 0 : P->fused_lds;  // the sums-only pass has no few-level sums
-        if constexpr (PM == 3) {  // factored, two or three FEs (build_pieces)
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, 3, C>), gr, bl, lds, h->stream, p);
-            else hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, 3, C>), gr, bl, lds, h->stream, p);
-        } else if (h->factored) {
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, true, PM, C>), gr, bl, lds, h->stream, p);
-            else if (h->G == 2)
-                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, true, PM, C>), gr, bl, lds, h->stream, p);
-            else if constexpr (C == 1)
-                hipLaunchKernelGGL((k_fused<T, Acc, K, 0, true, PM>), gr, bl, lds, h->stream, p);
-        } else {
-            if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, false, PM, C>), gr, bl, lds, h->stream, p);
-            else if (h->G == 2)
-                hipLaunchKernelGGL((k_fused<T, Acc, K, 2, false, PM, C>), gr, bl, lds, h->stream, p);
-            else if constexpr (C == 1)
-                hipLaunchKernelGGL((k_fused<T, Acc, K, 0, false, PM>), gr, bl, lds, h->stream, p);
-        }
-    }
+    // the fused pass on this handle: factored or cache form, then its lane
+    // map (fused_cols / fused_wide_c: the columns per lane), then the FE count
     template <int PM>
     static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
-        if constexpr (kWideC > 1) {
-            if (wide_now(h)) return launch_fused_c<PM, kWideC>(h, P, p);
+        if (h->factored) launch_fused_f<PM, true>(h, P, p);
+        else if constexpr (PM != 3) launch_fused_f<PM, false>(h, P, p);  // (the piece form is factored)
+    }
+    template <int PM, bool FACT>
+    static void launch_fused_f(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
+        constexpr int WC = fused_wide_c<Acc, K, FACT>();
+        if constexpr (WC > 1) {
+            if (fused_cols(h, sizeof(T), K) == WC) return launch_fused_g<PM, FACT, WC>(h, P, p);
         }
-        launch_fused_c<PM, 1>(h, P, p);
+        launch_fused_g<PM, FACT, 1>(h, P, p);
+    }
+    template <int PM, bool FACT, int C>
+    static void launch_fused_g(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
+        const dim3 gr(P->fused_blocks), bl(kBlock);
+        const size_t lds = PM == 1 ? 0 : P->fused_lds;  // the sums-only pass has no few-level sums
+        if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, FACT, PM, C>), gr, bl, lds, h->stream, p);
+        else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2, FACT, PM, C>), gr, bl, lds, h->stream, p);
+        else if constexpr (C == 1 && PM != 3)  // any FE count (the piece form has two or three)
+            hipLaunchKernelGGL((k_fused<T, Acc, K, 0, FACT, PM>), gr, bl, lds, h->stream, p);
     }
 
     // u <- b*u + a*A v with the raw D'u sums of FE s and of the dense FEs.
@@ -2708,8 +2767,12 @@ struct Engine {
     // r = sqrt(w) .* y ; b = copy(r) (:51-55) and x = A'b (:56)
     static int start(fehip_solver *h, Plan *P, const double *dY, int Kreal, int64_t ldY, bool x_from_At) {
         const int64_t NK = h->N * K;
-        hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
-                           h->perm_s, h->sqrtw_int, h->N, (T *)P->r, (T *)P->u);
+        if constexpr (K >= 8)
+            hipLaunchKernelGGL(k_load_cols_t<T>, dim3(grid_for(h->N, kTileRows)), dim3(kBlock), 0, h->stream, dY, ldY,
+                               K, Kreal, h->pos_s, h->sqrtw_int, h->N, (T *)P->r, (T *)P->u);
+        else
+            hipLaunchKernelGGL(k_load_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K,
+                               Kreal, h->perm_s, h->sqrtw_int, h->N, (T *)P->r, (T *)P->u);
         if (h->fwl_now) {
             // projected problem: b <- (I - A_s A_s') b
             CHK(fwl_setup(h, P, Kreal));
@@ -2747,8 +2810,12 @@ struct Engine {
         hipLaunchKernelGGL(k_set_ops, dim3(1), dim3(64), 0, h->stream, P->ops_once, -1.0, 1.0, K, Kreal);
         db(h, P, MODE_NONE, P->ops_once, (const T *)P->x, (T *)P->r, nullptr);
         if (h->fwl_now) proj_s(h, P, (T *)P->r, MODE_NONE);  // r = b - A x with A projected
-        hipLaunchKernelGGL(k_store_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K, Kreal,
-                           h->perm_s, h->sqrtw_int, h->N, (const T *)P->r);
+        if constexpr (K >= 8)
+            hipLaunchKernelGGL(k_store_cols_t<T>, dim3(grid_for(h->N, kTileRows)), dim3(kBlock), 0, h->stream, dY, ldY,
+                               K, Kreal, h->pos_s, h->sqrtw_int, h->N, (const T *)P->r);
+        else
+            hipLaunchKernelGGL(k_store_cols<T>, dim3(grid_for(NK, kBlock)), dim3(kBlock), 0, h->stream, dY, ldY, K,
+                               Kreal, h->perm_s, h->sqrtw_int, h->N, (const T *)P->r);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(h->stream));
         return 0;
@@ -3978,7 +4045,7 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out) {
         out[5] = P->nslot;
         out[6] = P->pc.np;
         out[7] = P->csr_blocks;
-        out[8] = (wc > 1 && fused_cols(h, fp32_storage ? 4 : 8, wc > 1 ? (int)P->K : 1) == wc) ? wc : 1;
+        out[8] = wc;
         return 0;
     };
     const int Kp = padded_K(K);
@@ -3988,20 +4055,20 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out) {
                 using E = decltype(e);
                 Plan *P = nullptr;
                 CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
-                return fill(P, fused_wide_c<typename E::AccT, E::kK>());
+                return fill(P, fused_cols(h, sizeof(typename E::TT), E::kK));
             });
         return for_K<float, F32x2, double>(Kp, [&](auto e) {
             using E = decltype(e);
             Plan *P = nullptr;
             CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
-            return fill(P, fused_wide_c<typename E::AccT, E::kK>());
+            return fill(P, fused_cols(h, sizeof(typename E::TT), E::kK));
         });
     }
     return for_K<double, double, double>(Kp, [&](auto e) {
         using E = decltype(e);
         Plan *P = nullptr;
         CHK((get_plan_for<typename E::TT, typename E::AccT, E::kK>(h, &P)));
-        return fill(P, fused_wide_c<typename E::AccT, E::kK>());
+        return fill(P, fused_cols(h, sizeof(typename E::TT), E::kK));
     });
 }
 

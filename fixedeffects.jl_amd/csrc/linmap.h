@@ -797,22 +797,30 @@ __host__ __device__ constexpr bool dense_phased() {
 }
 
 // Columns per lane of the fused pass's wide lane map (k_fused, C): batches
-// of 16 or more columns (FUSED_WIDE_C in FP64, FUSED_WIDE_C32 in the FP32
-// modes); 1 = the plain map.  The fused pass then sums few-level FEs in
-// phased per-column FP64 accumulators whatever the precision.  The host uses
-// it unless FEHIP_WIDE=0 (fused_cols in fehip.hip applies the same rule).
-// measured at K = 32 (fused pass, N = 1e7): FP64 C = 2 with two wave steps
-// in flight 1.44 ms, C = 4 with one 1.59 ms; FP32 C = 4 1.07 ms, C = 2 1.30
-// ms; C = 8, or C = 4 with two steps, spill registers (2.8-6.4 ms)
+// of 16 or more columns; 1 = the plain map.  The fused pass then sums
+// few-level FEs in phased per-column FP64 accumulators whatever the
+// precision.  The host uses it unless FEHIP_WIDE=0 (fused_cols in fehip.hip
+// applies the same rule).  Measured on the fused pass (FP64, default
+// schedule): the factored piece form at K = 32 (N = 1e7) 1.91 ms with C = 1,
+// 1.44 ms with C = 2 and two wave steps in flight, 1.59 ms with C = 4; the
+// cache form at K = 16 (N = 1e8, an interacted FE) 11.97 ms with C = 2, 8.13
+// ms with C = 4; the FP32 streams at K = 32 1.30 ms with C = 2, 1.07 ms with
+// C = 4.  C = 8, or C = 4 with two steps, spill registers (2.8-6.4 ms).
 #ifndef FUSED_WIDE_C
 #define FUSED_WIDE_C 2
+#endif
+#ifndef FUSED_WIDE_C_CACHE
+#define FUSED_WIDE_C_CACHE 4
 #endif
 #ifndef FUSED_WIDE_C32
 #define FUSED_WIDE_C32 4
 #endif
-template <typename Acc, int K>
+template <typename Acc, int K, bool FACT>
 __host__ __device__ constexpr int fused_wide_c() {
-    return K < 16 ? 1 : std::is_same<Acc, double>::value ? FUSED_WIDE_C : FUSED_WIDE_C32;
+    return K < 16                              ? 1
+           : !std::is_same<Acc, double>::value ? FUSED_WIDE_C32
+           : FACT                              ? FUSED_WIDE_C
+                                               : FUSED_WIDE_C_CACHE;
 }
 
 // Lane-private few-level accumulators (the form that is not phased): each
@@ -1007,6 +1015,10 @@ struct FusedParams {
 #ifndef FUSED_U_C2
 #define FUSED_U_C2 2
 #endif
+// waves per SIMD the fused pass is compiled for (VGPR budget 512 / FUSED_WPE)
+#ifndef FUSED_WPE
+#define FUSED_WPE 4
+#endif
 #ifndef FUSED_U_C4
 #define FUSED_U_C4 1
 #endif
@@ -1054,7 +1066,7 @@ __host__ __device__ constexpr bool stores_scaled_u() {
 // batches of 16 or more columns with compile-time FE positions, whose
 // few-level sums are phased.
 template <typename T, typename Acc, int K, int GT, bool FACT, int PM = 0, int C = 1>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(FusedParams<T> p) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WPE))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
     extern __shared__ __align__(16) unsigned char fused_smem[];
     static_assert(K % C == 0, "C divides K");

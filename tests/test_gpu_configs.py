@@ -312,8 +312,8 @@ def _solve_wide(p, K, wide, algorithm="lsmr", tol=1e-10, prec="fp64"):
 @pytest.mark.parametrize("prec,form", [("fp64", "pieces"), ("fp64", "cache"), ("fp64", "projected"),
                                        ("mixed", "pieces"), ("mixed", "cache"), ("fp32", "pieces")])
 def test_wide_lane_map_matches_plain(gpu, K, prec, form):
-    # batches of 16 or 32 columns: the fused pass with 2 (FP64) or 4 (FP32
-    # streams) columns per lane against the plain map (one column per lane) on the same panel, and the
+    # batches of 16 or 32 columns: the fused pass with 2 or 4 columns per
+    # lane against the plain map (one column per lane) on the same panel, and the
     # FP64 and mixed solves against the oracle's converged residual of the
     # first columns.  Forms: the factored piece form, the cache form (an
     # interacted FE), the projected iteration (sums-only and corrected
@@ -330,7 +330,9 @@ def test_wide_lane_map_matches_plain(gpu, K, prec, form):
     ip, rp, itp, cp = _solve_wide(p, K, False, algorithm, tol, prec)
     _say(f"  N={p.N} lanes {iw['fused_cols_per_lane']} / {ip['fused_cols_per_lane']} pieces {iw['pieces']} "
          f"iters {itw[:4]} / {itp[:4]}")
-    assert iw["fused_cols_per_lane"] == (2 if prec == "fp64" else 4) and ip["fused_cols_per_lane"] == 1
+    # 2 columns per lane for FP64 factored forms, 4 for the cache form and the FP32 streams
+    want = 2 if (prec == "fp64" and form != "cache") else 4
+    assert iw["fused_cols_per_lane"] == want and ip["fused_cols_per_lane"] == 1
     if form != "projected":  # (the projected iteration keeps the CSR pass whatever the plan holds)
         assert (iw["pieces"] > 0) == (form == "pieces")
     assert all(cw) and all(cp)
