@@ -346,7 +346,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--pcie-steps", type=int, default=2, help="timed solves through the host-buffer entry point")
     ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-sized default schedule")
-    ap.add_argument("--panels", type=int, default=64, help="panels of the uneven_panels workload")
+    ap.add_argument("--panels", type=int, default=256, help="panels of the uneven_panels workload")
     ap.add_argument("--alt-steps", type=int, default=3, help="timed steps of the projected LSMR beside `value`")
     ap.add_argument("--leg-steps", type=int, default=2, help="timed steps of each 32-column leg (0: no legs)")
     ap.add_argument("--algorithm", default="lsmr", choices=["lsmr", "projected"],

@@ -142,6 +142,9 @@ struct fehip_solver {
     // FP64 batches of 16 or more columns: the fused pass in the wide lane
     // map (several adjacent columns per lane, FUSED_WIDE_C)
     bool wide = true;
+    // the projected iteration sums FE t from the piece form's sums where the
+    // plan has one (FEHIP_PIECES_PROJ=0: the CSR pass)
+    bool pieces_proj = true;
     // schedule overrides (fehip_set_schedule / fehip_autotune); 0 or -1 =
     // the default (environment, else sized by occupancy)
     int64_t sched_tiles_s = 0, sched_tiles_c = 0;
@@ -700,6 +703,7 @@ static int create_impl(fehip_t **out, int64_t N, int32_t G, const R *const *refs
     h->factored = env_int("FEHIP_FACTORED", 1, 0, 1) != 0;
     h->seg_fix = env_int("FEHIP_SEG_FIX", 1, 0, 1) != 0;
     h->wide = env_int("FEHIP_WIDE", 1, 0, 1) != 0;
+    h->pieces_proj = env_int("FEHIP_PIECES_PROJ", 1, 0, 1) != 0;
     for (int g = 0; g < G; g++)
         if (interaction && interaction[g]) h->factored = false;
     if (const char *e = std::getenv("FEHIP_MIXED_TOL")) h->mixed_tol = std::atof(e);
@@ -955,6 +959,11 @@ struct PieceForm {
     // without observations; part 2 (after it): levels whose pieces cross tiles
     FixItem *items1 = nullptr, *items2 = nullptr;
     int32_t n1 = 0, n2 = 0;
+    // the projected iteration (FE s eliminated) runs FE t's tiles only: its
+    // parts 1 and 2 without FE s's items
+    FixItem *items1p = nullptr, *items2p = nullptr;
+    int32_t n1p = 0, n2p = 0;
+    unsigned fix1p = 1, fix2p = 1;
     void *psum = nullptr;            // np x K, T
 };
 
@@ -1389,6 +1398,22 @@ static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<F
         HIPCHK(hipMemcpyAsync(pc.items1, it1.data(), sizeof(FixItem) * pc.n1, hipMemcpyHostToDevice, h->stream));
     if (pc.n2)
         HIPCHK(hipMemcpyAsync(pc.items2, it2.data(), sizeof(FixItem) * pc.n2, hipMemcpyHostToDevice, h->stream));
+    // the projected iteration's lists: the same items without FE s's
+    std::vector<FixItem> it1p, it2p;
+    for (const FixItem &fi : it1)
+        if (fi.fe != s) it1p.push_back(fi);
+    for (const FixItem &fi : it2)
+        if (fi.fe != s) it2p.push_back(fi);
+    pc.n1p = (int32_t)it1p.size();
+    pc.n2p = (int32_t)it2p.size();
+    pc.fix1p = (unsigned)std::min<int64_t>(grid_for((int64_t)it1p.size(), kWavesPerBlock), 1024);
+    pc.fix2p = (unsigned)std::min<int64_t>(grid_for((int64_t)it2p.size(), kWavesPerBlock), 1024);
+    CHK(dalloc(h, &pc.items1p, std::max<int32_t>(pc.n1p, 1)));
+    CHK(dalloc(h, &pc.items2p, std::max<int32_t>(pc.n2p, 1)));
+    if (pc.n1p)
+        HIPCHK(hipMemcpyAsync(pc.items1p, it1p.data(), sizeof(FixItem) * pc.n1p, hipMemcpyHostToDevice, h->stream));
+    if (pc.n2p)
+        HIPCHK(hipMemcpyAsync(pc.items2p, it2p.data(), sizeof(FixItem) * pc.n2p, hipMemcpyHostToDevice, h->stream));
     CHK(dalloc(h, &pc.tileA, (size_t)std::max<int32_t>(t0, 1) * P->K));
     CHK(dalloc(h, &pc.tileB, (size_t)std::max<int32_t>(t0, 1) * P->K));
     CHK(dalloc(h, &pc.part, (size_t)(pc.fix1 + pc.blocks + pc.fix2) * P->K));
@@ -1400,7 +1425,7 @@ static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<F
 static void destroy_pieces(fehip_solver *h, PieceForm &pc) {
     void *ptrs[] = {pc.heads, pc.wprefix, pc.piece_s, pc.pt_perm, pc.pt_lev, pc.fs.heads, pc.fs.wprefix, pc.fs.seg_level,
                     pc.ft.heads, pc.ft.wprefix, pc.ft.seg_level, pc.tstart, pc.tileA, pc.tileB, pc.part,
-                    pc.items1, pc.items2, pc.psum};
+                    pc.items1, pc.items2, pc.items1p, pc.items2p, pc.psum};
     for (void *q : ptrs) dfree(h, q);
     pc = PieceForm{};
 }
@@ -1912,15 +1937,21 @@ struct Engine {
     static bool ut(const fehip_solver *h) { return h->factored && stores_scaled_u<K, true>(); }
     // the iteration runs in the piece form (plain LSMR on a plan that has it)
     static bool use_pieces(const fehip_solver *h, const Plan *P) { return P->pc.np > 0 && !h->fwl_now; }
-    // ... and stores u scaled (never in the piece form)
-    static bool ut_now(const fehip_solver *h, const Plan *P) { return ut(h) && !use_pieces(h, P); }
+    // the projected iteration runs in the piece form (its FE t half)
+    static bool use_pieces_proj(const fehip_solver *h, const Plan *P) {
+        return P->pc.np > 0 && h->fwl_now && h->pieces_proj;
+    }
+    // ... and stores u scaled (never in the piece forms)
+    static bool ut_now(const fehip_solver *h, const Plan *P) {
+        return ut(h) && !use_pieces(h, P) && !use_pieces_proj(h, P);
+    }
 
     // the fused pass on this handle: factored or cache form, then its lane
     // map (fused_cols / fused_wide_c: the columns per lane), then the FE count
     template <int PM>
     static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         if (h->factored) launch_fused_f<PM, true>(h, P, p);
-        else if constexpr (PM != 3) launch_fused_f<PM, false>(h, P, p);  // (the piece form is factored)
+        else if constexpr (PM < 3) launch_fused_f<PM, false>(h, P, p);  // (the piece forms are factored)
     }
     template <int PM, bool FACT>
     static void launch_fused_f(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
@@ -1936,7 +1967,7 @@ struct Engine {
         const size_t lds = PM == 1 ? 0 : P->fused_lds;  // the sums-only pass has no few-level sums
         if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, FACT, PM, C>), gr, bl, lds, h->stream, p);
         else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2, FACT, PM, C>), gr, bl, lds, h->stream, p);
-        else if constexpr (C == 1 && PM != 3)  // any FE count (the piece form has two or three)
+        else if constexpr (C == 1 && PM < 3)  // any FE count (the piece forms have two or three)
             hipLaunchKernelGGL((k_fused<T, Acc, K, 0, FACT, PM>), gr, bl, lds, h->stream, p);
     }
 
@@ -2005,14 +2036,19 @@ struct Engine {
             p.zsw = (T *)P->zs;
             p.gsrc[0] = (const T *)(h->factored ? h->scale2_T[pidx<T>()] : h->fe[h->s].scale_T[pidx<T>()]);
             launch_fused<1>(h, P, p);
-        } else if (pm == 3) {
+        } else if (pm == 3 || pm == 4) {
             const PieceForm &pc = P->pc;
             p.heads = pc.heads;
             p.wprefix = pc.wprefix;
             p.seg_level = pc.piece_s;
             p.nsegs_s = (int32_t)pc.np;
             p.psum = (T *)pc.psum;
-            launch_fused<3>(h, P, p);
+            if (pm == 4) {  // the projected u pass: FE s's correction at its level
+                p.gsrc[0] = (const T *)P->zs;
+                launch_fused<4>(h, P, p);
+            } else {
+                launch_fused<3>(h, P, p);
+            }
         } else if (pm == 2) {
             p.gsrc[0] = (const T *)P->zs;
             launch_fused<2>(h, P, p);
@@ -2220,12 +2256,12 @@ struct Engine {
     // FE t (its pieces through pt_perm) as segmented sums of the piece sums,
     // each level's sum taking its FE's scale (RAW)
     static DtxSegParams<T> piece_seg_params(fehip_solver *h, Plan *P, const ColOps *ops, T *coef,
-                                            const int32_t *early_exit) {
+                                            const int32_t *early_exit, bool proj = false) {
         const PieceForm &pc = P->pc;
         DtxSegParams<T> p{};
         p.N = pc.np;
         p.nseg = 2;
-        p.first_tile = 0;
+        p.first_tile = proj ? pc.tile0_t : 0;  // projected: FE t's tiles only
         p.total_tiles = pc.total_tiles;
         for (int q = 0; q < 2; q++) {
             const int g = q == 0 ? h->s : h->t_sec;
@@ -2249,7 +2285,7 @@ struct Engine {
         p.tileA = pc.tileA;
         p.tileB = pc.tileB;
         p.ops = ops;
-        p.partials = pc.part + (size_t)pc.fix1 * K;
+        p.partials = pc.part + (size_t)(proj ? pc.fix1p : pc.fix1) * K;
         p.all_done = early_exit;
         p.mode = MODE_ITER;
         return p;
@@ -2258,17 +2294,17 @@ struct Engine {
     // part 1: few-level FEs' levels and levels without observations (first
     // blocks of the piece launch); part 2: levels whose pieces cross tiles
     static FixParams<T> piece_fix_params(fehip_solver *h, Plan *P, const ColOps *ops, T *coef,
-                                         const int32_t *early_exit, int part) {
+                                         const int32_t *early_exit, int part, bool proj = false) {
         const PieceForm &pc = P->pc;
         FixParams<T> f{};
-        f.nitems = part == 1 ? pc.n1 : pc.n2;
-        f.items = part == 1 ? pc.items1 : pc.items2;
+        f.nitems = part == 1 ? (proj ? pc.n1p : pc.n1) : (proj ? pc.n2p : pc.n2);
+        f.items = part == 1 ? (proj ? pc.items1p : pc.items1) : (proj ? pc.items2p : pc.items2);
         f.tileA = pc.tileA;
         f.tileB = pc.tileB;
         f.dpart = P->dpart;
         f.v = coef;
         f.ops = ops;
-        f.partials = pc.part + (part == 2 ? (size_t)(pc.fix1 + pc.blocks) * K : 0);
+        f.partials = pc.part + (part == 2 ? (size_t)((proj ? pc.fix1p : pc.fix1) + pc.blocks) * K : 0);
         f.all_done = early_exit;
         f.mode = MODE_ITER;
         // raw sums of sqrt(w) .* u take their FE's scale here: the pieces' FEs
@@ -2280,24 +2316,28 @@ struct Engine {
         return f;
     }
 
-    static void piece_dtx(fehip_solver *h, Plan *P, const ColOps *ops, T *coef, const int32_t *early_exit) {
-        const DtxSegParams<T> p = piece_seg_params(h, P, ops, coef, early_exit);
-        const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 1);
-        const int nfix = (int)P->pc.fix1;
-        hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true, true>), dim3(P->pc.fix1 + P->pc.blocks), dim3(kBlock), 0,
-                           h->stream, p, f, nfix);
+    static void piece_dtx(fehip_solver *h, Plan *P, const ColOps *ops, T *coef, const int32_t *early_exit,
+                          bool proj = false) {
+        const DtxSegParams<T> p = piece_seg_params(h, P, ops, coef, early_exit, proj);
+        const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 1, proj);
+        const unsigned nfix = proj ? P->pc.fix1p : P->pc.fix1;
+        hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true, true>), dim3(nfix + P->pc.blocks), dim3(kBlock), 0,
+                           h->stream, p, f, (int)nfix);
     }
 
-    static void piece_fix2(fehip_solver *h, Plan *P, const ColOps *ops, T *coef, const int32_t *early_exit) {
-        const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 2);
-        hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(P->pc.fix2), dim3(kBlock), 0, h->stream, f);
+    static void piece_fix2(fehip_solver *h, Plan *P, const ColOps *ops, T *coef, const int32_t *early_exit,
+                           bool proj = false) {
+        const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 2, proj);
+        hipLaunchKernelGGL((k_dtx_fixup<T, K>), dim3(proj ? P->pc.fix2p : P->pc.fix2), dim3(kBlock), 0, h->stream, f);
     }
 
-    static void piece_fin_dtx(fehip_solver *h, Plan *P) {
+    static void piece_fin_dtx(fehip_solver *h, Plan *P, bool proj = false) {
         const PieceForm &pc = P->pc;
-        hipLaunchKernelGGL((k_fin_dtx<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream, (const double *)pc.part,
-                           (int64_t)(pc.fix1 + pc.blocks + (pc.n2 ? pc.fix2 : 0)), (const double *)nullptr,
-                           (int64_t)0, MODE_ITER, P->state, P->ops_db, P->all_done, (const int32_t *)P->all_done);
+        const int64_t n = proj ? pc.fix1p + pc.blocks + (pc.n2p ? pc.fix2p : 0)
+                               : pc.fix1 + pc.blocks + (pc.n2 ? pc.fix2 : 0);
+        hipLaunchKernelGGL((k_fin_dtx<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream, (const double *)pc.part, n,
+                           (const double *)nullptr, (int64_t)0, MODE_ITER, P->state, P->ops_db, P->all_done,
+                           (const int32_t *)P->all_done);
     }
 
     // one LSMR iteration in the piece form
@@ -2508,6 +2548,21 @@ struct Engine {
     // beta, D'u of the CSR FEs, the fix-up, alpha and the rotations, the
     // vector update, the stopping rules
     static void iteration(fehip_solver *h, Plan *P) {
+        if (h->fwl_now && use_pieces_proj(h, P)) {
+            // projected, piece form: the u pass leaves the piece sums; FE t's
+            // level sums come from them (FE s's coefficients stay zero)
+            fused(h, P, P->ops_one, (const T *)P->v, (T *)P->u, 1);
+            fix_s(h, P);
+            fused(h, P, P->ops_db, (const T *)P->v, (T *)P->u, 4);
+            fin_db(h, P, MODE_ITER, P->all_done, true);
+            if (!P->fused_dense) dtx_dense(h, P, P->ops_dtx, (const T *)P->u, P->all_done, false);
+            piece_dtx(h, P, P->ops_dtx, (T *)P->v, P->all_done, true);
+            if (P->pc.n2p) piece_fix2(h, P, P->ops_dtx, (T *)P->v, P->all_done, true);
+            piece_fin_dtx(h, P, true);
+            update(h, P);
+            fin_update(h, P);
+            return;
+        }
         if (h->fwl_now) {
             // projected: A = (I - A_s A_s') A_o, coefficients of FE s zero
             fused(h, P, P->ops_one, (const T *)P->v, (T *)P->u, 1);

@@ -1061,7 +1061,9 @@ __host__ __device__ constexpr bool stores_scaled_u() {
 // store or norm, no dense sums; 2 = the u pass without FE s segment sums.
 // PM = 3: the plain pass in the piece form (factored, GT > 0): the segments
 // are pieces, every tile starts one, and each piece's raw sum goes to psum
-// (its ordinal in position order); u is stored as it is.
+// (its ordinal in position order); u is stored as it is.  PM = 4: the
+// projected iteration's u pass (as PM = 2) in the piece form (sums as PM = 3,
+// for the FE t half of the piece launch).
 // C (columns per lane, wide lane map, seg_step_c): 1, or 2/4 for FP64
 // batches of 16 or more columns with compile-time FE positions, whose
 // few-level sums are phased.
@@ -1104,8 +1106,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
     // of a column add to in P ordered phases with LDS adds -- a quarter to a
     // half of the LDS
     constexpr bool PHASED = dense_phased<K, Acc>() || C > 1;
-    constexpr bool UT = stores_scaled_u<K, FACT>() && PM != 3;
-    static_assert(PM != 3 || (GT > 0 && FACT), "the piece form is a factored pass with compile-time FE positions");
+    constexpr bool PIECES = PM == 3 || PM == 4;
+    constexpr bool UT = stores_scaled_u<K, FACT>() && !PIECES;
+    static_assert(!PIECES || (GT > 0 && FACT), "the piece form is a factored pass with compile-time FE positions");
     static_assert(C == 1 || (GT > 0 && !UT), "the wide lane map: compile-time FE positions, u stored as it is");
     // the wide map's phased accumulators are FP64 in every precision (the
     // exact products of floats summed in doubles): measured on MI355X, LDS
@@ -1246,7 +1249,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
                         for (int j = 0; j < C; j++)
                             z.v[j] = (T)(-(FACT ? (double)st.g0[q] : 1.0) * acc_to_double(x[j]));
                         stv<C>(p.zsw, e, z);
-                    } else if constexpr (PM == 3) {
+                    } else if constexpr (PIECES) {
                         VT z;
 #pragma unroll
                         for (int j = 0; j < C; j++) z.v[j] = acc_to_T<T, SA>(x[j]);
@@ -1299,7 +1302,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
             //   compute step 0
             struct Refs {
                 int32_t r[GA][U];
-                int32_t o[U];  // PM = 3: the piece ordinal of each position
+                int32_t o[U];  // the piece forms: the piece ordinal of each position
                 uint64_t hw[NW];
                 uint32_t wp[NW];
             };
@@ -1361,7 +1364,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
                 }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    d.rs[q] = PM == 3 ? rr.o[q] : rr.r[0][q];  // where a segment ending here is written
+                    d.rs[q] = PIECES ? rr.o[q] : rr.r[0][q];  // where a segment ending here is written
                     d.rd[q] = rr.r[GT - 1][q];
                 }
 #pragma unroll
@@ -1498,8 +1501,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
                 compute(st, lb);
             }
         }
-        // (PM = 3: every piece ends inside its tile)
-        if (PM != 2 && PM != 3 && cur.open && anyon && psub == 0) {
+        // (the piece forms: every piece ends inside its tile)
+        if (PM != 2 && !PIECES && cur.open && anyon && psub == 0) {
 #pragma unroll
             for (int j = 0; j < C; j++) {
                 if (cur.cont) p.tileA[(int64_t)t * K + c0 + j] = acc_to_double(cur.carry[j]);
