@@ -2321,8 +2321,13 @@ struct Engine {
         const DtxSegParams<T> p = piece_seg_params(h, P, ops, coef, early_exit, proj);
         const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 1, proj);
         const unsigned nfix = proj ? P->pc.fix1p : P->pc.fix1;
-        hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true, true>), dim3(nfix + P->pc.blocks), dim3(kBlock), 0,
-                           h->stream, p, f, (int)nfix);
+        constexpr int WC = fused_wide_c<Acc, K, true>();  // the fused pass's factored width
+        if (WC > 1 && fused_cols(h, sizeof(T), K) == WC)
+            hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true, true, WC>), dim3(nfix + P->pc.blocks), dim3(kBlock),
+                               0, h->stream, p, f, (int)nfix);
+        else
+            hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true, true>), dim3(nfix + P->pc.blocks), dim3(kBlock), 0,
+                               h->stream, p, f, (int)nfix);
     }
 
     static void piece_fix2(fehip_solver *h, Plan *P, const ColOps *ops, T *coef, const int32_t *early_exit,

@@ -1678,10 +1678,158 @@ __global__ __launch_bounds__(kBlock) void k_dtx_fixup(FixParams<T> p) {
 // CSR tiles.  The fix-up's short dependent loads then overlap the CSR
 // pass's stream instead of running after it.  The two parts touch disjoint
 // levels of v and leave their norm partials in separate arrays.
-template <typename T, typename Acc, int K, bool RAW, bool PLEV = false>
+// The piece launch's tiles in the wide lane map (C adjacent columns per
+// lane, KL = K/C lanes and P = 64/KL positions per wave step; seg_step_c):
+// each position's order entry, level, head bits and scale are loaded once
+// for C columns, the piece sums and each level's v move as C-element
+// vectors.  RAW and PLEV (the piece form) only.  The sums of the FP32
+// streams' piece sums are FP64 (exact for floats).  A level's v is written
+// once (in the tile where its pieces end, when they do not cross tiles); a
+// column that is off keeps its old value.
+template <typename T, typename Acc, int K, int C>
+__device__ __forceinline__ void dtx_seg_body_wide(const DtxSegParams<T> &p, int bid, int nblk) {
+    constexpr int KL = K / C;
+    constexpr int P = kWave / KL;
+    constexpr int U = 2;  // wave steps whose loads are issued together
+    using SA = typename std::conditional<std::is_same<Acc, F32x2>::value, double, Acc>::type;
+    using VT = VecC<T, C>;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int psub = lane / KL, cg = lane % KL, c0 = cg * C;
+    bool on[C], keep[C];
+    T a[C], b[C];
+    bool anyon = false;
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+        const double bj = p.ops->b[c0 + j];
+        on[j] = p.ops->on[c0 + j] != 0;
+        a[j] = (T)p.ops->a[c0 + j];
+        b[j] = (T)bj;
+        keep[j] = bj != 0.0;
+        anyon = anyon || on[j];
+    }
+    const int gw = bid * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
+    const int nw = nblk * kWavesPerBlock;
+    double sq[C];
+#pragma unroll
+    for (int j = 0; j < C; j++) sq[j] = 0.0;
+    for (int t = p.first_tile + gw; t < p.total_tiles; t += nw) {
+        int f = 0;
+        while (f + 1 < p.nseg && t >= p.fe[f + 1].tile0) f++;
+        const SegFE<T> fe = p.fe[f];
+        const int64_t start = fe.tstart[t - fe.tile0];
+        const int64_t end = fe.tstart[t - fe.tile0 + 1];
+        const int64_t wbeg = start & ~(int64_t)63;
+        auto clip = [&](uint64_t w, int64_t wi) -> uint64_t {
+            const int64_t lo = start - (wi << 6), hi = end - (wi << 6);
+            if (hi < 0 || lo > 63) return 0ull;
+            uint64_t m = lo > 0 ? ~0ull << lo : ~0ull;
+            if (hi < 63) m &= (2ull << hi) - 1ull;
+            return w & m;
+        };
+        T *v = p.v + fe.coef_off * K;
+        SegCursorC<SA, C> cur;
+#pragma unroll
+        for (int j = 0; j < C; j++) cur.carry[j] = SA(0.f);
+        cur.cont = true;
+        cur.open = false;
+        int32_t in[U], lvn[U];
+        uint64_t hwn[U], hnn[U];
+        auto meta = [&](int64_t b0) {
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t base = b0 + q * P;
+                const int64_t pos = base + psub;
+                const int64_t pc = (pos >= start && pos < end) ? pos : start;
+                in[q] = __builtin_nontemporal_load(fe.perm + pc);
+                lvn[q] = fe.plev[pc];
+                hwn[q] = hnn[q] = 0;
+                if (base < end) {  // wave-uniform
+                    hwn[q] = fe.heads[base >> 6];
+                    hnn[q] = fe.heads[(base >> 6) + 1];
+                }
+            }
+        };
+        meta(wbeg);
+        for (int64_t base0 = wbeg; base0 < end; base0 += U * P) {
+            int32_t iq[U], lev[U];
+            uint64_t hw[U], hn[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                iq[q] = in[q];
+                lev[q] = lvn[q];
+                hw[q] = hwn[q];
+                hn[q] = hnn[q];
+            }
+            // unconditional loads (positions past the tile read its first
+            // position): every load of the step is in flight together
+            T cv[U];
+            VT uv[U], ov[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                cv[q] = fe.lscale[lev[q]];
+                uv[q] = ldv<C>(p.u, (uint32_t)iq[q] * K + c0);
+                ov[q] = ldv<C>(v, (uint32_t)lev[q] * K + c0);
+            }
+            meta(base0 + U * P);
+            SA val[U][C];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t pos = base0 + q * P + psub;
+                const bool inb = pos >= start && pos < end;
+#pragma unroll
+                for (int j = 0; j < C; j++) val[q][j] = (inb && on[j]) ? SA(uv[q].v[j]) : SA(0.f);
+            }
+            bool em[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) em[q] = false;
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const int64_t base = base0 + q * P;
+                if (base >= end) break;  // wave-uniform
+                const int64_t wi = base >> 6;
+                em[q] = seg_step_c<KL, C>(clip(hw[q], wi), clip(hn[q], wi + 1), (int)(base & 63),
+                                          base + psub >= start && base + psub < end, psub, cg, anyon, val[q], cur,
+                                          &p.tileA[(int64_t)t * K + c0]);
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                if (em[q]) {
+                    VT o;
+#pragma unroll
+                    for (int j = 0; j < C; j++) {
+                        T out = a[j] * (cv[q] * acc_to_T<T, SA>(val[q][j]));
+                        if (keep[j]) out = b[j] * ov[q].v[j] + out;
+                        o.v[j] = on[j] ? out : ov[q].v[j];
+                        if (on[j]) sq[j] += (double)out * (double)out;
+                    }
+                    stv<C>(v, (uint32_t)lev[q] * K + c0, o);
+                }
+            }
+        }
+        if (cur.open && anyon && psub == 0) {
+#pragma unroll
+            for (int j = 0; j < C; j++) {
+                if (cur.cont) p.tileA[(int64_t)t * K + c0 + j] = acc_to_double(cur.carry[j]);
+                else p.tileB[(int64_t)t * K + c0 + j] = acc_to_double(cur.carry[j]);
+            }
+        }
+    }
+    if (p.mode == MODE_NONE) return;
+    __shared__ double red[kBlock];
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+        const double s = block_column_sum<KL>(sq[j], red);
+        if (threadIdx.x < KL) p.partials[(int64_t)bid * K + threadIdx.x * C + j] = s;
+    }
+}
+
+// C > 1: the tiles in the wide lane map (the piece form)
+template <typename T, typename Acc, int K, bool RAW, bool PLEV = false, int C = 1>
 __global__ __launch_bounds__(kBlock) void k_dtx_seg_fix(DtxSegParams<T> p, FixParams<T> f, int nfix) {
+    static_assert(C == 1 || (RAW && PLEV), "the wide lane map serves the piece launch");
     if (p.all_done && *p.all_done) return;
     if ((int)blockIdx.x < nfix) dtx_fixup_body<T, K>(f, blockIdx.x, nfix);
+    else if constexpr (C > 1) dtx_seg_body_wide<T, Acc, K, C>(p, (int)blockIdx.x - nfix, (int)gridDim.x - nfix);
     else dtx_seg_body<T, Acc, K, RAW, PLEV>(p, (int)blockIdx.x - nfix, (int)gridDim.x - nfix);
 }
 
