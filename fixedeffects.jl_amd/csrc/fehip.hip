@@ -1280,7 +1280,10 @@ static int build_pieces(fehip_solver *h, Plan *P, size_t ts, const std::vector<F
     PieceForm &pc = P->pc;
     const int s = h->s, t = h->t_sec;
     const int64_t N = h->N;
-    if (!h->factored || t < 0 || h->fe[t].kind != KIND_CSR || h->G > 3 || env_int("FEHIP_PIECES", 1, 0, 1) == 0)
+    // FE s and FE t without interactions (a piece's sum of sqrt(w) .* u then
+    // serves both); other FEs may carry one (the cache form)
+    if (t < 0 || h->fe[t].kind != KIND_CSR || h->G > 3 || h->fe[s].z_orig || h->fe[t].z_orig ||
+        env_int("FEHIP_PIECES", 1, 0, 1) == 0)
         return 0;
     for (int g = 0; g < h->G; g++)
         if (g != t && h->fe[g].kind == KIND_CSR) return 0;
@@ -1939,7 +1942,7 @@ struct Engine {
     static bool use_pieces(const fehip_solver *h, const Plan *P) { return P->pc.np > 0 && !h->fwl_now; }
     // the projected iteration runs in the piece form (its FE t half)
     static bool use_pieces_proj(const fehip_solver *h, const Plan *P) {
-        return P->pc.np > 0 && h->fwl_now && h->pieces_proj;
+        return P->pc.np > 0 && h->fwl_now && h->pieces_proj && h->factored;
     }
     // ... and stores u scaled (never in the piece forms)
     static bool ut_now(const fehip_solver *h, const Plan *P) {
@@ -1951,7 +1954,7 @@ struct Engine {
     template <int PM>
     static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         if (h->factored) launch_fused_f<PM, true>(h, P, p);
-        else if constexpr (PM < 3) launch_fused_f<PM, false>(h, P, p);  // (the piece forms are factored)
+        else if constexpr (PM != 4) launch_fused_f<PM, false>(h, P, p);  // (the projected piece form is factored)
     }
     template <int PM, bool FACT>
     static void launch_fused_f(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
@@ -2307,10 +2310,12 @@ struct Engine {
         f.partials = pc.part + (part == 2 ? (size_t)((proj ? pc.fix1p : pc.fix1) + pc.blocks) * K : 0);
         f.all_done = early_exit;
         f.mode = MODE_ITER;
-        // raw sums of sqrt(w) .* u take their FE's scale here: the pieces' FEs
-        // and the few-level FEs summed in the fused pass
+        // raw sums of sqrt(w) .* u take their FE's scale here: the pieces' FEs,
+        // and the few-level FEs summed in a factored fused pass (the cache
+        // form's few-level sums carry their caches)
         for (int g = 0; g < h->G; g++) {
-            const bool raw = g == h->s || g == h->t_sec || (h->fe[g].kind == KIND_DENSE && P->fused_dense);
+            const bool raw = g == h->s || g == h->t_sec ||
+                             (h->fe[g].kind == KIND_DENSE && P->fused_dense && h->factored);
             if (raw) f.fscale[g] = (const T *)h->fe[g].scale_T[pidx<T>()];
         }
         return f;

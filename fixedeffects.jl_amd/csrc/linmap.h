@@ -1059,9 +1059,10 @@ __host__ __device__ constexpr bool stores_scaled_u() {
 // PM (projected iteration, FE s eliminated): 0 = plain pass; 1 = sums only:
 // the FE s segment sums of cache_s * (D*v over the other FEs), no u load,
 // store or norm, no dense sums; 2 = the u pass without FE s segment sums.
-// PM = 3: the plain pass in the piece form (factored, GT > 0): the segments
-// are pieces, every tile starts one, and each piece's raw sum goes to psum
-// (its ordinal in position order); u is stored as it is.  PM = 4: the
+// PM = 3: the plain pass in the piece form (GT > 0; factored, or the cache
+// form with FE s and FE t not interacted): the segments are pieces, every
+// tile starts one, and each piece's raw sum of sqrt(w) .* u goes to psum (its
+// ordinal in position order); u is stored as it is.  PM = 4: the
 // projected iteration's u pass (as PM = 2) in the piece form (sums as PM = 3,
 // for the FE t half of the piece launch).
 // C (columns per lane, wide lane map, seg_step_c): 1, or 2/4 for FP64
@@ -1108,7 +1109,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
     constexpr bool PHASED = dense_phased<K, Acc>() || C > 1;
     constexpr bool PIECES = PM == 3 || PM == 4;
     constexpr bool UT = stores_scaled_u<K, FACT>() && !PIECES;
-    static_assert(!PIECES || (GT > 0 && FACT), "the piece form is a factored pass with compile-time FE positions");
+    static_assert(!PIECES || GT > 0, "the piece forms have compile-time FE positions");
+    static_assert(PM != 4 || FACT, "the projected piece form is factored");
     static_assert(C == 1 || (GT > 0 && !UT), "the wide lane map: compile-time FE positions, u stored as it is");
     // the wide map's phased accumulators are FP64 in every precision (the
     // exact products of floats summed in doubles): measured on MI355X, LDS
@@ -1147,7 +1149,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
         const uint32_t len = (uint32_t)(end - start);
         // tile-relative bases; positions below are offsets from start
         T *ub = p.target + start * K;
-        const T *swb = (FACT && p.sqrtw) ? p.sqrtw + start : nullptr;
+        // sqrt(w): the factored form's multiplier, and the piece sums' in the
+        // cache form (FE s and FE t carry no interaction there, so a piece's
+        // sum of sqrt(w) .* u serves both, each scaled later)
+        const T *swb = ((FACT || PIECES) && p.sqrtw) ? p.sqrtw + start : nullptr;
         SegCursorC<SA, C> cur;
 #pragma unroll
         for (int j = 0; j < C; j++) cur.carry[j] = SA(0.f);
@@ -1348,7 +1353,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
                         d.uo[q] = ldv<C>(ub, i * K + c0);
                     }
                     d.sw[q] = (T)1;
-                    if constexpr (FACT) {
+                    if constexpr (FACT || PIECES) {
                         if (swb) d.sw[q] = ldo(swb, i);  // wave-uniform test
                     }
                 }
@@ -1407,7 +1412,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
                                 st.sum[q][j] += (PM == 1 && g == 0) ? (T)0 : d.cv[g][q] * d.gv[g][q].v[j];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
-                        st.cs[q] = d.cv[0][q];
+                        st.cs[q] = PIECES ? d.sw[q] : d.cv[0][q];
                         if (p.ndense) st.cd[q] = d.cv[GT - 1][q];
                     }
                 }

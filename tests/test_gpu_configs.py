@@ -333,8 +333,9 @@ def test_wide_lane_map_matches_plain(gpu, K, prec, form):
     # 2 columns per lane for FP64 factored forms, 4 for the cache form and the FP32 streams
     want = 2 if (prec == "fp64" and form != "cache") else 4
     assert iw["fused_cols_per_lane"] == want and ip["fused_cols_per_lane"] == 1
-    if form != "projected":  # (the projected iteration keeps the CSR pass whatever the plan holds)
-        assert (iw["pieces"] > 0) == (form == "pieces")
+    # the piece form applies to both panels (the cache form's interacted FE
+    # is the few-level one)
+    assert iw["pieces"] > 0
     assert all(cw) and all(cp)
     worst = max(_rel(a, b) for a, b in zip(rw, rp))
     _say(f"  max rel diff wide vs plain {worst:.2e}")
@@ -351,3 +352,38 @@ def test_wide_lane_map_matches_plain(gpu, K, prec, form):
     for c in range(min(2, p.Y.shape[1])):
         ro = o.solve_residuals(p.Y[:, c].copy(), tol=1e-12, maxiter=100_000)[0]
         assert _rel(rw[c], ro) <= RES_RTOL, (c, _rel(rw[c], ro))
+
+
+# -- device columns with a leading dimension beyond N, K below its padding ------------
+
+@pytest.mark.parametrize("K", [1, 20])
+def test_device_columns_with_gap(gpu, K):
+    # fehip_solve_residuals_device on a column-major block whose leading
+    # dimension exceeds N (ld = N + 13) and, at K = 20, fewer columns than the
+    # padded batch (32: the wide lane map and the tiled loads and stores of
+    # the caller's columns): the residuals equal, bit for bit, those of the
+    # same columns packed (ld = N), the gap between columns is left as it was,
+    # and the first columns are within 1e-8 of the oracle's converged residual
+    _say(f"case: device columns with a gap K={K}")
+    p = synth.config("worker_firm_1e7_k32", scale=0.01)
+    N = p.N
+    Y = np.ascontiguousarray(p.Y[:, :K])
+    ld = N + 13
+    gap = np.full((ld - N,), 7.25)
+    with api.FixedEffectSolverROCm(_fes(p), p.weights, "fp64") as s:
+        packed = device.DeviceBuffer(N * K)
+        packed.upload(np.asfortranarray(Y).ravel(order="F"))
+        its_p, cv_p = s.solve_residuals_device(packed.address, K, N, tol=1e-10, maxiter=100_000)
+        rp = packed.download(N * K).reshape(K, N)
+        host = np.concatenate([np.concatenate([Y[:, c], gap]) for c in range(K)])
+        spaced = device.DeviceBuffer(ld * K)
+        spaced.upload(host)
+        its_s, cv_s = s.solve_residuals_device(spaced.address, K, ld, tol=1e-10, maxiter=100_000)
+        rs = spaced.download(ld * K).reshape(K, ld)
+    assert all(cv_p) and all(cv_s) and list(its_p) == list(its_s)
+    np.testing.assert_array_equal(rs[:, :N], rp)
+    np.testing.assert_array_equal(rs[:, N:], np.broadcast_to(gap, (K, ld - N)))
+    o = oracle.OracleSolver(p.refs, p.n, p.interactions, p.weights, nthreads=NTHREADS)
+    for c in range(min(2, K)):
+        ro = o.solve_residuals(Y[:, c].copy(), tol=1e-12, maxiter=100_000)[0]
+        assert _rel(rp[c], ro) <= RES_RTOL, (c, _rel(rp[c], ro))
