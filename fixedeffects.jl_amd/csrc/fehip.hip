@@ -1942,7 +1942,7 @@ struct Engine {
     static bool use_pieces(const fehip_solver *h, const Plan *P) { return P->pc.np > 0 && !h->fwl_now; }
     // the projected iteration runs in the piece form (its FE t half)
     static bool use_pieces_proj(const fehip_solver *h, const Plan *P) {
-        return P->pc.np > 0 && h->fwl_now && h->pieces_proj && h->factored;
+        return P->pc.np > 0 && h->fwl_now && h->pieces_proj;
     }
     // ... and stores u scaled (never in the piece forms)
     static bool ut_now(const fehip_solver *h, const Plan *P) {
@@ -1954,7 +1954,7 @@ struct Engine {
     template <int PM>
     static void launch_fused(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         if (h->factored) launch_fused_f<PM, true>(h, P, p);
-        else if constexpr (PM != 4) launch_fused_f<PM, false>(h, P, p);  // (the projected piece form is factored)
+        else launch_fused_f<PM, false>(h, P, p);
     }
     template <int PM, bool FACT>
     static void launch_fused_f(fehip_solver *h, Plan *P, const FusedParams<T> &p) {

@@ -1110,7 +1110,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
     constexpr bool PIECES = PM == 3 || PM == 4;
     constexpr bool UT = stores_scaled_u<K, FACT>() && !PIECES;
     static_assert(!PIECES || GT > 0, "the piece forms have compile-time FE positions");
-    static_assert(PM != 4 || FACT, "the projected piece form is factored");
     static_assert(C == 1 || (GT > 0 && !UT), "the wide lane map: compile-time FE positions, u stored as it is");
     // the wide map's phased accumulators are FP64 in every precision (the
     // exact products of floats summed in doubles): measured on MI355X, LDS
