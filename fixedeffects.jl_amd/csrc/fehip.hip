@@ -1867,10 +1867,15 @@ struct Engine {
     }
 
     // beta = ||u|| and the D'u coefficients (partials of k_db, or of k_fused)
+    // in the iteration (MODE_ITER) it first runs the previous iteration's
+    // stopping test on the vector update's ||x|| partials (k_fin_db)
     static void fin_db(fehip_solver *h, Plan *P, int mode, const int32_t *early_exit, bool fused = false) {
+        const bool it = mode == MODE_ITER;
         hipLaunchKernelGGL((k_fin_db<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream,
                            (const double *)(fused ? P->part_fused : P->part_db),
-                           (int64_t)(fused ? P->fused_blocks : P->db_blocks), mode, P->state, P->ops_dtx, early_exit);
+                           (int64_t)(fused ? P->fused_blocks : P->db_blocks), mode, P->state, P->ops_dtx, early_exit,
+                           it ? (const double *)P->part_upd : (const double *)nullptr, (int64_t)(it ? P->upd_blocks : 0),
+                           P->ops_db, P->all_done);
     }
 
     // alpha = ||v|| and the plane rotations (partials of k_dtx_seg over all
@@ -1885,10 +1890,6 @@ struct Engine {
     }
 
     // ||x|| and the stopping rules
-    static void fin_update(fehip_solver *h, Plan *P) {
-        hipLaunchKernelGGL((k_fin_update<Tr, K>), dim3(1), dim3(kFinBlock), 0, h->stream, (const double *)P->part_upd,
-                           (int64_t)P->upd_blocks, P->state, P->ops_db, P->all_done);
-    }
 
     // coef <- b*coef + a*A'y : segmented tiles, few-level chunks, fix-up.
     // csr_only: the tiles of the CSR FEs only (FE s is done by k_fused).
@@ -2359,7 +2360,6 @@ struct Engine {
         if (P->pc.n2) piece_fix2(h, P, P->ops_dtx, (T *)P->v, P->all_done);
         piece_fin_dtx(h, P);
         update(h, P);
-        fin_update(h, P);
     }
 
     static void dtx(fehip_solver *h, Plan *P, int mode, const ColOps *ops, const T *y, T *coef,
@@ -2400,8 +2400,10 @@ struct Engine {
         hipEvent_t ev[10];
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         // the launches of iteration(), one after another on one stream;
-        // out_ms slots: fused, fin_db, CSR D'u, separate few-level D'u,
-        // fix-up (both parts), fin_dtx, update, fin_update
+        // out_ms slots: fused, fin_db (with the previous iteration's stopping
+        // test), CSR D'u, separate few-level D'u, fix-up (both parts),
+        // fin_dtx, update, and a slot kept for the former stopping-test
+        // launch (0)
         static const int slot_of_gap[9] = {0, 1, 4, 3, 2, 4, 5, 6, 7};
         auto one = [&](bool rec) {
             auto mark = [&](int q) {
@@ -2426,7 +2428,6 @@ struct Engine {
                 mark(7);
                 update(h, P);
                 mark(8);
-                fin_update(h, P);
                 mark(9);
                 return;
             }
@@ -2452,7 +2453,6 @@ struct Engine {
             mark(7);
             update(h, P);
             mark(8);
-            fin_update(h, P);
             mark(9);
         };
         one(false);
@@ -2570,7 +2570,6 @@ struct Engine {
             if (P->pc.n2p) piece_fix2(h, P, P->ops_dtx, (T *)P->v, P->all_done, true);
             piece_fin_dtx(h, P, true);
             update(h, P);
-            fin_update(h, P);
             return;
         }
         if (h->fwl_now) {
@@ -2590,7 +2589,6 @@ struct Engine {
                 fin_dtx(h, P, MODE_ITER, P->all_done, true);
             }
             update(h, P);
-            fin_update(h, P);
             return;
         }
         if (use_pieces(h, P)) {
@@ -2607,7 +2605,6 @@ struct Engine {
             if (P->nitems_c) dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true, false, 2);
             fin_dtx(h, P, MODE_ITER, P->all_done, true, P->nitems_c > 0);
             update(h, P);
-            fin_update(h, P);
             return;
         }
         if (K < kSplitFixK) {
@@ -2616,7 +2613,6 @@ struct Engine {
             dtx_fixup(h, P, MODE_ITER, P->ops_dtx, (T *)P->v, P->all_done, true);
             fin_dtx(h, P, MODE_ITER, P->all_done, true);
             update(h, P);
-            fin_update(h, P);
             return;
         }
         // wide batches: the FE s fix-up needs only the fused pass and beta,
@@ -2631,7 +2627,6 @@ struct Engine {
         (void)hipStreamWaitEvent(h->stream, h->join_ev, 0);
         fin_dtx(h, P, MODE_ITER, P->all_done, true, true);
         update(h, P);
-        fin_update(h, P);
     }
 
     static int ensure_graph(fehip_solver *h, Plan *P) {

@@ -39,6 +39,7 @@ struct LsmrState {
     double atol, btol;
     int64_t iter, maxiter;
     int32_t istop, done, vskip, active;
+    int32_t pending;  // this iteration's stopping test is due (the next beta finalize runs it)
 };
 
 // Coefficients the LSMR bookkeeping hands to the next matvec launch, per

@@ -178,7 +178,7 @@ __device__ __forceinline__ A seg_scan(A v, int psub, int rs) {
 // ---------------------------------------------------------------------------
 // LSMR scalar bookkeeping (src/utils/lsmr.jl), run by one thread per column
 // of the single-workgroup finalize launch that follows each reduction
-// (k_fin_db, k_fin_dtx, k_fin_update).  Contraction is disabled so the scalar
+// (k_fin_db, k_fin_dtx).  Contraction is disabled so the scalar
 // recurrences round like the reference's.
 // ---------------------------------------------------------------------------
 template <typename Tr>
@@ -310,12 +310,20 @@ __device__ void fin_after_dtx(LsmrState<Tr> &s, double vsq, int mode, int c, Col
     if (s.iter > 1) s.minrbar = s.minrbar < rhobarold ? s.minrbar : rhobarold;
     s.condA = (s.maxrbar > rhotemp ? s.maxrbar : rhotemp) / (s.minrbar < rhotemp ? s.minrbar : rhotemp);
     s.normAr = s.zetabar < (Tr)0 ? -s.zetabar : s.zetabar;  // :191
+    // the next fused pass's coefficients, u = A v - alpha u (:104); this
+    // iteration's stopping test (:193-215) needs ||x|| after the vector
+    // update and runs in the next beta finalize (fin_db), after that pass
+    nx_db->a[c] = 1.0;
+    nx_db->b[c] = (double)(-(s.alpha * s.inv_beta));
+    nx_db->on[c] = 1;
+    s.pending = 1;
 }
 
 // after the h/hbar/x update: ||x|| and the stopping rules (:192-215).
 template <typename Tr>
 __device__ void fin_after_update(LsmrState<Tr> &s, double xsq, int c, ColOps *nx_db) {
 #pragma clang fp contract(off)
+    s.pending = 0;
     if (s.done) {
         nx_db->on[c] = 0;
         return;
@@ -338,10 +346,6 @@ __device__ void fin_after_update(LsmrState<Tr> &s, double xsq, int c, ColOps *nx
         s.istop = istop;
         s.done = 1;
         nx_db->on[c] = 0;
-    } else {
-        nx_db->a[c] = 1.0;  // u = A v - alpha u (:104)
-        nx_db->b[c] = (double)(-(s.alpha * s.inv_beta));
-        nx_db->on[c] = 1;
     }
 }
 
@@ -1917,13 +1921,26 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdParams<T> p) {
 // device-scope fences out of the streaming kernels: the kernel boundary
 // publishes the partials.
 // ---------------------------------------------------------------------------
+// MODE_ITER: first the stopping test of the previous iteration (its ||x||
+// partials from the vector update; the fused pass that ran since changes
+// nothing the test or the solution reads), which may finish columns; then
+// beta of this iteration for the columns still running.  Moving the test
+// here saves a launch per iteration.
 template <typename Tr, int K>
 __global__ __launch_bounds__(kFinBlock) void k_fin_db(const double *partials, int64_t nparts, int mode, void *state,
-                                                   ColOps *ops_next_dtx, const int32_t *all_done) {
+                                                   ColOps *ops_next_dtx, const int32_t *all_done,
+                                                   const double *upd_partials, int64_t nupd, ColOps *ops_db,
+                                                   int32_t *all_done_w) {
     if (all_done && *all_done) return;
     __shared__ double red[kFinBlock];
-    const double tot = block_sum_partials<K, kFinBlock>(partials, nparts, nullptr, 0, red);
     LsmrState<Tr> *st = (LsmrState<Tr> *)state;
+    if (mode == MODE_ITER && upd_partials) {
+        const double xsq = block_sum_partials<K, kFinBlock>(upd_partials, nupd, nullptr, 0, red);
+        if (threadIdx.x < K && st[threadIdx.x].pending) fin_after_update<Tr>(st[threadIdx.x], xsq, threadIdx.x, ops_db);
+        __syncthreads();
+        publish_all_done<K, Tr>(st, all_done_w);
+    }
+    const double tot = block_sum_partials<K, kFinBlock>(partials, nparts, nullptr, 0, red);
     if (threadIdx.x < K) fin_after_db<Tr>(st[threadIdx.x], tot, mode, threadIdx.x, ops_next_dtx);
 }
 
@@ -1940,17 +1957,6 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_dtx(const double *p0, int64_t
     if (mode == MODE_INIT) publish_all_done<K, Tr>(st, all_done_w);
 }
 
-template <typename Tr, int K>
-__global__ __launch_bounds__(kFinBlock) void k_fin_update(const double *partials, int64_t nparts, void *state,
-                                                       ColOps *ops_next_db, int32_t *all_done) {
-    if (*all_done) return;
-    __shared__ double red[kFinBlock];
-    const double tot = block_sum_partials<K, kFinBlock>(partials, nparts, nullptr, 0, red);
-    LsmrState<Tr> *st = (LsmrState<Tr> *)state;
-    if (threadIdx.x < K) fin_after_update<Tr>(st[threadIdx.x], tot, threadIdx.x, ops_next_db);
-    __syncthreads();
-    publish_all_done<K, Tr>(st, all_done);
-}
 
 // LSMR set-up of the p-length vectors (:68, :77-78): v /= alpha; h = v; hbar = 0.
 template <typename T, typename Tr, int K>

@@ -52,7 +52,8 @@ typedef struct fehip_mlp fehip_mlp_t;  /* the warm-start network */
 #define FEHIP_FP64 0   /* Float64 everywhere: double_precision = true */
 #define FEHIP_FP32 1   /* Float32 everywhere, LSMR scalars included:
                           double_precision = false (src/utils/lsmr.jl:59) */
-#define FEHIP_MIXED 2  /* Float32 streams with compensated segment sums,
+#define FEHIP_MIXED 2  /* Float32 streams with segment sums beyond Float32
+                          (FP64 sums of exact products, compensated pairs),
                           Float64 LSMR scalars, Float64 refinement sweeps */
 
 /* Builds the solver: uploads refs, builds the per-FE CSR permutations on the
@@ -311,7 +312,8 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out);
  *             with FEHIP_SEG_FIX=0, the whole fix-up
  *   out_ms[5] alpha = ||v|| and plane-rotation finalize
  *   out_ms[6] LSMR vector update (src/utils/lsmr.jl:138-142, :192)
- *   out_ms[7] ||x|| and stopping-rule finalize (one launch) */
+ *   out_ms[7] 0: the ||x|| and stopping-rule step runs inside the next
+ *             iteration's beta finalize (out_ms[1]) */
 int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms);
 
 /* Schedule of the LSMR launches (no reference counterpart; the reference's
