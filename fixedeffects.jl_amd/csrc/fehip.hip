@@ -2327,8 +2327,10 @@ struct Engine {
         const DtxSegParams<T> p = piece_seg_params(h, P, ops, coef, early_exit, proj);
         const FixParams<T> f = piece_fix_params(h, P, ops, coef, early_exit, 1, proj);
         const unsigned nfix = proj ? P->pc.fix1p : P->pc.fix1;
-        constexpr int WC = fused_wide_c<Acc, K, true>();  // the fused pass's factored width
-        if (WC > 1 && fused_cols(h, sizeof(T), K) == WC)
+        // the factored fused pass's width (the piece sums are sqrt(w) .* u sums
+        // in both forms), whenever the wide lane map is on
+        constexpr int WC = fused_wide_c<Acc, K, true>();
+        if (WC > 1 && fused_cols(h, sizeof(T), K) > 1)
             hipLaunchKernelGGL((k_dtx_seg_fix<T, Acc, K, true, true, WC>), dim3(nfix + P->pc.blocks), dim3(kBlock),
                                0, h->stream, p, f, (int)nfix);
         else
