@@ -5,13 +5,17 @@
 // Set-up (fehip_create / fehip_update_weights) builds on the device:
 //   * int32 0-based refs, range-checked;
 //   * the stable CSR bucketing of every FE (radix sort, csr_build.h);
-//   * the internal observation order (stable sort by the FE with most levels);
+//   * the internal observation order (stable sort by the FE with most levels,
+//     then by the largest other CSR FE; the former's levels numbered by their
+//     main level of the latter) and, where it applies, the piece form (runs of
+//     one level pair cut at the fused pass's tiles);
 //   * scale (serial per level, in the reference's summation order) and cache.
 // A solve (fehip_solve_residuals*) runs the whole LSMR loop on the device:
-// each iteration is D*v, D'u (+ fix-up) and the fused vector update; the
-// scalar recurrences run in a single-workgroup finalize launch after each
-// reduction, so the host only replays a captured hipGraph of several
-// iterations and polls one flag.
+// each iteration is the fused pass (D*v, the new u, the raw D'u sums of the
+// sorted and few-level FEs or of the pieces), the D'u launch of the other FEs
+// (+ fix-up) and the fused vector update; the scalar recurrences run in a
+// single-workgroup finalize launch after each reduction, so the host only
+// replays a captured hipGraph of several iterations and polls one flag.
 
 #include <hip/hip_runtime.h>
 

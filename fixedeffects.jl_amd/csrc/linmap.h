@@ -1844,7 +1844,8 @@ __global__ __launch_bounds__(kBlock) void k_dtx_seg_fix(DtxSegParams<T> p, FixPa
 // ---------------------------------------------------------------------------
 // Fused LSMR vector update (src/utils/lsmr.jl:110, :138-142, :192):
 //   v <- v/alpha ; hbar <- c_hbar*hbar + h ; x <- x + c_x*hbar ;
-//   h <- c_h*h + v ; ||x||^2 partials -> stopping rules in the last block.
+//   h <- c_h*h + v ; ||x||^2 partials (the next beta finalize, k_fin_db, runs the
+//   stopping rules on them).
 // Replaces rmul!/axpy!/norm on FixedEffectCoefficients
 // (src/FixedEffectCoefficients.jl:21-55).
 // ---------------------------------------------------------------------------
