@@ -2408,8 +2408,8 @@ struct Engine {
         // the launches of iteration(), one after another on one stream;
         // out_ms slots: fused, fin_db (with the previous iteration's stopping
         // test), CSR D'u, separate few-level D'u, fix-up (both parts),
-        // fin_dtx, update, and a slot kept for the former stopping-test
-        // launch (0)
+        // fin_dtx, update, and an empty event pair (the overhead of the
+        // timing itself, included in every other slot)
         static const int slot_of_gap[9] = {0, 1, 4, 3, 2, 4, 5, 6, 7};
         auto one = [&](bool rec) {
             auto mark = [&](int q) {

@@ -135,6 +135,17 @@ __device__ __forceinline__ float lane_value(float v, int l) {
 }
 __device__ __forceinline__ F32x2 lane_value(F32x2 v, int l) { return F32x2(lane_value(v.hi, l), lane_value(v.lo, l)); }
 
+// Load of element i of set-up data (head words and their prefixes) at a
+// wave-uniform index, through the scalar unit: the constant address space
+// tells the compiler that no kernel of the iteration writes it.  As a vector
+// load it took a vector-memory slot per wave step and, since vector loads
+// complete in order, its move to a scalar register waited for every load the
+// software pipeline had in flight.
+template <typename V>
+__device__ __forceinline__ V ld_const(const V *base, int64_t i) {
+    return ((const __attribute__((address_space(4))) V *)base)[i];
+}
+
 // Segmented inclusive scan over the positions of one wave step.  rs is the
 // psub of the latest segment head at or before this lane (0 if none); lanes
 // only add values of their own run.  K = 1: scan within rows of 16 lanes
@@ -678,9 +689,9 @@ __device__ __forceinline__ void dtx_seg_body(const DtxSegParams<T> &p, int bid, 
                 hwn[q] = hnn[q] = 0;
                 wpn[q] = 0;
                 if (base < end) {  // wave-uniform
-                    hwn[q] = fe.heads[base >> 6];
-                    hnn[q] = fe.heads[(base >> 6) + 1];
-                    if constexpr (!PLEV) wpn[q] = fe.wprefix[base >> 6];
+                    hwn[q] = ld_const(fe.heads, base >> 6);
+                    hnn[q] = ld_const(fe.heads, (base >> 6) + 1);
+                    if constexpr (!PLEV) wpn[q] = ld_const(fe.wprefix, base >> 6);
                 }
             }
         };
@@ -1156,6 +1167,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
         // cache form (FE s and FE t carry no interaction there, so a piece's
         // sum of sqrt(w) .* u serves both, each scaled later)
         const T *swb = ((FACT || PIECES) && p.sqrtw) ? p.sqrtw + start : nullptr;
+        const T *swl = swb ? swb : ub;  // any in-bounds source (offsets below len)
         SegCursorC<SA, C> cur;
 #pragma unroll
         for (int j = 0; j < C; j++) cur.carry[j] = SA(0.f);
@@ -1181,7 +1193,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
 #pragma unroll
             for (int j = 0; j < NW; j++) {
                 const int64_t wi = (b0 >> 6) + j;
-                hw[j] = (lb < len && wi <= ((end - 1) >> 6) + 1) ? p.heads[wi] : 0ull;
+                hw[j] = (lb < len && wi <= ((end - 1) >> 6) + 1) ? ld_const(p.heads, wi) : 0ull;
             }
         };
         // the same words with their head-count prefixes (heads before the word)
@@ -1191,8 +1203,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
             for (int j = 0; j < NW; j++) {
                 const int64_t wi = (b0 >> 6) + j;
                 const bool ok = lb < len && wi <= ((end - 1) >> 6) + 1;
-                hw[j] = ok ? p.heads[wi] : 0ull;
-                wp[j] = ok ? p.wprefix[wi] : 0u;
+                hw[j] = ok ? ld_const(p.heads, wi) : 0ull;
+                wp[j] = ok ? ld_const(p.wprefix, wi) : 0u;
             }
         };
         // FE s level of lane position lb + q*P + psub: the ordinal of its
@@ -1357,7 +1369,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
                     }
                     d.sw[q] = (T)1;
                     if constexpr (FACT || PIECES) {
-                        if (swb) d.sw[q] = ldo(swb, i);  // wave-uniform test
+                        // loaded either way (from u's tile under unit
+                        // weights): a load in a branch makes the compiler's
+                        // waits count it as absent, so the wait for the
+                        // previous step also waited for this step's refs
+                        const T x = ldo(swl, i);
+                        d.sw[q] = swb ? x : (T)1;
                     }
                 }
 #pragma unroll
@@ -1752,8 +1769,8 @@ __device__ __forceinline__ void dtx_seg_body_wide(const DtxSegParams<T> &p, int 
                 lvn[q] = fe.plev[pc];
                 hwn[q] = hnn[q] = 0;
                 if (base < end) {  // wave-uniform
-                    hwn[q] = fe.heads[base >> 6];
-                    hnn[q] = fe.heads[(base >> 6) + 1];
+                    hwn[q] = ld_const(fe.heads, base >> 6);
+                    hnn[q] = ld_const(fe.heads, (base >> 6) + 1);
                 }
             }
         };

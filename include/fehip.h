@@ -312,8 +312,10 @@ int fehip_plan_info(fehip_t *h, int32_t K, int32_t fp32_storage, int64_t *out);
  *             with FEHIP_SEG_FIX=0, the whole fix-up
  *   out_ms[5] alpha = ||v|| and plane-rotation finalize
  *   out_ms[6] LSMR vector update (src/utils/lsmr.jl:138-142, :192)
- *   out_ms[7] 0: the ||x|| and stopping-rule step runs inside the next
- *             iteration's beta finalize (out_ms[1]) */
+ *   out_ms[7] two events recorded back to back with no launch between
+ *             them: the timing's own overhead, included in every slot above
+ *             (the ||x|| and stopping-rule step runs inside the next
+ *             iteration's beta finalize, out_ms[1]) */
 int fehip_time_kernels(fehip_t *h, int32_t K, int32_t reps, double *out_ms);
 
 /* Schedule of the LSMR launches (no reference counterpart; the reference's
