@@ -118,4 +118,43 @@ __device__ double block_sum_partials(const double *p0, int64_t n0, const double 
     return out;
 }
 
+// Two such sums in one pass, A over pa (na parts) and B over pb (nb parts),
+// each in block_sum_partials's order (so each total is bitwise that
+// function's with one span): their loads are in flight together.
+template <int K, int NT>
+__device__ void block_sum_partials_pair(const double *pa, int64_t na, const double *pb, int64_t nb,
+                                        double *red /* 2 * NT doubles */, double &oa, double &ob) {
+    constexpr int R = NT / K;
+    const int t = threadIdx.x, c = t % K, r = t / K;
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t ja = r, jb = r;
+    while (ja + 3 * R < na || jb + 3 * R < nb) {
+        if (ja + 3 * R < na) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) a[k] += pa[(ja + k * R) * K + c];
+            ja += 4 * R;
+        }
+        if (jb + 3 * R < nb) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) b[k] += pb[(jb + k * R) * K + c];
+            jb += 4 * R;
+        }
+    }
+    for (; ja < na; ja += R) a[0] += pa[ja * K + c];
+    for (; jb < nb; jb += R) b[0] += pb[jb * K + c];
+    red[t] = (a[0] + a[1]) + (a[2] + a[3]);
+    red[NT + t] = (b[0] + b[1]) + (b[2] + b[3]);
+    __syncthreads();
+    for (int w = R / 2; w >= 1; w >>= 1) {
+        if (r < w) {
+            red[t] += red[t + w * K];
+            red[NT + t] += red[NT + t + w * K];
+        }
+        __syncthreads();
+    }
+    oa = (t < K) ? red[t] : 0.0;
+    ob = (t < K) ? red[NT + t] : 0.0;
+    __syncthreads();
+}
+
 }  // namespace fehip

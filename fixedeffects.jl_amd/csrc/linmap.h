@@ -1097,6 +1097,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
     constexpr int U = C >= 4 ? (sizeof(T) == 8 ? FUSED_U_C4 : FUSED_U_C4F)
                       : C == 2 ? FUSED_U_C2 : (sizeof(T) == 8 || K > 1) ? FUSED_U64 : 4;
     constexpr uint32_t UP = U * P;
+    // the head-word indexing (jw = q*P/64 per step, NW words per loop step)
+    // needs a loop step to cover whole words or to divide one
+    static_assert(UP % kWave == 0 || kWave % UP == 0, "U * P: whole words or a divisor of one");
     // head words one loop step touches: steps q use word (q*P)/64 and the next
     constexpr int NW = UP / kWave + 1 > 2 ? UP / kWave + 1 : 2;
     constexpr int GA = GT > 0 ? GT : 1;
@@ -1956,15 +1959,18 @@ __global__ __launch_bounds__(kFinBlock) void k_fin_db(const double *partials, in
                                                    const double *upd_partials, int64_t nupd, ColOps *ops_db,
                                                    int32_t *all_done_w) {
     if (all_done && *all_done) return;
-    __shared__ double red[kFinBlock];
+    __shared__ double red[2 * kFinBlock];
     LsmrState<Tr> *st = (LsmrState<Tr> *)state;
+    double tot;
     if (mode == MODE_ITER && upd_partials) {
-        const double xsq = block_sum_partials<K, kFinBlock>(upd_partials, nupd, nullptr, 0, red);
+        double xsq;
+        block_sum_partials_pair<K, kFinBlock>(upd_partials, nupd, partials, nparts, red, xsq, tot);
         if (threadIdx.x < K && st[threadIdx.x].pending) fin_after_update<Tr>(st[threadIdx.x], xsq, threadIdx.x, ops_db);
         __syncthreads();
         publish_all_done<K, Tr>(st, all_done_w);
+    } else {
+        tot = block_sum_partials<K, kFinBlock>(partials, nparts, nullptr, 0, red);
     }
-    const double tot = block_sum_partials<K, kFinBlock>(partials, nparts, nullptr, 0, red);
     if (threadIdx.x < K) fin_after_db<Tr>(st[threadIdx.x], tot, mode, threadIdx.x, ops_next_dtx);
 }
 
