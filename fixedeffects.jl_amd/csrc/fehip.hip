@@ -1973,7 +1973,12 @@ struct Engine {
     static void launch_fused_g(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         const dim3 gr(P->fused_blocks), bl(kBlock);
         const size_t lds = PM == 1 ? 0 : P->fused_lds;  // the sums-only pass has no few-level sums
-        if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, FACT, PM, C>), gr, bl, lds, h->stream, p);
+        // the packed ref stream read once (PK) in the piece forms and the
+        // projected sums-only pass
+        constexpr bool PKF = PM == 1 || PM == 3 || PM == 4;
+        if (h->G == 3 && PKF && p.packed)
+            hipLaunchKernelGGL((k_fused<T, Acc, K, 3, FACT, PM, C, PKF>), gr, bl, lds, h->stream, p);
+        else if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, FACT, PM, C>), gr, bl, lds, h->stream, p);
         else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2, FACT, PM, C>), gr, bl, lds, h->stream, p);
         else if constexpr (C == 1 && PM < 3)  // any FE count (the piece forms have two or three)
             hipLaunchKernelGGL((k_fused<T, Acc, K, 0, FACT, PM>), gr, bl, lds, h->stream, p);

@@ -1083,7 +1083,9 @@ __host__ __device__ constexpr bool stores_scaled_u() {
 // C (columns per lane, wide lane map, seg_step_c): 1, or 2/4 for FP64
 // batches of 16 or more columns with compile-time FE positions, whose
 // few-level sums are phased.
-template <typename T, typename Acc, int K, int GT, bool FACT, int PM = 0, int C = 1>
+// PK (GT = 3, the launch's packed flag set): the packed ref stream is loaded
+// once and split at the level stage.
+template <typename T, typename Acc, int K, int GT, bool FACT, int PM = 0, int C = 1, bool PK = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WPE))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
     extern __shared__ __align__(16) unsigned char fused_smem[];
@@ -1323,7 +1325,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
                 for (int g = 0; g < GT; g++) {
                     const int32_t *rb = p.refs[g] + start;
 #pragma unroll
-                    for (int q = 0; q < U; q++) rr.r[g][q] = g == 0 ? 0 : ldo(rb, loc(lb, q));
+                    for (int q = 0; q < U; q++) rr.r[g][q] = (g == 0 || (PK && g == 2)) ? 0 : ldo(rb, loc(lb, q));
                 }
                 load_words(lb, rr.hw);
             };
@@ -1347,7 +1349,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
                 }
 #pragma unroll
                 for (int q = 0; q < U; q++) rr.r[0][q] = level_s(lb, q, rr.hw, rr.wp, rr.o[q]);
-                if constexpr (GT == 3) {
+                if constexpr (GT == 3 && PK) {
+                    // one packed stream: ref_1 in the low 26 bits, ref_2 above
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        const uint32_t raw = (uint32_t)rr.r[1][q];
+                        rr.r[1][q] = (int32_t)(raw & 0x3FFFFFFu);
+                        rr.r[2][q] = (int32_t)(raw >> 26);
+                    }
+                } else if constexpr (GT == 3) {
                     // packed pair: branch-free unpack (the same loads either way)
                     const uint32_t mask = p.packed ? 0x3FFFFFFu : 0xFFFFFFFFu;
                     const uint32_t shift = p.packed ? 26u : 0u;
