@@ -914,6 +914,17 @@ static int fused_cols(const fehip_solver *h, size_t ts, int K) {
     return ts != 8 ? FUSED_WIDE_C32 : h->factored ? FUSED_WIDE_C : FUSED_WIDE_C_CACHE;
 }
 
+static size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+// levels of the few-level FE whose gathered vector the fused pass's packed
+// forms stage in LDS after acc bytes of accumulators (k_fused's nvl, PK):
+// the packed pair present and everything within 64 KB, else 0 (those forms
+// then read the packed stream twice and gather the vector from memory)
+static int32_t staged_levels(const fehip_solver *h, size_t ts, int K, size_t acc) {
+    if (h->G != 3 || !h->packed || h->pack_d < 0) return 0;
+    const size_t b = (size_t)h->fe[h->pack_d].n * K * ts;
+    return align16(acc) + b <= 65536 ? (int32_t)h->fe[h->pack_d].n : 0;
+}
+
 static int64_t tile_size(int64_t N, int64_t ntiles) {
     const int64_t t = (N + ntiles - 1) / ntiles;
     return std::max<int64_t>(256, (t + 255) / 256 * 256);
@@ -993,6 +1004,7 @@ struct Plan {
     std::vector<int32_t> slot0;
     unsigned fused_blocks = 0;
     size_t fused_lds = 0;
+    int32_t nvl = 0;  // levels of the vector the packed fused forms stage in LDS
     double *part_fused = nullptr, *wsum = nullptr;
     void *vt = nullptr;  // p x K, scale .* v (factored fused pass)
     uint8_t *sdirect = nullptr;
@@ -1140,7 +1152,9 @@ static int get_plan(fehip_solver *h, int K, size_t ts, Plan **out, int64_t waves
     if (P->fused_dense) {
         chunk = P->R;
         P->dense_wpb = kWavesPerBlock;
-        P->fused_lds = (size_t)P->nslot * per_slot_fused * kWavesPerBlock;
+        const size_t acc = (size_t)P->nslot * per_slot_fused * kWavesPerBlock;
+        P->nvl = staged_levels(h, ts, K, acc);
+        P->fused_lds = P->nvl ? align16(acc) + (size_t)P->nvl * K * ts : acc;
     } else {
         chunk = (N + 4095) / 4096;
         chunk = std::max<int64_t>(chunk, 16 * Pq);
@@ -1450,9 +1464,10 @@ static size_t fused_lds_bytes(const fehip_solver *h, size_t ts, int K) {
     for (int g = 0; g < h->G; g++)
         if (h->fe[g].kind == KIND_DENSE) maxlev = std::max<int64_t>(maxlev, h->fe[g].n);
     const size_t b = (size_t)nslot * per_slot * kWavesPerBlock;
-    return (b <= 65536 && (size_t)maxlev * per_slot * kWavesPerBlock <= 65536 && fuse_dense_on(h))
-               ? (size_t)nslot * per_slot_fused * kWavesPerBlock
-               : 0;
+    if (!(b <= 65536 && (size_t)maxlev * per_slot * kWavesPerBlock <= 65536 && fuse_dense_on(h))) return 0;
+    const size_t acc = (size_t)nslot * per_slot_fused * kWavesPerBlock;
+    const int32_t nvl = staged_levels(h, ts, K, acc);
+    return nvl ? align16(acc) + (size_t)nvl * K * ts : acc;
 }
 
 // resident waves of a kernel on this device (all CUs, kBlock threads)
@@ -1972,11 +1987,12 @@ struct Engine {
     template <int PM, bool FACT, int C>
     static void launch_fused_g(fehip_solver *h, Plan *P, const FusedParams<T> &p) {
         const dim3 gr(P->fused_blocks), bl(kBlock);
-        const size_t lds = PM == 1 ? 0 : P->fused_lds;  // the sums-only pass has no few-level sums
-        // the packed ref stream read once (PK) in the piece forms and the
-        // projected sums-only pass
+        // the sums-only pass has no few-level sums (only the staged vector)
+        const size_t lds = PM == 1 ? (size_t)p.nvl * K * sizeof(T) : P->fused_lds;
+        // the packed ref stream read once and the few-level vector staged in
+        // LDS (PK) in the piece forms and the projected sums-only pass
         constexpr bool PKF = PM == 1 || PM == 3 || PM == 4;
-        if (h->G == 3 && PKF && p.packed)
+        if (h->G == 3 && PKF && p.packed && p.nvl > 0)
             hipLaunchKernelGGL((k_fused<T, Acc, K, 3, FACT, PM, C, PKF>), gr, bl, lds, h->stream, p);
         else if (h->G == 3) hipLaunchKernelGGL((k_fused<T, Acc, K, 3, FACT, PM, C>), gr, bl, lds, h->stream, p);
         else if (h->G == 2) hipLaunchKernelGGL((k_fused<T, Acc, K, 2, FACT, PM, C>), gr, bl, lds, h->stream, p);
@@ -2017,6 +2033,7 @@ struct Engine {
         if (dense_in && h->G == 3 && h->packed && kord[1] == h->pack_g && kord[2] == h->pack_d) {
             p.refs[1] = p.refs[2] = h->packed;
             p.packed = 1;
+            p.nvl = P->nvl;
         }
         if (dense_in) {
             p.ndense = P->ndense;

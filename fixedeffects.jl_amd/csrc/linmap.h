@@ -984,7 +984,11 @@ __global__ __launch_bounds__(kBlock) void k_dtx_dense(DtxDenseParams<T> p) {
 template <typename T>
 struct FusedParams {
     int64_t N, R;
-    int32_t G, s, ntiles, ndense, nslot, pad;
+    int32_t G, s, ntiles, ndense, nslot;
+    // PK kernels: levels of the few-level FE at position GT-1 whose gathered
+    // vector gsrc[GT-1] (nvl x K) the workgroup stages in LDS after the
+    // accumulators
+    int32_t nvl;
     const int32_t *refs[kMaxG];
     const T *cache[kMaxG];  // internal order
     int64_t coef_off[kMaxG];
@@ -1083,8 +1087,9 @@ __host__ __device__ constexpr bool stores_scaled_u() {
 // C (columns per lane, wide lane map, seg_step_c): 1, or 2/4 for FP64
 // batches of 16 or more columns with compile-time FE positions, whose
 // few-level sums are phased.
-// PK (GT = 3, the launch's packed flag set): the packed ref stream is loaded
-// once and split at the level stage.
+// PK (GT = 3, the launch's packed flag set, nvl > 0): the packed ref stream
+// is loaded once and split at the level stage, and the few-level FE's
+// gathered vector is read from LDS.
 template <typename T, typename Acc, int K, int GT, bool FACT, int PM = 0, int C = 1, bool PK = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WPE))) void k_fused(FusedParams<T> p) {
     if (*p.all_done) return;
@@ -1146,6 +1151,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
         for (int e = lane; e < p.nslot * K; e += kWave) acc[e] = DA(0.f);
     } else {
         for (int k = 0; k < p.nslot; k++) acc[k * kWave + lane] = DA(0.f);
+    }
+    // PK: the few-level FE's vector (nvl x K) staged after the accumulators;
+    // its gathers become LDS reads.  Every wave of the workgroup reaches the
+    // barrier (before the tile test).
+    T *vl = reinterpret_cast<T *>(
+        fused_smem + (((size_t)kWavesPerBlock * p.nslot * (PHASED ? K : kWave) * sizeof(DA) + 15) & ~(size_t)15));
+    if constexpr (PK) {
+        const T *src = p.gsrc[GA - 1];
+        for (int e = threadIdx.x; e < p.nvl * K; e += kBlock) vl[e] = src[e];
+        __syncthreads();
     }
     auto dense_add = [&](const int32_t (&sl)[U], const DA (&x)[U][C]) {
         if constexpr (C > 1) {
@@ -1402,6 +1417,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED_WP
 #pragma unroll
                     for (int q = 0; q < U; q++) {
                         if (PM == 1 && g == 0) d.gv[g][q].v[0] = ldo(cf, (uint32_t)rr.r[g][q]);
+                        else if (PK && g == GA - 1)
+                            d.gv[g][q] = *reinterpret_cast<const VT *>(vl + (uint32_t)rr.r[g][q] * K + c0);
                         else d.gv[g][q] = ldv<C>(cf, (uint32_t)rr.r[g][q] * K + c0);
                         if constexpr (!FACT) d.cv[g][q] = ldo(p.cache[g] + start, loc(lb, q));
                     }
