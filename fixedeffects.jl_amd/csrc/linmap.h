@@ -644,7 +644,10 @@ __device__ __forceinline__ int32_t seg_level_of(uint32_t wp, const int32_t *seg_
 template <typename T, typename Acc, int K, bool RAW, bool PLEV = false>
 __device__ __forceinline__ void dtx_seg_body(const DtxSegParams<T> &p, int bid, int nblk) {
     constexpr int P = kWave / K;
-    constexpr int U = 4;  // wave steps whose loads are issued together
+    // wave steps whose loads are issued together; the piece launch (PLEV)
+    // takes two (measured at K = 1: 26.1 / 25.3 us against 27.3 / 28.5 with
+    // four, 35 with eight)
+    constexpr int U = PLEV ? 2 : 4;
     const int lane = threadIdx.x & (kWave - 1);
     const int psub = lane / K, c = lane % K;
     const int on = p.ops->on[c];
