@@ -51,7 +51,7 @@ def log(*a):
 PMC_TRAFFIC = {
     ("worker_firm_1e7", "fp64", 1, "csr"): ({"k_fused": 324.1e6, "k_dtx_csr": 311.1e6, "k_update": 88.1e6},
                                             "profiles/worker_firm_1e7_fp64_k1_csr.md"),
-    ("worker_firm_1e7", "fp64", 1, "pieces"): ({"k_fused": 332.5e6, "k_dtx_csr": 102.6e6, "k_update": 88.1e6},
+    ("worker_firm_1e7", "fp64", 1, "pieces"): ({"k_fused": 332.4e6, "k_dtx_csr": 103.3e6, "k_update": 88.1e6},
                                                "profiles/worker_firm_1e7_fp64_k1.md"),
     ("worker_firm_1e7_k32", "fp64", 32, "pieces"): ({"k_fused": 6065.9e6, "k_dtx_csr": 1379.6e6,
                                                      "k_update": 2554.9e6}, "profiles/worker_firm_1e7_fp64_k32.md"),
